@@ -1,0 +1,115 @@
+/* lucivant-sim MI355X engine: the C ABI of liblsim.so.
+ *
+ * This is the drop-in boundary underneath the Python engine classes `Statevector` and
+ * `DensityMatrix` (lucivant-sim 0.3.0, src/lucivant_sim/engines.py:138-264) and the drivers that
+ * call them (src/lucivant_sim/simulator.py:229-317, 404-527). The reference has no FFI of its
+ * own: each entry point below names the NumPy routine of the reference whose arithmetic it
+ * replaces, and `lucivant_sim/_native.py` binds them with ctypes.
+ *
+ * Conventions for every entry point:
+ *  - the return value is a status: LSIM_OK (0) or an LSIM_E* code; lsim_last_error() then holds
+ *    a message for the calling thread. No C++ exception crosses this boundary.
+ *  - `states` is device memory owned by the caller: `batch` states of 2^nbits amplitudes each,
+ *    stored back to back; amplitude index bit q is qubit q (little-endian, engines.py:8-9).
+ *    `dtype` is LSIM_C128 (interleaved float64 re/im) or LSIM_C64 (interleaved float32).
+ *  - a density matrix of n qubits is passed as a state of nbits = 2n: flat index row*2^n + col,
+ *    so vector bit n+q is the row bit and bit q the column bit of qubit q (engines.py:215-231).
+ *  - `targets` is a HOST array of k bit positions; the first listed is the most significant bit
+ *    of the operator's matrix index (gates.py:1-7, engines.py:83-89).
+ *  - matrices, diagonals, selectors, outputs and workspaces are DEVICE pointers.
+ *  - `stream` is a hipStream_t (NULL = the null stream); every call only enqueues work.
+ */
+#ifndef LSIM_H
+#define LSIM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSIM_OK 0
+#define LSIM_EINVAL 1 /* bad argument: sizes, bit positions, pointers */
+#define LSIM_EHIP 2   /* the HIP runtime reported an error */
+
+#define LSIM_C128 0
+#define LSIM_C64 1
+
+#define LSIM_MAX_K 5        /* widest dense operator: 32 x 32 */
+#define LSIM_MAX_TERMS 64   /* Pauli strings per expectation call */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lsim_last_error(void);
+
+/* ABI version of this library (bumped on any signature change). */
+int lsim_abi_version(void);
+
+/* Device 0 facts: compute units, HBM bytes, gfx architecture name. */
+int lsim_device_info(int* cu_count, long long* hbm_bytes, char* arch, int arch_len);
+
+/* |index> in every state: replaces Statevector.__init__ / DensityMatrix.__init__
+ * (engines.py:139-148, 216-225; for a density matrix pass nbits=2n, index=0). */
+int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,
+                      void* stream);
+
+/* psi <- M psi on k <= 5 target bits, in place: replaces _apply (engines.py:38-42) and
+ * _apply_slices (engines.py:45-119), and for a density matrix DensityMatrix.apply and
+ * apply_channel (engines.py:233-250) given the superoperator U (x) conj(U) or sum K (x) conj(K)
+ * on targets [rows..., cols...].
+ * `mats`: M matrices of 2^k x 2^k, row-major, in `dtype`. `sel`: NULL (every state uses
+ * mats[0]) or `batch` ints, state b uses mats[sel[b]] and is left untouched if sel[b] < 0. */
+int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int k,
+                        const int* targets, const void* mats, const int* sel, void* stream);
+
+/* psi <- diag(d) psi on k <= 5 target bits: the diagonal case of _apply_slices
+ * (engines.py:91-104; rz, s, t, cz, cp). `diags`: M diagonals of 2^k entries; `sel` as above. */
+int lsim_sv_apply_diag(void* states, int dtype, int nbits, long long batch, int k,
+                       const int* targets, const void* diags, const int* sel, void* stream);
+
+/* psi <- c psi in every state: Statevector.scale (engines.py:156-157). */
+int lsim_sv_scale(void* states, int dtype, int nbits, long long batch, double re, double im,
+                  void* stream);
+
+/* y <- y + (re + i im) x over `batch` states: the accumulation of DensityMatrix.apply_channel's
+ * Kraus sum (engines.py:245-250) for channels too wide for one superoperator pass. */
+int lsim_sv_axpy(void* y, const void* x, int dtype, int nbits, long long batch, double re,
+                 double im, void* stream);
+
+/* probs[b][i] = |psi_b[i]|^2 as float64: Statevector.probabilities (engines.py:207-209). */
+int lsim_sv_probabilities(const void* states, int dtype, int nbits, long long batch,
+                          double* probs, void* stream);
+
+/* Bytes of device workspace the reductions below need for these sizes. `width` is k for
+ * lsim_sv_reduced_dm and the number of terms for the expectation calls. */
+size_t lsim_reduce_workspace_bytes(int nbits, long long batch, int width);
+
+/* Reduced density matrix of each state on k <= 2 target bits, out[b] = 2^k x 2^k complex128,
+ * row-major, index bits as for matrices. Deterministic (fixed summation order). It yields
+ * probability_of_one (engines.py:185-187, the [1][1] entry for k = 1) and every Kraus branch
+ * weight ||K psi||^2 = Tr(K rho_T K^dagger) of Statevector.apply_channel (engines.py:171-183)
+ * from one read of the state. */
+int lsim_sv_reduced_dm(const void* states, int dtype, int nbits, long long batch, int k,
+                       const int* targets, double* out, double* workspace, void* stream);
+
+/* out[b][t] = <psi_b| P_t |psi_b> for nterms <= 64 Pauli strings P_t given by HOST bit masks
+ * (x: qubits with X or Y, z: qubits with Z or Y): the per-term probe loop of expectation
+ * (simulator.py:517-527), P|j> = i^{nY} (-1)^{popcount(j & z)} |j ^ x>. */
+int lsim_sv_pauli_expect(const void* states, int dtype, int nbits, long long batch, int nterms,
+                         const unsigned long long* xmask, const unsigned long long* zmask,
+                         double* out, double* workspace, void* stream);
+
+/* probs[b][i] = max(Re rho_b[i][i], 0) for n-qubit density matrices: DensityMatrix.probabilities
+ * (engines.py:257-260). */
+int lsim_dm_probabilities(const void* rho, int dtype, int n, long long batch, double* probs,
+                          void* stream);
+
+/* out[b][t] = Re Tr(P_t rho_b): the noisy branch of expectation (simulator.py:505-516, 530-533). */
+int lsim_dm_pauli_expect(const void* rho, int dtype, int n, long long batch, int nterms,
+                         const unsigned long long* xmask, const unsigned long long* zmask,
+                         double* out, double* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSIM_H */

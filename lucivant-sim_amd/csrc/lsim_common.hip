@@ -1,0 +1,82 @@
+// Error reporting, device facts and target-offset tables shared by every launcher.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "lsim_internal.h"
+
+namespace lsim {
+
+static thread_local char g_error[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_error, sizeof g_error, fmt, ap);
+  va_end(ap);
+}
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_error, sizeof g_error, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LSIM_EHIP, "%s: %s", what, hipGetErrorString(e));
+  return LSIM_OK;
+}
+
+int make_targets(int nbits, int k, const int* targets, Targets* out) {
+  LSIM_REQUIRE(k >= 1 && k <= kMaxK, "k must be between 1 and %d, not %d", kMaxK, k);
+  LSIM_REQUIRE(k <= nbits, "a %d-bit operator does not fit a %d-bit state", k, nbits);
+  LSIM_REQUIRE(targets != nullptr, "targets is NULL");
+  uint32_t mask = 0;
+  for (int i = 0; i < k; ++i) {
+    LSIM_REQUIRE(targets[i] >= 0 && targets[i] < nbits, "target bit %d is outside 0..%d",
+                 targets[i], nbits - 1);
+    LSIM_REQUIRE(!(mask & (1u << targets[i])), "target bit %d is given twice", targets[i]);
+    mask |= 1u << targets[i];
+  }
+  int j = 0;
+  for (int b = 0; b < nbits; ++b)
+    if (mask & (1u << b)) out->sorted[j++] = (uint32_t)b;
+  for (; j < kMaxK; ++j) out->sorted[j] = 0;
+  for (int i = 0; i < kMaxK; ++i) out->listed[i] = i < k ? (uint32_t)targets[i] : 0u;
+  for (int c = 0; c < (1 << kMaxK); ++c) {
+    uint32_t off = 0;
+    if (c < (1 << k))
+      for (int i = 0; i < k; ++i)
+        if ((c >> (k - 1 - i)) & 1) off |= 1u << targets[i];
+    out->offs[c] = off;
+  }
+  return LSIM_OK;
+}
+
+}  // namespace lsim
+
+extern "C" {
+
+const char* lsim_last_error(void) { return lsim::g_error; }
+
+int lsim_abi_version(void) { return 1; }
+
+int lsim_device_info(int* cu_count, long long* hbm_bytes, char* arch, int arch_len) {
+  hipDeviceProp_t p;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&p, dev);
+  if (e != hipSuccess) return lsim::fail(LSIM_EHIP, "device info: %s", hipGetErrorString(e));
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (long long)p.totalGlobalMem;
+  if (arch && arch_len > 0) {
+    strncpy(arch, p.gcnArchName, (size_t)arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  return LSIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Shared definitions for the lucivant-sim MI355X (gfx950) kernels and their C-ABI launchers.
+//
+// Layout of every state buffer: `batch` states of 2^nbits amplitudes each, back to back, amplitude
+// index bit q = qubit q (little-endian, as the reference engines flatten their tensors). A density
+// matrix of n qubits is a "state" of 2n bits: flat index = row * 2^n + col, so vector bit n+q is
+// the row bit of qubit q and vector bit q its column bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lsim.h"
+
+namespace lsim {
+
+// ---------------------------------------------------------------------------- error reporting
+
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+int check_launch(const char* what);
+
+#define LSIM_REQUIRE(cond, ...)                             \
+  do {                                                      \
+    if (!(cond)) return ::lsim::fail(LSIM_EINVAL, __VA_ARGS__); \
+  } while (0)
+
+// ------------------------------------------------------------------------------ value types
+
+template <typename R>
+struct Cx;
+template <>
+struct Cx<double> {
+  using V = double2;
+};
+template <>
+struct Cx<float> {
+  using V = float2;
+};
+
+constexpr int kMaxK = 5;              // widest fused dense operator (32 x 32)
+constexpr int kMaxBits = 30;          // widest vector handled (2^30 amplitudes per state)
+constexpr int kBlock = 256;           // threads per workgroup for the streaming kernels
+constexpr int kMaxGridY = 65535;      // batch entries per launch (grid y)
+
+// Targets of a k-qubit operator: `offs[c]` is the amplitude offset of matrix index c (the first
+// listed target is the matrix's most significant bit), `sorted` the target bits ascending, used
+// to spread a group number over the non-target bits.
+struct Targets {
+  uint32_t sorted[kMaxK];
+  uint32_t listed[kMaxK];
+  uint32_t offs[1 << kMaxK];
+};
+
+int make_targets(int nbits, int k, const int* targets, Targets* out);
+
+__device__ __forceinline__ uint32_t spread(uint32_t g, const uint32_t* sorted, int k) {
+#pragma unroll
+  for (int j = 0; j < kMaxK; ++j) {
+    if (j < k) {
+      const uint32_t s = sorted[j];
+      g = ((g >> s) << (s + 1)) | (g & ((1u << s) - 1u));
+    }
+  }
+  return g;
+}
+
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+// Wavefront (64 lanes) then workgroup sum of a double; every thread returns the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline size_t dtype_bytes(int dtype) { return dtype == LSIM_C64 ? 8 : 16; }
+
+}  // namespace lsim

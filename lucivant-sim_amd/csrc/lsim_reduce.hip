@@ -1,0 +1,348 @@
+// Reductions over states: reduced density matrices on 1-2 target bits (measurement
+// probabilities and Kraus branch weights), Pauli-string expectations for statevectors and
+// density matrices, and density-matrix diagonals.
+//
+// Every reduction is deterministic: workgroup b of a state sums a fixed, strided subset of the
+// state with wavefront shuffles and LDS, writes one partial per value, and a second kernel adds
+// the partials of each state in a fixed order. Repeating a call gives the same bits, which keeps
+// seeded trajectory decisions reproducible run to run. Accumulation is float64 for both dtypes.
+#include "lsim_internal.h"
+
+namespace lsim {
+
+// Workgroups per state for a reduction over `items` work items (about 8 items per thread).
+static uint32_t reduce_blocks(uint64_t items) {
+  uint64_t nb = items / (kBlock * 8);
+  if (nb < 1) nb = 1;
+  if (nb > 256) nb = 256;
+  return (uint32_t)nb;
+}
+
+// Sum NV per-thread values across the workgroup; thread 0 writes them to out[0..NV).
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&acc)[NV], double* __restrict__ out) {
+  __shared__ double part[kBlock / 64][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double s = wave_sum(acc[v]);
+    if (lane == 0) part[wave][v] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += part[w][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// Reduced density matrix on K target bits. Values per state: D diagonal reals, then for each
+// pair r < c (row-major) the real and imaginary parts of sum a_r conj(a_c).
+template <int K, typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_rdm(const typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t groups,
+          uint32_t nb, Targets t, double* __restrict__ partial, uint32_t batch_base) {
+  using V = typename Cx<R>::V;
+  constexpr int D = 1 << K;
+  constexpr int NV = D * D;
+  const uint32_t b = batch_base + blockIdx.y;
+  const V* __restrict__ st = states + ((size_t)b << nbits);
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0;
+  for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < groups; g += nb * kBlock) {
+    const uint32_t base = spread(g, t.sorted, K);
+    double x[D], y[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const V a = st[base + t.offs[c]];
+      x[c] = (double)a.x;
+      y[c] = (double)a.y;
+    }
+    int v = D;
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      acc[r] += x[r] * x[r] + y[r] * y[r];
+#pragma unroll
+      for (int c = r + 1; c < D; ++c) {
+        acc[v++] += x[r] * x[c] + y[r] * y[c];
+        acc[v++] += y[r] * x[c] - x[r] * y[c];
+      }
+    }
+  }
+  block_sum_store<NV>(acc, partial + ((size_t)b * nb + blockIdx.x) * NV);
+}
+
+// Sum the partials of each state and expand to a full Hermitian 2^K x 2^K complex matrix.
+template <int K>
+__global__ void k_rdm_finish(const double* __restrict__ partial, uint32_t nb, uint64_t batch,
+                             double2* __restrict__ out) {
+  constexpr int D = 1 << K;
+  constexpr int NV = D * D;
+  const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= batch * NV) return;
+  const uint64_t b = id / NV;
+  const int rc = (int)(id % NV), r = rc / D, c = rc % D;
+  const double* p = partial + b * nb * NV;
+  double2 v;
+  if (r == c) {
+    double s = 0;
+    for (uint32_t j = 0; j < nb; ++j) s += p[(size_t)j * NV + r];
+    v.x = s;
+    v.y = 0;
+  } else {
+    const int lo = r < c ? r : c, hi = r < c ? c : r;
+    // position of pair (lo, hi) among the pairs r < c in row-major order
+    const int pair = lo * D - lo * (lo + 1) / 2 + (hi - lo - 1);
+    double sx = 0, sy = 0;
+    for (uint32_t j = 0; j < nb; ++j) {
+      sx += p[(size_t)j * NV + D + 2 * pair];
+      sy += p[(size_t)j * NV + D + 2 * pair + 1];
+    }
+    v.x = sx;
+    v.y = r < c ? sy : -sy;
+  }
+  out[b * NV + rc] = v;
+}
+
+struct PauliTerms {
+  uint64_t x[LSIM_MAX_TERMS];
+  uint64_t z[LSIM_MAX_TERMS];
+};
+
+__device__ __forceinline__ double re_times_ipow(int w, double zx, double zy) {
+  // Re(i^w * z)
+  switch (w & 3) {
+    case 0: return zx;
+    case 1: return -zy;
+    case 2: return -zx;
+    default: return zy;
+  }
+}
+
+// <psi|P|psi> = sum_j conj(psi[j^x]) i^nY (-1)^popcount(j&z) psi[j]. With x != 0 the terms of j
+// and j^x are complex conjugates, so the sum runs over j with the top bit of x clear and adds
+// twice the real part: each amplitude is read once per string.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_pauli_sv(const typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t nb,
+               int nterms, PauliTerms terms, double* __restrict__ partial, uint32_t batch_base) {
+  using V = typename Cx<R>::V;
+  const uint32_t t = blockIdx.y % nterms;
+  const uint32_t b = batch_base + blockIdx.y / nterms;
+  const uint64_t X = terms.x[t], Z = terms.z[t];
+  const int w = __popcll(X & Z) & 3;
+  const V* __restrict__ st = states + ((size_t)b << nbits);
+  double acc[1] = {0};
+  if (X == 0) {
+    const uint64_t size = 1ull << nbits;
+    for (uint64_t j = blockIdx.x * kBlock + threadIdx.x; j < size; j += (uint64_t)nb * kBlock) {
+      const V a = st[j];
+      const double p = (double)a.x * a.x + (double)a.y * a.y;
+      acc[0] += (__popcll(j & Z) & 1) ? -p : p;
+    }
+  } else {
+    const uint32_t h = 63 - __clzll(X);
+    const uint64_t half = 1ull << (nbits - 1);
+    for (uint64_t g = blockIdx.x * kBlock + threadIdx.x; g < half; g += (uint64_t)nb * kBlock) {
+      const uint64_t j = ((g >> h) << (h + 1)) | (g & ((1ull << h) - 1));
+      const V a = st[j], p = st[j ^ X];
+      // z = conj(p) * a
+      const double zx = (double)p.x * a.x + (double)p.y * a.y;
+      const double zy = (double)p.x * a.y - (double)p.y * a.x;
+      const double v = 2.0 * re_times_ipow(w, zx, zy);
+      acc[0] += (__popcll(j & Z) & 1) ? -v : v;
+    }
+  }
+  block_sum_store<1>(acc, partial + ((size_t)b * nterms + t) * nb + blockIdx.x);
+}
+
+// Tr(P rho) = sum_m i^nY (-1)^popcount(m&z) rho[m][m^x] (real part) for an n-qubit rho.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_pauli_dm(const typename Cx<R>::V* __restrict__ rho, uint32_t n, uint32_t nb, int nterms,
+               PauliTerms terms, double* __restrict__ partial, uint32_t batch_base) {
+  using V = typename Cx<R>::V;
+  const uint32_t t = blockIdx.y % nterms;
+  const uint32_t b = batch_base + blockIdx.y / nterms;
+  const uint64_t X = terms.x[t], Z = terms.z[t];
+  const int w = __popcll(X & Z) & 3;
+  const V* __restrict__ m0 = rho + ((size_t)b << (2 * n));
+  const uint64_t dim = 1ull << n;
+  double acc[1] = {0};
+  for (uint64_t m = blockIdx.x * kBlock + threadIdx.x; m < dim; m += (uint64_t)nb * kBlock) {
+    const V e = m0[m * dim + (m ^ X)];
+    const double v = re_times_ipow(w, (double)e.x, (double)e.y);
+    acc[0] += (__popcll(m & Z) & 1) ? -v : v;
+  }
+  block_sum_store<1>(acc, partial + ((size_t)b * nterms + t) * nb + blockIdx.x);
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, uint32_t nb, uint64_t count,
+                               double* __restrict__ out) {
+  const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= count) return;
+  double s = 0;
+  for (uint32_t j = 0; j < nb; ++j) s += partial[id * nb + j];
+  out[id] = s;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_dm_probs(const typename Cx<R>::V* __restrict__ rho, uint32_t n, uint64_t batch,
+               double* __restrict__ probs) {
+  const uint64_t dim = 1ull << n;
+  const uint64_t id = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (id >= batch * dim) return;
+  const uint64_t b = id / dim, i = id % dim;
+  const double v = (double)rho[(b << (2 * n)) + i * dim + i].x;
+  probs[id] = v > 0.0 ? v : 0.0;
+}
+
+static int check_common(const void* states, int dtype, int nbits, long long batch) {
+  LSIM_REQUIRE(states != nullptr, "the state pointer is NULL");
+  LSIM_REQUIRE(dtype == LSIM_C128 || dtype == LSIM_C64, "dtype must be LSIM_C128 or LSIM_C64");
+  LSIM_REQUIRE(nbits >= 1 && nbits <= kMaxBits, "nbits must be between 1 and %d, not %d", kMaxBits,
+               nbits);
+  LSIM_REQUIRE(batch >= 1 && batch <= (1ll << 31), "batch must be between 1 and 2^31");
+  return LSIM_OK;
+}
+
+static int load_terms(int nterms, const unsigned long long* xmask, const unsigned long long* zmask,
+                      int nbits, PauliTerms* terms) {
+  LSIM_REQUIRE(nterms >= 1 && nterms <= LSIM_MAX_TERMS, "nterms must be between 1 and %d",
+               LSIM_MAX_TERMS);
+  LSIM_REQUIRE(xmask != nullptr && zmask != nullptr, "the Pauli masks are NULL");
+  const unsigned long long valid = nbits >= 64 ? ~0ull : ((1ull << nbits) - 1);
+  for (int t = 0; t < LSIM_MAX_TERMS; ++t) {
+    terms->x[t] = t < nterms ? xmask[t] : 0;
+    terms->z[t] = t < nterms ? zmask[t] : 0;
+    LSIM_REQUIRE((terms->x[t] | terms->z[t]) <= valid, "Pauli term %d acts on a qubit beyond %d",
+                 t, nbits);
+  }
+  return LSIM_OK;
+}
+
+}  // namespace lsim
+
+using namespace lsim;
+
+extern "C" {
+
+size_t lsim_reduce_workspace_bytes(int nbits, long long batch, int width) {
+  if (nbits < 1 || nbits > kMaxBits || batch < 1 || width < 1) return 0;
+  // the widest user: a reduced density matrix on 2 bits (16 values per workgroup), or one value
+  // per Pauli term
+  const uint64_t nb = reduce_blocks(1ull << (nbits - 1));
+  const uint64_t per = width <= 2 ? (uint64_t)(1 << (2 * width)) : (uint64_t)width;
+  const uint64_t per_rdm = 16;
+  return (size_t)(batch * nb * (per > per_rdm ? per : per_rdm) * sizeof(double));
+}
+
+int lsim_sv_reduced_dm(const void* states, int dtype, int nbits, long long batch, int k,
+                       const int* targets, double* out, double* workspace, void* stream) {
+  if (int rc = check_common(states, dtype, nbits, batch)) return rc;
+  LSIM_REQUIRE(k == 1 || k == 2, "reduced density matrices are on 1 or 2 bits, not %d", k);
+  LSIM_REQUIRE(out != nullptr && workspace != nullptr, "output or workspace pointer is NULL");
+  Targets t;
+  if (int rc = make_targets(nbits, k, targets, &t)) return rc;
+  const uint32_t groups = 1u << (nbits - k);
+  const uint32_t nb = reduce_blocks(groups);
+  hipStream_t s = as_stream(stream);
+  for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
+    dim3 grid(nb, gy);
+    if (dtype == LSIM_C128) {
+      if (k == 1)
+        k_rdm<1, double><<<grid, kBlock, 0, s>>>((const double2*)states, nbits, groups, nb, t,
+                                                 workspace, (uint32_t)b0);
+      else
+        k_rdm<2, double><<<grid, kBlock, 0, s>>>((const double2*)states, nbits, groups, nb, t,
+                                                 workspace, (uint32_t)b0);
+    } else {
+      if (k == 1)
+        k_rdm<1, float><<<grid, kBlock, 0, s>>>((const float2*)states, nbits, groups, nb, t,
+                                                workspace, (uint32_t)b0);
+      else
+        k_rdm<2, float><<<grid, kBlock, 0, s>>>((const float2*)states, nbits, groups, nb, t,
+                                                workspace, (uint32_t)b0);
+    }
+  }
+  const uint64_t count = (uint64_t)batch << (2 * k);
+  const unsigned fb = (unsigned)((count + kBlock - 1) / kBlock);
+  if (k == 1)
+    k_rdm_finish<1><<<fb, kBlock, 0, s>>>(workspace, nb, (uint64_t)batch, (double2*)out);
+  else
+    k_rdm_finish<2><<<fb, kBlock, 0, s>>>(workspace, nb, (uint64_t)batch, (double2*)out);
+  return check_launch("reduced density matrix");
+}
+
+int lsim_sv_pauli_expect(const void* states, int dtype, int nbits, long long batch, int nterms,
+                         const unsigned long long* xmask, const unsigned long long* zmask,
+                         double* out, double* workspace, void* stream) {
+  if (int rc = check_common(states, dtype, nbits, batch)) return rc;
+  LSIM_REQUIRE(out != nullptr && workspace != nullptr, "output or workspace pointer is NULL");
+  PauliTerms terms;
+  if (int rc = load_terms(nterms, xmask, zmask, nbits, &terms)) return rc;
+  const uint32_t nb = reduce_blocks(1ull << (nbits - 1));
+  hipStream_t s = as_stream(stream);
+  const long long per_launch = kMaxGridY / nterms;
+  for (long long b0 = 0; b0 < batch; b0 += per_launch) {
+    const long long nbatch = batch - b0 < per_launch ? batch - b0 : per_launch;
+    dim3 grid(nb, (uint32_t)(nbatch * nterms));
+    if (dtype == LSIM_C128)
+      k_pauli_sv<double><<<grid, kBlock, 0, s>>>((const double2*)states, nbits, nb, nterms, terms,
+                                                 workspace, (uint32_t)b0);
+    else
+      k_pauli_sv<float><<<grid, kBlock, 0, s>>>((const float2*)states, nbits, nb, nterms, terms,
+                                                workspace, (uint32_t)b0);
+  }
+  const uint64_t count = (uint64_t)batch * nterms;
+  k_sum_partials<<<(unsigned)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(workspace, nb, count,
+                                                                           out);
+  return check_launch("Pauli expectation");
+}
+
+int lsim_dm_probabilities(const void* rho, int dtype, int n, long long batch, double* probs,
+                          void* stream) {
+  if (int rc = check_common(rho, dtype, 2 * n, batch)) return rc;
+  LSIM_REQUIRE(probs != nullptr, "the output pointer is NULL");
+  const uint64_t count = (uint64_t)batch << n;
+  const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+  if (dtype == LSIM_C128)
+    k_dm_probs<double><<<blocks, kBlock, 0, as_stream(stream)>>>((const double2*)rho, n, batch,
+                                                                 probs);
+  else
+    k_dm_probs<float><<<blocks, kBlock, 0, as_stream(stream)>>>((const float2*)rho, n, batch,
+                                                                probs);
+  return check_launch("density-matrix probabilities");
+}
+
+int lsim_dm_pauli_expect(const void* rho, int dtype, int n, long long batch, int nterms,
+                         const unsigned long long* xmask, const unsigned long long* zmask,
+                         double* out, double* workspace, void* stream) {
+  if (int rc = check_common(rho, dtype, 2 * n, batch)) return rc;
+  LSIM_REQUIRE(out != nullptr && workspace != nullptr, "output or workspace pointer is NULL");
+  PauliTerms terms;
+  if (int rc = load_terms(nterms, xmask, zmask, n, &terms)) return rc;
+  const uint32_t nb = reduce_blocks(1ull << n);
+  hipStream_t s = as_stream(stream);
+  const long long per_launch = kMaxGridY / nterms;
+  for (long long b0 = 0; b0 < batch; b0 += per_launch) {
+    const long long nbatch = batch - b0 < per_launch ? batch - b0 : per_launch;
+    dim3 grid(nb, (uint32_t)(nbatch * nterms));
+    if (dtype == LSIM_C128)
+      k_pauli_dm<double><<<grid, kBlock, 0, s>>>((const double2*)rho, n, nb, nterms, terms,
+                                                 workspace, (uint32_t)b0);
+    else
+      k_pauli_dm<float><<<grid, kBlock, 0, s>>>((const float2*)rho, n, nb, nterms, terms,
+                                                workspace, (uint32_t)b0);
+  }
+  const uint64_t count = (uint64_t)batch * nterms;
+  k_sum_partials<<<(unsigned)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(workspace, nb, count,
+                                                                           out);
+  return check_launch("density-matrix Pauli expectation");
+}
+
+}  // extern "C"

@@ -1,0 +1,382 @@
+"""The statevector and density-matrix engines, resident in MI355X HBM.
+
+Both classes keep the duck-typed engine protocol of lucivant-sim 0.3.0
+src/lucivant_sim/engines.py:138-264 (``apply``, ``scale``, ``apply_channel``, ``measure``,
+``collapse``, ``reset``, ``probabilities``, ``vector``/``matrix``) so the drivers in
+:mod:`lucivant_sim.simulator` read as the reference's do. The state lives in a torch device buffer
+(complex128 by default, qubit q = amplitude index bit q); every operation is a call into
+liblsim.so, the hand-written gfx950 kernels, on torch's current stream. Only scalars and the
+answers the API returns cross back to the host.
+
+:meth:`Statevector.run` and :meth:`DensityMatrix.run` execute a whole planned program (see
+:mod:`lucivant_sim.plan`): every pass's matrix is uploaded in one copy, then the passes are
+launched back to back with no host synchronisation.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Sequence
+
+import numpy as np
+import torch
+from numpy.typing import NDArray
+
+from lucivant_sim import _native as nat
+from lucivant_sim.gates import Matrix
+from lucivant_sim.noise import Channel
+from lucivant_sim.plan import RESET_CHANNEL, Pass, is_diagonal, superop_of_gate
+
+#: Memory guards of the reference (engines.py:29-31): 2^26 amplitudes and 4^13 entries, 1 GiB each.
+MAX_STATEVECTOR_QUBITS = 26
+MAX_DENSITY_QUBITS = 13
+
+
+class SimulationError(RuntimeError):
+    """The circuit can't be simulated as asked (too large, or the method doesn't fit it)."""
+
+
+def require_device() -> torch.device:
+    """The GPU the engines run on; raises when none is visible (there is no CPU fallback)."""
+    if not torch.cuda.is_available():
+        raise nat.NativeError("no GPU is visible: the lucivant_sim engines run on the MI355X (gfx950) only")
+    nat.load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+_TORCH_DTYPE = {nat.C128: torch.complex128, nat.C64: torch.complex64}
+_NP_DTYPE = {nat.C128: np.complex128, nat.C64: np.complex64}
+
+
+class Workspace:
+    """Device scratch for the reductions, grown on demand and reused."""
+
+    def __init__(self) -> None:
+        self.buf: torch.Tensor | None = None
+
+    def get(self, nbytes: int) -> torch.Tensor:
+        if self.buf is None or self.buf.numel() * 8 < nbytes:
+            self.buf = torch.empty(max(nbytes // 8 + 1, 1024), dtype=torch.float64, device=require_device())
+        return self.buf
+
+
+WORKSPACE = Workspace()
+
+
+def upload(arrays: Sequence[np.ndarray], dtype: int = nat.C128) -> tuple[torch.Tensor, list[int]]:
+    """Copy several small complex arrays to the device in one transfer; returns the buffer and
+    each array's device address."""
+    npdt = _NP_DTYPE[dtype]
+    flat = [np.ascontiguousarray(a, dtype=npdt).reshape(-1) for a in arrays]
+    sizes = [f.size for f in flat]
+    host = np.concatenate(flat) if flat else np.zeros(1, dtype=npdt)
+    dev = torch.from_numpy(host).to(require_device(), non_blocking=False)
+    base = ptr(dev)
+    item = np.dtype(npdt).itemsize
+    addrs, off = [], 0
+    for n in sizes:
+        addrs.append(base + off * item)
+        off += n
+    return dev, addrs
+
+
+def run_passes(buf: torch.Tensor, dtype: int, nbits: int, batch: int, passes: Sequence[Pass]) -> None:
+    """Launch ``passes`` in order on ``batch`` states of ``nbits`` bits held in ``buf``."""
+    if not passes:
+        return
+    dev, addrs = upload([p.matrix for p in passes], dtype)
+    stream = stream_handle()
+    state = ptr(buf)
+    for p, addr in zip(passes, addrs):
+        if p.kind == "scale":
+            z = complex(p.matrix[0])
+            nat.call("lsim_sv_scale", state, dtype, nbits, batch, z.real, z.imag, stream)
+            continue
+        fn = "lsim_sv_apply_dense" if p.kind == "dense" else "lsim_sv_apply_diag"
+        nat.call(fn, state, dtype, nbits, batch, p.k, nat.int_array(p.bits), addr, None, stream)
+    # keep the matrices alive until the launches that read them have run
+    dev.record_stream(torch.cuda.current_stream())
+
+
+def _single(matrix: Matrix, bits: tuple[int, ...]) -> Pass:
+    m = np.asarray(matrix, dtype=np.complex128)
+    if is_diagonal(m):
+        return Pass("diag", bits, np.ascontiguousarray(np.diag(m)))
+    return Pass("dense", bits, np.ascontiguousarray(m))
+
+
+def pauli_masks(label: str) -> tuple[int, int]:
+    """(x, z) bit masks of a Pauli label whose rightmost letter is qubit 0."""
+    x = z = 0
+    for q, letter in enumerate(reversed(label.upper())):
+        if letter in "XY":
+            x |= 1 << q
+        if letter in "ZY":
+            z |= 1 << q
+    return x, z
+
+
+class _DeviceState:
+    nbits: int
+    dtype: int
+    buffer: torch.Tensor
+
+    def _stream(self) -> int:
+        return stream_handle()
+
+    def run(self, passes: Sequence[Pass]) -> None:
+        run_passes(self.buffer, self.dtype, self.nbits, 1, passes)
+
+    def _reduced(self, bits: tuple[int, ...]) -> NDArray[np.complex128]:
+        k = len(bits)
+        out = torch.empty(1 << (2 * k), dtype=torch.complex128, device=self.buffer.device)
+        ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(self.nbits, 1, k))
+        nat.call(
+            "lsim_sv_reduced_dm",
+            ptr(self.buffer),
+            self.dtype,
+            self.nbits,
+            1,
+            k,
+            nat.int_array(bits),
+            ptr(out),
+            ptr(ws),
+            self._stream(),
+        )
+        return out.cpu().numpy().reshape(1 << k, 1 << k)
+
+
+class Statevector(_DeviceState):
+    """2^n amplitudes in HBM, starting in |0...0>."""
+
+    def __init__(self, num_qubits: int, *, dtype: int = nat.C128) -> None:
+        if num_qubits > MAX_STATEVECTOR_QUBITS:
+            raise SimulationError(
+                f"{num_qubits} qubits need more memory than the statevector engine allows "
+                f"(up to {MAX_STATEVECTOR_QUBITS})"
+            )
+        device = require_device()
+        self.n = self.nbits = num_qubits
+        self.dtype = dtype
+        self.buffer = torch.empty(1 << num_qubits, dtype=_TORCH_DTYPE[dtype], device=device)
+        nat.call("lsim_sv_set_basis", ptr(self.buffer), dtype, num_qubits, 1, 0, self._stream())
+
+    # -- the engine protocol (engines.py:150-212) ---------------------------------------------
+
+    def apply(self, matrix: Matrix, qubits: tuple[int, ...]) -> None:
+        if not qubits:
+            self.scale(complex(np.asarray(matrix).reshape(-1)[0]))
+            return
+        if len(qubits) > nat.MAX_K:
+            raise SimulationError(f"gates on more than {nat.MAX_K} qubits aren't supported by the device engine")
+        self.run([_single(matrix, tuple(qubits))])
+
+    def scale(self, factor: complex) -> None:
+        f = complex(factor)
+        nat.call("lsim_sv_scale", ptr(self.buffer), self.dtype, self.nbits, 1, f.real, f.imag, self._stream())
+
+    def apply_channel(self, channel: Channel, qubits: tuple[int, ...], rng: np.random.Generator) -> None:
+        """One branch of the channel with its Born probability, consuming exactly one draw
+        (engines.py:159-183): a mixture picks by cumulative probability (the identity branch
+        leaves the state alone); Kraus operators take the first branch whose cumulative weight
+        passes the draw, with every weight from one reduced density matrix."""
+        r = rng.random()
+        if channel.mixture is not None:
+            acc = 0.0
+            for p, unitary in channel.mixture:
+                acc += p
+                if r < acc:
+                    if not np.allclose(unitary, np.eye(unitary.shape[0])):
+                        self.apply(unitary, qubits)
+                    return
+            return
+        rho = self._reduced(tuple(qubits))
+        acc = 0.0
+        weight = 0.0
+        for kraus in channel.kraus:
+            weight = float(np.real(np.trace(kraus @ rho @ kraus.conj().T)))
+            acc += weight
+            if r < acc and weight > 0:
+                self.apply(kraus / math.sqrt(weight), qubits)
+                return
+        if weight > 0:
+            self.apply(channel.kraus[-1] / math.sqrt(weight), qubits)
+
+    def probability_of_one(self, qubit: int) -> float:
+        return float(self._reduced((qubit,))[1, 1].real)
+
+    def measure(self, qubit: int, rng: np.random.Generator) -> int:
+        p1 = min(max(self.probability_of_one(qubit), 0.0), 1.0)
+        outcome = int(rng.random() < p1)
+        self.collapse(qubit, outcome, p1 if outcome else 1 - p1)
+        return outcome
+
+    def collapse(self, qubit: int, outcome: int, probability: float) -> None:
+        scale = 1.0 / math.sqrt(probability) if probability > 0 else 1.0
+        d = np.zeros(2, dtype=np.complex128)
+        d[outcome] = scale
+        self.run([Pass("diag", (qubit,), d)])
+
+    def reset(self, qubit: int, rng: np.random.Generator) -> None:
+        if self.measure(qubit, rng):
+            self.apply(np.array([[0, 1], [1, 0]], dtype=np.complex128), (qubit,))
+
+    def probabilities(self) -> NDArray[np.float64]:
+        out = torch.empty(1 << self.n, dtype=torch.float64, device=self.buffer.device)
+        nat.call("lsim_sv_probabilities", ptr(self.buffer), self.dtype, self.n, 1, ptr(out), self._stream())
+        return out.cpu().numpy()
+
+    def vector(self) -> NDArray[np.complex128]:
+        return self.buffer.cpu().numpy().astype(np.complex128, copy=True)
+
+    @property
+    def tensor(self) -> NDArray[np.complex128]:
+        """A host copy shaped [2]*n (axis n-1-q is qubit q); assigning one uploads it."""
+        v = self.vector()
+        return v.reshape([2] * self.n) if self.n else v
+
+    @tensor.setter
+    def tensor(self, value: NDArray[np.complex128]) -> None:
+        host = np.ascontiguousarray(np.asarray(value).reshape(-1), dtype=_NP_DTYPE[self.dtype])
+        self.buffer.copy_(torch.from_numpy(host))
+
+    # -- beyond the protocol -------------------------------------------------------------------
+
+    def expectations(self, labels: Sequence[str]) -> NDArray[np.float64]:
+        """<psi|P|psi> for each Pauli label (qubit 0 rightmost), from one launch per 64 labels."""
+        return _expect("lsim_sv_pauli_expect", self.buffer, self.dtype, self.n, self.n, labels)
+
+
+class DensityMatrix(_DeviceState):
+    """A 2^n x 2^n density matrix in HBM as a vector of 2n bits (row bits above column bits)."""
+
+    def __init__(self, num_qubits: int, *, dtype: int = nat.C128) -> None:
+        if num_qubits > MAX_DENSITY_QUBITS:
+            raise SimulationError(
+                f"{num_qubits} qubits need more memory than the density-matrix engine allows "
+                f"(up to {MAX_DENSITY_QUBITS}); use method='trajectories'"
+            )
+        device = require_device()
+        self.n = num_qubits
+        self.nbits = 2 * num_qubits
+        self.dtype = dtype
+        self.buffer = torch.empty(1 << self.nbits, dtype=_TORCH_DTYPE[dtype], device=device)
+        nat.call("lsim_sv_set_basis", ptr(self.buffer), dtype, self.nbits, 1, 0, self._stream())
+
+    def _bits(self, qubits: tuple[int, ...]) -> tuple[int, ...]:
+        return tuple(self.n + q for q in qubits) + tuple(qubits)
+
+    def apply(self, matrix: Matrix, qubits: tuple[int, ...]) -> None:
+        """rho <- U rho U^dagger: one superoperator pass for up to two qubits, else U on the row
+        bits and conj(U) on the column bits (engines.py:233-235)."""
+        u = np.asarray(matrix, dtype=np.complex128)
+        q = tuple(qubits)
+        if not q:
+            return
+        if len(q) <= 2:
+            self.run([_single(superop_of_gate(u), self._bits(q))])
+        else:
+            if len(q) > nat.MAX_K:
+                raise SimulationError(f"gates on more than {nat.MAX_K} qubits aren't supported by the device engine")
+            self.run([_single(u, tuple(self.n + x for x in q)), _single(u.conj(), q)])
+
+    def scale(self, factor: complex) -> None:
+        """A global phase leaves a density matrix unchanged."""
+
+    def apply_channel(self, channel: Channel, qubits: tuple[int, ...]) -> None:
+        q = tuple(qubits)
+        if len(q) <= 2:
+            self.run([_single(channel.superoperator(), self._bits(q))])
+            return
+        self._kraus_sum(channel, q)
+
+    def _kraus_sum(self, channel: Channel, q: tuple[int, ...]) -> None:
+        """sum_i K_i rho K_i^dagger out of place, for channels on more than two qubits."""
+        if len(q) > nat.MAX_K:
+            raise SimulationError(f"channels on more than {nat.MAX_K} qubits aren't supported by the device engine")
+        total = torch.zeros_like(self.buffer)
+        stream = self._stream()
+        for k in channel.kraus:
+            term = self.buffer.clone()
+            run_passes(term, self.dtype, self.nbits, 1, [_single(k, tuple(self.n + x for x in q)), _single(k.conj(), q)])
+            nat.call("lsim_sv_axpy", ptr(total), ptr(term), self.dtype, self.nbits, 1, 1.0, 0.0, stream)
+        self.buffer = total
+
+    def reset(self, qubit: int) -> None:
+        self.apply_channel(RESET_CHANNEL, (qubit,))
+
+    def run_kraus(self, channel: Channel, qubits: tuple[int, ...]) -> None:
+        self._kraus_sum(channel, tuple(qubits))
+
+    def probabilities(self) -> NDArray[np.float64]:
+        out = torch.empty(1 << self.n, dtype=torch.float64, device=self.buffer.device)
+        nat.call("lsim_dm_probabilities", ptr(self.buffer), self.dtype, self.n, 1, ptr(out), self._stream())
+        return out.cpu().numpy()
+
+    def matrix(self) -> NDArray[np.complex128]:
+        d = 1 << self.n
+        return self.buffer.cpu().numpy().astype(np.complex128, copy=True).reshape(d, d)
+
+    @property
+    def tensor(self) -> NDArray[np.complex128]:
+        m = self.matrix()
+        return m.reshape([2] * (2 * self.n)) if self.n else m
+
+    @tensor.setter
+    def tensor(self, value: NDArray[np.complex128]) -> None:
+        host = np.ascontiguousarray(np.asarray(value).reshape(-1), dtype=_NP_DTYPE[self.dtype])
+        self.buffer.copy_(torch.from_numpy(host))
+
+    def expectations(self, labels: Sequence[str]) -> NDArray[np.float64]:
+        """Re Tr(P rho) for each Pauli label (qubit 0 rightmost)."""
+        return _expect("lsim_dm_pauli_expect", self.buffer, self.dtype, self.n, 2 * self.n, labels)
+
+
+def _expect(fn: str, buf: torch.Tensor, dtype: int, n: int, nbits: int, labels: Sequence[str]) -> NDArray[np.float64]:
+    labels = list(labels)
+    if n == 0:
+        # a 0-qubit state is the scalar 1 (or rho = [[1]]): every (empty) string has value 1
+        return np.ones(len(labels))
+    out = np.empty(len(labels))
+    stream = stream_handle()
+    for start in range(0, len(labels), nat.MAX_TERMS):
+        chunk = labels[start : start + nat.MAX_TERMS]
+        masks = [pauli_masks(lbl) for lbl in chunk]
+        xs = nat.ull_array([m[0] for m in masks])
+        zs = nat.ull_array([m[1] for m in masks])
+        res = torch.empty(len(chunk), dtype=torch.float64, device=buf.device)
+        ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(nbits, 1, len(chunk)))
+        nat.call(fn, ptr(buf), dtype, n, 1, len(chunk), xs, zs, ptr(res), ptr(ws), stream)
+        out[start : start + len(chunk)] = res.cpu().numpy()
+    return out
+
+
+def marginal(probabilities: NDArray[np.float64], num_qubits: int, qubits: list[int]) -> NDArray[np.float64]:
+    """Probabilities of ``qubits`` alone (index bit i = i-th smallest of them), by summing the
+    others out exactly as engines.py:267-275 does, so sampling sees the same numbers."""
+    keep = sorted(qubits)
+    if num_qubits == 0:
+        return probabilities
+    tensor = probabilities.reshape([2] * num_qubits)
+    drop = tuple(num_qubits - 1 - q for q in range(num_qubits) if q not in keep)
+    reduced = tensor.sum(axis=drop) if drop else tensor
+    return np.asarray(reduced).reshape(-1)
+
+
+__all__ = [
+    "MAX_DENSITY_QUBITS",
+    "MAX_STATEVECTOR_QUBITS",
+    "DensityMatrix",
+    "SimulationError",
+    "Statevector",
+    "marginal",
+    "pauli_masks",
+]

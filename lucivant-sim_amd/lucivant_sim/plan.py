@@ -1,0 +1,203 @@
+"""Fusion planning: turning a circuit's operations into the device passes that run it.
+
+A pass is one read and one write of the whole state in HBM, so the planner's job is to apply as
+many gates as possible per pass. Statevector: consecutive gates are merged greedily into dense
+operators on at most ``width`` qubits (the kernel handles up to 5); a gate joins the latest block
+that touches any of its qubits when the union still fits, which is always legal because no later
+block touches those qubits. Diagonal results run on the diagonal kernel. Global phases fold into
+the first pass. Density matrix: every gate and its noise channel become superoperators on the
+2k vector bits of their qubits (U (x) conj(U) for a gate, sum K (x) conj(K) for a channel,
+the analytic depolariser as (1-p) I + (p/d)|I>><<I|) and are merged into blocks of at most
+``width`` qubits (2 by default, a 16 x 16 superoperator on 4 bits).
+
+The reference runs one NumPy sweep per gate (engines.py:38-119 via simulator.py:229-251) and
+merges only runs of one-qubit gates (simulator.py:196-226). Fusing more gates changes results by
+rounding only.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from lucivant_sim.circuit import Barrier, Gate, GlobalPhase, Measure, Operation, Reset
+from lucivant_sim.gates import Matrix
+from lucivant_sim.noise import Channel, NoiseModel
+
+#: Widest operator the dense kernel applies (bits).
+KERNEL_MAX_BITS = 5
+
+
+def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
+    """``sub`` acting on local positions ``positions`` (first = its most significant bit) of a
+    k-position operator whose position j is index bit k-1-j, as a 2^k x 2^k matrix."""
+    m = len(positions)
+    d = 1 << k
+    idx = np.arange(d)
+    sub_idx = np.zeros(d, dtype=np.int64)
+    mask = 0
+    for i, p in enumerate(positions):
+        sub_idx |= ((idx >> (k - 1 - p)) & 1) << (m - 1 - i)
+        mask |= 1 << (k - 1 - p)
+    rest = idx & ~mask
+    return np.where(rest[:, None] == rest[None, :], sub[sub_idx[:, None], sub_idx[None, :]], 0).astype(np.complex128)
+
+
+def is_diagonal(m: Matrix) -> bool:
+    return not np.any(m - np.diag(np.diag(m)))
+
+
+@dataclass
+class Pass:
+    """One device pass: ``matrix`` (2^k x 2^k) on vector ``bits`` (first = most significant
+    matrix bit). ``kind`` is "dense", "diag" (matrix holds the diagonal only) or "scale"."""
+
+    kind: str
+    bits: tuple[int, ...]
+    matrix: np.ndarray
+    gates: int = 1  # reference operations this pass replaces
+
+    @property
+    def k(self) -> int:
+        return len(self.bits)
+
+
+@dataclass
+class _Block:
+    qubits: list[int]
+    matrix: Matrix
+    gates: int = 0
+
+    def extend(self, qubits: tuple[int, ...]) -> None:
+        new = [q for q in qubits if q not in self.qubits]
+        if new:
+            self.matrix = np.kron(self.matrix, np.eye(1 << len(new), dtype=np.complex128))
+            self.qubits.extend(new)
+
+    def absorb(self, matrix: Matrix, qubits: tuple[int, ...]) -> None:
+        self.extend(qubits)
+        pos = [self.qubits.index(q) for q in qubits]
+        self.matrix = embed(matrix, pos, len(self.qubits)) @ self.matrix
+        self.gates += 1
+
+
+def _greedy(items: list[tuple[Matrix, tuple[int, ...]]], width: int) -> list[_Block]:
+    blocks: list[_Block] = []
+    last: dict[int, int] = {}  # qubit -> index of the latest block touching it
+    for matrix, qubits in items:
+        j = max((last.get(q, -1) for q in qubits), default=-1)
+        if j >= 0 and len(set(blocks[j].qubits) | set(qubits)) <= width:
+            blocks[j].absorb(matrix, qubits)
+        else:
+            blk = _Block(list(qubits), np.eye(1 << len(qubits), dtype=np.complex128))
+            blk.absorb(matrix, qubits)
+            blocks.append(blk)
+            j = len(blocks) - 1
+        for q in qubits:
+            last[q] = j
+    return blocks
+
+
+def plan_statevector(ops: tuple[Operation, ...], num_qubits: int, width: int = 4) -> list[Pass]:
+    """Passes for an ideal static circuit on a statevector (measurements deferred, resets of
+    untouched qubits and barriers are no-ops, as in simulator.py:229-251)."""
+    if not 1 <= width <= KERNEL_MAX_BITS:
+        raise ValueError(f"fusion width must be between 1 and {KERNEL_MAX_BITS}")
+    items: list[tuple[Matrix, tuple[int, ...]]] = []
+    phase = 1.0 + 0.0j
+    for op in ops:
+        if isinstance(op, Gate):
+            if len(op.qubits) > KERNEL_MAX_BITS:
+                raise ValueError(f"{op.name} acts on {len(op.qubits)} qubits; gates of up to {KERNEL_MAX_BITS} are supported")
+            items.append((np.asarray(op.matrix, dtype=np.complex128), tuple(op.qubits)))
+        elif isinstance(op, GlobalPhase):
+            phase *= np.exp(1j * op.angle)
+        elif isinstance(op, (Measure, Reset, Barrier)):
+            continue
+        else:
+            raise ValueError(f"a static statevector can't run {type(op).__name__}")
+    passes: list[Pass] = []
+    for blk in _greedy(items, width):
+        passes.append(_as_pass(blk.matrix, tuple(blk.qubits), blk.gates))
+    if phase != 1:
+        if passes:
+            p = passes[0]
+            p.matrix = p.matrix * phase
+        elif num_qubits > 0:
+            passes.append(Pass("scale", (), np.array([phase], dtype=np.complex128), 0))
+    return passes
+
+
+def _as_pass(matrix: Matrix, bits: tuple[int, ...], gates: int) -> Pass:
+    if is_diagonal(matrix):
+        return Pass("diag", bits, np.ascontiguousarray(np.diag(matrix)), gates)
+    return Pass("dense", bits, np.ascontiguousarray(matrix), gates)
+
+
+def superop_of_gate(u: Matrix) -> Matrix:
+    """U (x) conj(U): rho -> U rho U^dagger on the row-then-column vectorisation."""
+    return np.kron(u, u.conj())
+
+
+def plan_density(
+    ops: tuple[Operation, ...], num_qubits: int, noise: NoiseModel | None, width: int = 2
+) -> list[Pass | tuple[str, object]]:
+    """Passes for a static circuit on an n-qubit density matrix (vector bits: n+q row, q column).
+
+    Each gate and channel on at most ``width`` qubits becomes a superoperator on the vector bits
+    (n+q..., q...) of its qubits; these are fused by the same greedy rule as statevector gates,
+    with the width counted in vector bits (2 per qubit). A wider gate becomes two passes, U on its
+    row bits then conj(U) on its column bits; a wider channel is returned as
+    ("kraus", (channel, qubits)) for the engine's out-of-place Kraus sum."""
+    n = num_qubits
+    if not 1 <= width <= 2:
+        raise ValueError("density-matrix fusion width is 1 or 2 qubits")
+    out: list[Pass | tuple[str, object]] = []
+    items: list[tuple[Matrix, tuple[int, ...]]] = []
+
+    def vbits(q: tuple[int, ...]) -> tuple[int, ...]:
+        return tuple(n + x for x in q) + tuple(q)
+
+    def flush() -> None:
+        for blk in _greedy(items, 2 * width):
+            out.append(_as_pass(blk.matrix, tuple(blk.qubits), blk.gates))
+        items.clear()
+
+    def channel(ch: Channel, q: tuple[int, ...]) -> None:
+        if len(q) <= width:
+            items.append((ch.superoperator(), vbits(q)))
+        else:
+            flush()
+            out.append(("kraus", (ch, q)))
+
+    for op in ops:
+        if isinstance(op, Gate):
+            u = np.asarray(op.matrix, dtype=np.complex128)
+            q = tuple(op.qubits)
+            if len(q) <= width:
+                items.append((superop_of_gate(u), vbits(q)))
+            else:
+                flush()
+                if len(q) > KERNEL_MAX_BITS:
+                    raise ValueError(f"{op.name} acts on {len(q)} qubits; gates of up to {KERNEL_MAX_BITS} are supported")
+                out.append(_as_pass(u, tuple(n + x for x in q), 1))
+                out.append(_as_pass(np.ascontiguousarray(u.conj()), q, 0))
+            ch = noise.gate_error(op.name, q) if noise is not None else None
+            if ch is not None:
+                channel(ch, q)
+        elif isinstance(op, Reset):
+            channel(RESET_CHANNEL, (op.qubit,))
+        elif isinstance(op, (Measure, Barrier, GlobalPhase)):
+            continue
+        else:
+            raise ValueError(f"a density matrix can't run {type(op).__name__} (it needs trajectories)")
+    flush()
+    return out
+
+
+RESET_CHANNEL = Channel(
+    "reset",
+    1,
+    (np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 1], [0, 0]], dtype=np.complex128)),
+)

@@ -1,0 +1,405 @@
+"""Running circuits: :func:`run` for counts; :func:`statevector`, :func:`density_matrix`,
+:func:`probabilities`, :func:`outcome_probabilities` and :func:`expectation` for exact answers.
+
+The public functions, their arguments, errors, method selection, seeding, sampling, result
+digests and fingerprints follow lucivant-sim 0.3.0 src/lucivant_sim/simulator.py so that a
+program, noise model, shot count and seed give the same counts and the same fingerprint on
+either implementation (and ``verify`` accepts results saved by the other). What differs is
+where the state lives: every engine here is the MI355X device engine of
+:mod:`lucivant_sim.engines`, fed by the fusion planner of :mod:`lucivant_sim.plan`, and
+trajectories run as one batch of states (:mod:`lucivant_sim.trajectories`).
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import secrets
+import time
+from collections.abc import Mapping
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Any, Literal
+
+import numpy as np
+from numpy.typing import NDArray
+
+from lucivant_sim import __version__
+from lucivant_sim.circuit import Circuit, IfElse
+from lucivant_sim.clifford import clifford_problem
+from lucivant_sim.engines import MAX_DENSITY_QUBITS, DensityMatrix, SimulationError, Statevector, marginal
+from lucivant_sim.noise import NoiseModel
+from lucivant_sim.plan import Pass, plan_density, plan_statevector
+from lucivant_sim.qasm import load, parse
+
+Method = Literal[
+    "automatic", "statevector", "density_matrix", "trajectories", "stabilizer", "statevector_aer", "stabilizer_stim"
+]
+METHODS: tuple[str, ...] = (
+    "automatic",
+    "statevector",
+    "density_matrix",
+    "trajectories",
+    "stabilizer",
+    "statevector_aer",
+    "stabilizer_stim",
+)
+AUTO_DENSITY_QUBITS = 10
+AUTO_STABILIZER_QUBITS = 12
+MAX_SHOTS = 10_000_000
+
+#: Identifies the engine that produced a result. It is reported beside the result, not inside
+#: the fingerprint, so results stay verifiable against the reference implementation.
+ENGINE = "lucivant-sim-mi355x"
+
+#: Statevector fusion width (qubits per dense pass); set by the autotuner when it has run.
+FUSION_WIDTH = 4
+
+Program = str | os.PathLike[str] | Circuit | Any
+
+
+def to_circuit(program: Program, params: Mapping[str, float] | None = None) -> Circuit:
+    """A Circuit from OpenQASM text, a .qasm path, or an already built Circuit."""
+    if isinstance(program, Circuit):
+        if params:
+            raise ValueError("params apply to OpenQASM source, not to an already-built Circuit")
+        return program
+    if isinstance(program, os.PathLike):
+        return load(Path(program), params)
+    if isinstance(program, str):
+        return parse(program, params)
+    module = type(program).__module__
+    if module.startswith("qiskit"):
+        from qiskit import qasm3  # type: ignore[import-not-found]
+
+        return parse(qasm3.dumps(program), params)
+    if module.startswith("cirq"):
+        import cirq  # type: ignore[import-not-found]
+
+        return parse(cirq.qasm(program), params)
+    raise TypeError(f"can't simulate a {type(program).__name__}; give OpenQASM, a path, or a Qiskit or Cirq circuit")
+
+
+def _digest(data: Any) -> str:
+    return "sha256:" + hashlib.sha256(json.dumps(data, separators=(",", ":"), sort_keys=True).encode()).hexdigest()
+
+
+@dataclass(frozen=True)
+class Result:
+    counts: dict[str, int]
+    shots: int
+    seed: int
+    method: str
+    num_qubits: int
+    num_clbits: int
+    circuit_digest: str
+    noise_digest: str | None
+    version: str
+    elapsed_seconds: float
+    #: Final state of an ideal static circuit before its measurements (statevector method).
+    statevector: NDArray[np.complex128] | None = field(default=None, repr=False, compare=False)
+    engine: str = field(default=ENGINE, compare=False)
+
+    @property
+    def fingerprint(self) -> str:
+        """Identifies what was run: the same fingerprint gives the same counts."""
+        return _digest(
+            {
+                "engine": "lucivant-sim",
+                "version": self.version,
+                "circuit": self.circuit_digest,
+                "noise": self.noise_digest,
+                "shots": self.shots,
+                "seed": self.seed,
+                "method": self.method,
+            }
+        )
+
+    @property
+    def counts_digest(self) -> str:
+        return _digest(self.counts)
+
+    def frequencies(self) -> dict[str, float]:
+        return {k: v / self.shots for k, v in self.counts.items()} if self.shots else {}
+
+    def most_frequent(self) -> str | None:
+        return max(self.counts, key=lambda k: (self.counts[k], k)) if self.counts else None
+
+    def to_dict(self) -> dict[str, Any]:
+        return {
+            "engine": "lucivant-sim",
+            "version": self.version,
+            "method": self.method,
+            "shots": self.shots,
+            "seed": self.seed,
+            "num_qubits": self.num_qubits,
+            "num_clbits": self.num_clbits,
+            "counts": self.counts,
+            "circuit_digest": self.circuit_digest,
+            "noise_digest": self.noise_digest,
+            "fingerprint": self.fingerprint,
+            "counts_digest": self.counts_digest,
+            "elapsed_seconds": self.elapsed_seconds,
+        }
+
+
+def choose_method(circuit: Circuit, noise: NoiseModel | None, method: str = "automatic") -> str:
+    """The method that runs ``circuit``: the decision table and refusals of simulator.py:157-193."""
+    if method not in METHODS:
+        raise ValueError(f"method must be one of {', '.join(METHODS)}")
+    gate_noise = noise is not None and noise.has_gate_errors()
+    dynamic = circuit.is_dynamic()
+    if method == "automatic":
+        if circuit.num_qubits >= AUTO_STABILIZER_QUBITS and clifford_problem(circuit, noise) is None:
+            return "stabilizer"
+        if dynamic:
+            return "trajectories"
+        if gate_noise:
+            return "density_matrix" if circuit.num_qubits <= AUTO_DENSITY_QUBITS else "trajectories"
+        return "statevector"
+    if method == "stabilizer":
+        problem = clifford_problem(circuit, noise)
+        if problem:
+            raise SimulationError(f"the stabilizer method runs Clifford circuits only: {problem}")
+        return method
+    if method == "stabilizer_stim":
+        if noise is not None and not noise.is_empty():
+            raise SimulationError("stabilizer_stim runs ideal circuits; use 'stabilizer' for Pauli noise")
+        if any(isinstance(op, IfElse) for op in circuit.operations):
+            raise SimulationError("stabilizer_stim runs circuits without classical conditions; use 'stabilizer'")
+        problem = clifford_problem(circuit, None)
+        if problem:
+            raise SimulationError(f"the stabilizer methods run Clifford circuits only: {problem}")
+        return method
+    if method in ("statevector", "statevector_aer") and (gate_noise or dynamic):
+        raise SimulationError(
+            "the statevector method runs ideal circuits without mid-circuit measurement; "
+            "use 'density_matrix' or 'trajectories'"
+        )
+    if method == "density_matrix" and dynamic:
+        raise SimulationError("this circuit uses measurement results mid-circuit; use 'trajectories'")
+    if method == "density_matrix" and circuit.num_qubits > MAX_DENSITY_QUBITS:
+        raise SimulationError(f"the density-matrix method handles up to {MAX_DENSITY_QUBITS} qubits")
+    return method
+
+
+def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseModel | None, width: int | None = None) -> None:
+    """Static evolution (simulator.py:229-251): gates, their noise, resets of untouched qubits;
+    measurements deferred. Runs as fused device passes."""
+    if isinstance(state, Statevector):
+        state.run(plan_statevector(circuit.operations, circuit.num_qubits, width or FUSION_WIDTH))
+        return
+    group: list[Pass] = []
+    for item in plan_density(circuit.operations, circuit.num_qubits, noise):
+        if isinstance(item, Pass):
+            group.append(item)
+            continue
+        state.run(group)
+        group = []
+        _, (channel, qubits) = item  # type: ignore[misc]
+        state.run_kraus(channel, qubits)
+    state.run(group)
+
+
+def _sample(
+    circuit: Circuit, probs: NDArray[np.float64], shots: int, noise: NoiseModel | None, rng: np.random.Generator
+) -> dict[str, int]:
+    """Counts from exact probabilities with the run's generator (simulator.py:254-284): inverse
+    CDF over the measured qubits' marginal, then readout flips per (qubit, clbit) pair in
+    final-measurement order, drawn only for non-zero errors."""
+    pairs = circuit.final_measurements()
+    if not pairs or shots == 0:
+        return {}
+    measured = sorted({q for q, _ in pairs})
+    position = {q: i for i, q in enumerate(measured)}
+    dist = marginal(probs, circuit.num_qubits, measured)
+    cumulative = np.cumsum(dist)
+    cumulative /= cumulative[-1]
+    outcomes = np.searchsorted(cumulative, rng.random(shots), side="right").astype(np.int64)
+    outcomes = np.minimum(outcomes, len(dist) - 1)
+    wide = circuit.num_clbits > 62
+    bits = np.zeros(shots, dtype=object if wide else np.int64)
+    for q, c in pairs:
+        value = (outcomes >> position[q]) & 1
+        err = noise.readout_error(q) if noise else None
+        if err is not None and (err.p01 or err.p10):
+            flip = rng.random(shots) < np.where(value == 1, err.p10, err.p01)
+            value = value ^ flip.astype(np.int64)
+        if wide:
+            bits = bits + np.array([int(v) << c for v in value], dtype=object)
+        else:
+            bits |= value << c
+    values, counts = np.unique(bits, return_counts=True)
+    return {circuit.format_bits(int(v)): int(n) for v, n in zip(values, counts)}
+
+
+def run(
+    program: Program,
+    shots: int = 1024,
+    *,
+    seed: int | None = None,
+    noise: NoiseModel | None = None,
+    method: Method | str = "automatic",
+    params: Mapping[str, float] | None = None,
+) -> Result:
+    """Simulate ``program`` ``shots`` times and return the counts (simulator.py:320-393)."""
+    if not 0 <= shots <= MAX_SHOTS:
+        raise ValueError(f"shots must be between 0 and {MAX_SHOTS:,}")
+    circuit = to_circuit(program, params)
+    chosen = choose_method(circuit, noise, method)
+    seed = secrets.randbelow(2**63) if seed is None else int(seed)
+    if seed < 0:
+        raise ValueError("seed must not be negative")
+    rng = np.random.default_rng(seed)
+    started = time.perf_counter()
+    vector = None
+    if chosen in ("stabilizer", "stabilizer_stim", "statevector_aer"):
+        raise SimulationError(
+            f"the {chosen!r} method isn't part of the MI355X build; pass method='statevector' "
+            "(or 'trajectories' for noisy or dynamic circuits)"
+        )
+    if chosen == "trajectories":
+        from lucivant_sim.trajectories import run_trajectories
+
+        if circuit.num_qubits > 0:
+            Statevector(circuit.num_qubits)  # the size check before any shot
+        tally = run_trajectories(circuit, noise, shots, rng)
+        counts = {circuit.format_bits(b): n for b, n in sorted(tally.items())} if circuit.num_clbits else {}
+    else:
+        state: Statevector | DensityMatrix = (
+            Statevector(circuit.num_qubits) if chosen == "statevector" else DensityMatrix(circuit.num_qubits)
+        )
+        _evolve(state, circuit, noise)
+        if isinstance(state, Statevector):
+            vector = state.vector()
+        counts = _sample(circuit, state.probabilities(), shots, noise, rng)
+    elapsed = round(time.perf_counter() - started, 6)
+    return Result(
+        counts=dict(sorted(counts.items())),
+        shots=shots,
+        seed=seed,
+        method=chosen,
+        num_qubits=circuit.num_qubits,
+        num_clbits=circuit.num_clbits,
+        circuit_digest=circuit.digest(),
+        noise_digest=noise.digest() if noise is not None and not noise.is_empty() else None,
+        version=__version__,
+        elapsed_seconds=elapsed,
+        statevector=vector,
+    )
+
+
+def _static(circuit: Circuit, what: str) -> None:
+    if circuit.is_dynamic():
+        raise SimulationError(f"{what} needs a circuit without mid-circuit measurement or conditions")
+
+
+def statevector(program: Program, params: Mapping[str, float] | None = None) -> NDArray[np.complex128]:
+    """The ideal final state (measurements ignored), index bit q = qubit q."""
+    circuit = to_circuit(program, params)
+    _static(circuit, "a statevector")
+    state = Statevector(circuit.num_qubits)
+    _evolve(state, circuit, None)
+    return state.vector()
+
+
+def density_matrix(
+    program: Program, noise: NoiseModel | None = None, params: Mapping[str, float] | None = None
+) -> NDArray[np.complex128]:
+    """The final mixed state under ``noise`` (measurements ignored)."""
+    circuit = to_circuit(program, params)
+    _static(circuit, "a density matrix")
+    state = DensityMatrix(circuit.num_qubits)
+    _evolve(state, circuit, noise)
+    return state.matrix()
+
+
+def _exact_state(circuit: Circuit, noise: NoiseModel | None) -> Statevector | DensityMatrix:
+    gate_noise = noise is not None and noise.has_gate_errors()
+    state: Statevector | DensityMatrix = DensityMatrix(circuit.num_qubits) if gate_noise else Statevector(circuit.num_qubits)
+    _evolve(state, circuit, noise)
+    return state
+
+
+def probabilities(
+    program: Program, noise: NoiseModel | None = None, params: Mapping[str, float] | None = None, *, threshold: float = 1e-12
+) -> dict[str, float]:
+    """Probabilities of every basis state above ``threshold`` (qubit 0 rightmost), before readout
+    error; the density matrix when there is gate noise, else the statevector."""
+    circuit = to_circuit(program, params)
+    _static(circuit, "exact probabilities")
+    probs = _exact_state(circuit, noise).probabilities()
+    n = circuit.num_qubits
+    keep = np.flatnonzero(probs > threshold)
+    return {format(int(i), f"0{n}b"): float(probs[i]) for i in keep}
+
+
+def outcome_probabilities(
+    program: Program, noise: NoiseModel | None = None, params: Mapping[str, float] | None = None, *, threshold: float = 1e-12
+) -> dict[str, float]:
+    """Probabilities of the classical outcomes, keyed as Result.counts, with readout error
+    (simulator.py:445-486)."""
+    circuit = to_circuit(program, params)
+    _static(circuit, "exact outcome probabilities")
+    pairs = circuit.final_measurements()
+    if not pairs:
+        return {}
+    state = _exact_state(circuit, noise)
+    measured = sorted({q for q, _ in pairs})
+    position = {q: i for i, q in enumerate(measured)}
+    dist = marginal(state.probabilities(), circuit.num_qubits, measured)
+    out: dict[int, float] = {}
+    for index, p in enumerate(dist):
+        if p <= threshold:
+            continue
+        branches = [(0, float(p))]
+        for q, c in pairs:
+            bit = (index >> position[q]) & 1
+            err = noise.readout_error(q) if noise else None
+            flip = (err.p10 if bit else err.p01) if err is not None else 0.0
+            nxt = []
+            for bits, w in branches:
+                if flip < 1:
+                    nxt.append((bits | (bit << c), w * (1 - flip)))
+                if flip > 0:
+                    nxt.append((bits | ((bit ^ 1) << c), w * flip))
+            branches = nxt
+        for bits, w in branches:
+            out[bits] = out.get(bits, 0.0) + w
+    return {circuit.format_bits(b): w for b, w in sorted(out.items()) if w > threshold}
+
+
+def expectation(
+    program: Program,
+    observable: str | Mapping[str, float],
+    noise: NoiseModel | None = None,
+    params: Mapping[str, float] | None = None,
+) -> float:
+    """<O> for a Pauli string ("ZZ", qubit 0 rightmost) or a weighted sum {"ZI": 0.5, "XX": -1}.
+    Every term is evaluated by one device reduction over the final state (simulator.py:489-527)."""
+    circuit = to_circuit(program, params)
+    _static(circuit, "an expectation value")
+    terms = {observable: 1.0} if isinstance(observable, str) else dict(observable)
+    n = circuit.num_qubits
+    for label in terms:
+        if len(label) != n or set(label.upper()) - set("IXYZ"):
+            raise ValueError(f"{label!r} must be a Pauli string of length {n} (qubit 0 rightmost)")
+    state = _exact_state(circuit, noise)
+    labels = list(terms)
+    values = state.expectations(labels) if labels else np.zeros(0)
+    total = 0.0
+    for label, v in zip(labels, values):
+        total += terms[label] * float(v)
+    return total
+
+
+def verify(saved: Mapping[str, Any] | str | Path, program: Program, noise: NoiseModel | None = None) -> bool:
+    """Rerun a saved result (``Result.to_dict()``) and check the counts are identical."""
+    data = json.loads(Path(saved).read_text(encoding="utf-8")) if isinstance(saved, (str, Path)) else dict(saved)
+    again = run(program, int(data["shots"]), seed=int(data["seed"]), noise=noise, method=str(data["method"]))
+    if again.circuit_digest != data.get("circuit_digest"):
+        return False
+    if (again.noise_digest or None) != (data.get("noise_digest") or None):
+        return False
+    return again.counts_digest == data.get("counts_digest") and again.counts == data.get("counts")

@@ -1,0 +1,320 @@
+"""NumPy restatement of the reference engines and drivers (test infrastructure only).
+
+Follows lucivant-sim 0.3.0:
+- state layout, gate contraction: src/lucivant_sim/engines.py:8-9, 38-42, 138-157;
+- trajectory channel choice, measurement, collapse, reset: engines.py:159-205;
+- density matrix, channels, analytic depolarising: engines.py:122-135, 215-264;
+- marginal: engines.py:267-275;
+- static evolution, sampling, trajectories, expectation: simulator.py:229-317, 489-533.
+
+The circuit comes in as lucivant_sim's Circuit IR (plain host data: gates already resolved to
+matrices). Every state update here is written independently of the device engine: tensordot on
+the [2]*n tensor, exactly as the reference contracts it, with no fusion.
+"""
+
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+PAULI = {
+    "I": np.eye(2, dtype=np.complex128),
+    "X": np.array([[0, 1], [1, 0]], dtype=np.complex128),
+    "Y": np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+    "Z": np.array([[1, 0], [0, -1]], dtype=np.complex128),
+}
+
+
+# --------------------------------------------------------------------------- gate application
+
+
+def contract(tensor: np.ndarray, matrix: np.ndarray, axes: list[int]) -> np.ndarray:
+    """U on tensor axes ``axes`` (first = matrix MSB): engines.py:38-42."""
+    k = len(axes)
+    out = np.tensordot(np.asarray(matrix).reshape([2] * (2 * k)), tensor, axes=(list(range(k, 2 * k)), axes))
+    return np.moveaxis(out, list(range(k)), axes)
+
+
+def apply_sv(psi: np.ndarray, matrix: np.ndarray, qubits: tuple[int, ...], n: int) -> np.ndarray:
+    """psi <- U psi on ``qubits`` (qubit q = axis n-1-q = index bit q)."""
+    if n == 0:
+        return psi * np.asarray(matrix).reshape(-1)[0]
+    t = psi.reshape([2] * n)
+    return contract(t, matrix, [n - 1 - q for q in qubits]).reshape(-1)
+
+
+def full_operator(matrix: np.ndarray, qubits: tuple[int, ...], n: int) -> np.ndarray:
+    """The 2^n x 2^n operator of a k-qubit matrix on ``qubits``, built entry by entry:
+    <i|O|j> = U[r(i), r(j)] when i and j agree outside ``qubits``, where r(x) reads x's bits at
+    ``qubits`` with the first qubit most significant. The brute-force check of ``apply_sv``."""
+    d = 1 << n
+    k = len(qubits)
+    idx = np.arange(d)
+    r = np.zeros(d, dtype=np.int64)
+    mask = 0
+    for i, q in enumerate(qubits):
+        r |= ((idx >> q) & 1) << (k - 1 - i)
+        mask |= 1 << q
+    same = (idx[:, None] & ~mask) == (idx[None, :] & ~mask)
+    return np.where(same, np.asarray(matrix)[r[:, None], r[None, :]], 0)
+
+
+# ------------------------------------------------------------------------------- statevector
+
+
+def evolve_sv(circuit) -> np.ndarray:  # type: ignore[no-untyped-def]
+    """Ideal static evolution, one contraction per gate (simulator.py:229-251)."""
+    from lucivant_sim.circuit import Gate, GlobalPhase
+
+    n = circuit.num_qubits
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1
+    for op in circuit.operations:
+        if isinstance(op, Gate):
+            psi = apply_sv(psi, op.matrix, op.qubits, n)
+        elif isinstance(op, GlobalPhase):
+            psi = psi * np.exp(1j * op.angle)
+    return psi
+
+
+def probabilities_sv(psi: np.ndarray) -> np.ndarray:
+    return np.abs(psi) ** 2
+
+
+def pauli_matrix(label: str) -> np.ndarray:
+    """Dense matrix of a label whose rightmost letter is qubit 0 (so the leftmost is the MSB)."""
+    m = np.array([[1]], dtype=np.complex128)
+    for letter in label.upper():
+        m = np.kron(m, PAULI[letter])
+    return m
+
+
+def expectation_sv(psi: np.ndarray, label: str, n: int) -> float:
+    """<psi|P|psi> by applying the letters one by one (simulator.py:517-527)."""
+    probe = psi.copy()
+    for q, letter in enumerate(reversed(label.upper())):
+        if letter != "I":
+            probe = apply_sv(probe, PAULI[letter], (q,), n)
+    return float(np.vdot(psi, probe).real)
+
+
+# --------------------------------------------------------------------------- density matrix
+
+
+def apply_dm(rho: np.ndarray, matrix: np.ndarray, qubits: tuple[int, ...], n: int) -> np.ndarray:
+    """U on the row axes, conj(U) on the column axes (engines.py:233-235)."""
+    t = rho.reshape([2] * (2 * n))
+    t = contract(t, matrix, [n - 1 - q for q in qubits])
+    t = contract(t, np.asarray(matrix).conj(), [2 * n - 1 - q for q in qubits])
+    return t.reshape(1 << n, 1 << n)
+
+
+def depolarizing_trace(rho: np.ndarray, p: float, qubits: tuple[int, ...], n: int) -> np.ndarray:
+    """(1 - p) rho + p I/d (x) Tr_targets(rho) (engines.py:122-135)."""
+    k = len(qubits)
+    d = 1 << k
+    rows = [n - 1 - q for q in qubits]
+    cols = [2 * n - 1 - q for q in qubits]
+    t = rho.reshape([2] * (2 * n))
+    moved = np.moveaxis(t, rows + cols, range(2 * k))
+    m = moved.reshape((d, d, *moved.shape[2 * k :]))
+    partial = np.trace(m, axis1=0, axis2=1)
+    out = (1 - p) * m
+    for i in range(d):
+        out[i, i] += (p / d) * partial
+    return np.moveaxis(out.reshape(moved.shape), range(2 * k), rows + cols).reshape(1 << n, 1 << n)
+
+
+def apply_channel_dm(rho: np.ndarray, channel, qubits: tuple[int, ...], n: int) -> np.ndarray:  # type: ignore[no-untyped-def]
+    """engines.py:240-250: the analytic depolariser for k in (1, 2), else the Kraus sum."""
+    if channel.analytic_depolarizing and len(qubits) in (1, 2):
+        return depolarizing_trace(rho, channel.params[0], qubits, n)
+    total = np.zeros_like(rho)
+    for k in channel.kraus:
+        total = total + apply_dm(rho, k, qubits, n)
+    return total
+
+
+RESET_KRAUS = (np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 1], [0, 0]], dtype=np.complex128))
+
+
+def evolve_dm(circuit, noise) -> np.ndarray:  # type: ignore[no-untyped-def]
+    """Static evolution of rho with gate noise and resets (simulator.py:229-251)."""
+    from lucivant_sim.circuit import Gate, Reset
+
+    n = circuit.num_qubits
+    rho = np.zeros((1 << n, 1 << n), dtype=np.complex128)
+    rho[0, 0] = 1
+    for op in circuit.operations:
+        if isinstance(op, Gate):
+            rho = apply_dm(rho, op.matrix, op.qubits, n)
+            ch = noise.gate_error(op.name, op.qubits) if noise is not None else None
+            if ch is not None:
+                rho = apply_channel_dm(rho, ch, op.qubits, n)
+        elif isinstance(op, Reset):
+            rho = sum(apply_dm(rho, k, (op.qubit,), n) for k in RESET_KRAUS)
+    return rho
+
+
+def probabilities_dm(rho: np.ndarray) -> np.ndarray:
+    return np.clip(np.diagonal(rho).real, 0.0, None)
+
+
+def expectation_dm(rho: np.ndarray, label: str, n: int) -> float:
+    """Tr(P rho) by left-multiplying the letters (simulator.py:505-516, 530-533)."""
+    t = rho.reshape([2] * (2 * n))
+    for q, letter in enumerate(reversed(label.upper())):
+        if letter != "I":
+            t = contract(t, PAULI[letter], [n - 1 - q])
+    return float(np.trace(t.reshape(1 << n, 1 << n)).real)
+
+
+# ---------------------------------------------------------------------------------- sampling
+
+
+def marginal(probs: np.ndarray, n: int, qubits: list[int]) -> np.ndarray:
+    """engines.py:267-275."""
+    keep = sorted(qubits)
+    if n == 0:
+        return probs
+    t = probs.reshape([2] * n)
+    drop = tuple(n - 1 - q for q in range(n) if q not in keep)
+    return np.asarray(t.sum(axis=drop) if drop else t).reshape(-1)
+
+
+def sample(circuit, probs: np.ndarray, shots: int, noise, rng: np.random.Generator) -> dict[str, int]:  # type: ignore[no-untyped-def]
+    """simulator.py:254-284 (counts from exact probabilities, readout flips)."""
+    pairs = circuit.final_measurements()
+    if not pairs or shots == 0:
+        return {}
+    measured = sorted({q for q, _ in pairs})
+    pos = {q: i for i, q in enumerate(measured)}
+    dist = marginal(probs, circuit.num_qubits, measured)
+    cum = np.cumsum(dist)
+    cum /= cum[-1]
+    idx = np.minimum(np.searchsorted(cum, rng.random(shots), side="right").astype(np.int64), len(dist) - 1)
+    bits = np.zeros(shots, dtype=np.int64)
+    for q, c in pairs:
+        v = (idx >> pos[q]) & 1
+        err = noise.readout_error(q) if noise else None
+        if err is not None and (err.p01 or err.p10):
+            v = v ^ (rng.random(shots) < np.where(v == 1, err.p10, err.p01)).astype(np.int64)
+        bits |= v << c
+    values, counts = np.unique(bits, return_counts=True)
+    return {circuit.format_bits(int(v)): int(k) for v, k in zip(values, counts)}
+
+
+# ------------------------------------------------------------------------------ trajectories
+
+
+class _Shot:
+    """One trajectory's statevector with the reference's per-shot operations."""
+
+    def __init__(self, n: int) -> None:
+        self.n = n
+        self.psi = np.zeros(1 << n, dtype=np.complex128)
+        self.psi[0] = 1
+
+    def apply(self, m: np.ndarray, qubits: tuple[int, ...]) -> None:
+        self.psi = apply_sv(self.psi, m, qubits, self.n)
+
+    def channel(self, ch, qubits: tuple[int, ...], rng: np.random.Generator) -> None:  # type: ignore[no-untyped-def]
+        """engines.py:159-183."""
+        r = rng.random()
+        if ch.mixture is not None:
+            acc = 0.0
+            for p, u in ch.mixture:
+                acc += p
+                if r < acc:
+                    if not np.allclose(u, np.eye(u.shape[0])):
+                        self.apply(u, qubits)
+                    return
+            return
+        acc = 0.0
+        cand = self.psi
+        for k in ch.kraus:
+            cand = apply_sv(self.psi, k, qubits, self.n)
+            w = float(np.vdot(cand, cand).real)
+            acc += w
+            if r < acc and w > 0:
+                self.psi = cand / np.sqrt(w)
+                return
+        norm = float(np.vdot(cand, cand).real)
+        if norm > 0:
+            self.psi = cand / np.sqrt(norm)
+
+    def p1(self, q: int) -> float:
+        one = self.psi[(np.arange(1 << self.n) >> q) & 1 == 1]
+        return float(np.vdot(one, one).real)
+
+    def measure(self, q: int, rng: np.random.Generator) -> int:
+        """engines.py:189-201."""
+        p1 = min(max(self.p1(q), 0.0), 1.0)
+        o = int(rng.random() < p1)
+        p = p1 if o else 1 - p1
+        self.psi[((np.arange(1 << self.n) >> q) & 1) != o] = 0
+        if p > 0:
+            self.psi = self.psi / np.sqrt(p)
+        return o
+
+
+def trajectory_counts(circuit, noise, shots: int, rng: np.random.Generator) -> dict[str, int]:  # type: ignore[no-untyped-def]
+    """The per-shot loop of simulator.py:287-317, 364-371."""
+    from lucivant_sim.circuit import Gate, GlobalPhase, IfElse, Measure, Reset
+
+    tally: dict[int, int] = {}
+    for _ in range(shots):
+        s = _Shot(circuit.num_qubits)
+        bits = [0] * circuit.num_clbits
+
+        def run(ops) -> None:  # type: ignore[no-untyped-def]
+            for op in ops:
+                if isinstance(op, Gate):
+                    s.apply(op.matrix, op.qubits)
+                    ch = noise.gate_error(op.name, op.qubits) if noise else None
+                    if ch is not None:
+                        s.channel(ch, op.qubits, rng)
+                elif isinstance(op, Measure):
+                    o = s.measure(op.qubit, rng)
+                    err = noise.readout_error(op.qubit) if noise else None
+                    if err is not None and rng.random() < (err.p10 if o else err.p01):
+                        o ^= 1
+                    if op.clbit is not None:
+                        bits[op.clbit] = o
+                elif isinstance(op, Reset):
+                    if s.measure(op.qubit, rng):
+                        s.apply(PAULI["X"], (op.qubit,))
+                elif isinstance(op, GlobalPhase):
+                    s.psi = s.psi * np.exp(1j * op.angle)
+                elif isinstance(op, IfElse):
+                    run(op.then if op.condition.holds(bits) else op.otherwise)
+
+        run(circuit.operations)
+        v = sum(b << i for i, b in enumerate(bits))
+        tally[v] = tally.get(v, 0) + 1
+    if not circuit.num_clbits:
+        return {}
+    return {circuit.format_bits(b): c for b, c in sorted(tally.items())}
+
+
+def run_counts(circuit, shots: int, seed: int, noise=None, method: str = "statevector") -> dict[str, int]:  # type: ignore[no-untyped-def]
+    """Counts of ``run`` for an explicit method (simulator.py:320-393), seeded the same way."""
+    rng = np.random.default_rng(seed)
+    if method == "trajectories":
+        return trajectory_counts(circuit, noise, shots, rng)
+    if method == "statevector":
+        probs = probabilities_sv(evolve_sv(circuit))
+    else:
+        probs = probabilities_dm(evolve_dm(circuit, noise))
+    return dict(sorted(sample(circuit, probs, shots, noise, rng).items()))
+
+
+def tvd(counts: dict[str, int], probs: dict[str, float]) -> float:
+    total = sum(counts.values())
+    keys = set(counts) | set(probs)
+    return 0.5 * sum(abs(counts.get(k, 0) / total - probs.get(k, 0.0)) for k in keys)
+
+
+def all_labels(n: int) -> list[str]:
+    return ["".join(s) for s in itertools.product("IXYZ", repeat=n)]

@@ -1,0 +1,258 @@
+"""The public API on the MI355X against the oracle and the reference's own counts.
+
+Amplitudes, probabilities and expectations: within 1e-10 of the oracle (complex128). Counts:
+identical to the reference's golden counts for the same program, noise and seed (the draw
+stream and sampling rule are the reference's; a mismatch could only come from a draw landing
+within rounding of a CDF or branch boundary, which none of these fixtures has). Statistical
+behaviour follows the reference tests (test_sim.py) with their tolerances.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+
+import numpy as np
+import pytest
+
+import lucivant_sim as ls
+from lucivant_sim import _native as nat
+from lucivant_sim import workloads as wl
+from lucivant_sim.trajectories import run_batched, run_sequential
+from oracle import engine as orc
+
+pytestmark = pytest.mark.gpu
+
+BELL = """
+OPENQASM 3;
+include "stdgates.inc";
+qubit[2] q;
+bit[2] c;
+h q[0];
+cx q[0], q[1];
+c = measure q;
+"""
+
+NOISES = {
+    "bell_seed7": None,
+    "c1_12q_seed7": None,
+    "c3_recipe_6q_seed7": "c3",
+    "c4_recipe_9q_seed7": "c4",
+    "teleport_noisy_seed3": "teleport",
+    "teleport_ideal_seed4": None,
+    "bell_noisy_seed42": "bell",
+    "c4_20q_256shots_seed7": "c4",
+}
+
+
+def noise_for(tag: str | None) -> ls.NoiseModel | None:
+    if tag is None:
+        return None
+    if tag == "c3":
+        return wl.noise_c3()
+    if tag == "c4":
+        return wl.noise_c4()
+    if tag == "teleport":
+        return ls.NoiseModel.uniform(single=0.01, two=0.02, readout=0.01)
+    return ls.NoiseModel.uniform(single=0.01, two=0.03, readout=0.01)
+
+
+def tvd(counts: dict[str, int], probs: dict[str, float]) -> float:
+    return orc.tvd(counts, probs)
+
+
+@pytest.mark.parametrize("name", sorted(NOISES))
+def test_counts_and_fingerprints_equal_the_reference(name: str, golden_runs: dict, golden_programs: dict) -> None:
+    if name not in golden_runs:
+        pytest.skip("fixture not generated")
+    rec = golden_runs[name]
+    res = ls.run(golden_programs[name], rec["shots"], seed=rec["seed"], noise=noise_for(NOISES[name]), method=rec["method"])
+    assert res.method == rec["method"]
+    assert res.counts == rec["counts"]
+    assert res.fingerprint == rec["fingerprint"]
+    assert res.counts_digest == rec["counts_digest"]
+    assert ls.verify(rec, golden_programs[name], noise_for(NOISES[name]))
+
+
+def test_c1_statevector_within_1e10_of_the_oracle() -> None:
+    src = wl.config_c1()
+    want = orc.evolve_sv(ls.parse(src))
+    from lucivant_sim import simulator
+
+    for width in (1, 2, 3, 4, 5):
+        old = simulator.FUSION_WIDTH
+        simulator.FUSION_WIDTH = width
+        try:
+            got = ls.statevector(src)
+        finally:
+            simulator.FUSION_WIDTH = old
+        assert np.abs(got - want).max() < 1e-10, width
+    res = ls.run(src, shots=10, seed=1)
+    assert res.statevector is not None and np.abs(res.statevector - want).max() < 1e-10
+
+
+def test_density_matrix_within_1e10_of_the_oracle() -> None:
+    src = wl.config_c3(6, 8)
+    noise = wl.noise_c3()
+    want = orc.evolve_dm(ls.parse(src), noise)
+    got = ls.density_matrix(src, noise)
+    assert np.abs(got - want).max() < 1e-10
+    probs = ls.probabilities(src, noise)
+    p = orc.probabilities_dm(want)
+    assert set(probs) == {format(i, "06b") for i in range(64) if p[i] > 1e-12}
+    assert max(abs(probs[k] - p[int(k, 2)]) for k in probs) < 1e-10
+
+
+def test_wide_gates_and_channels_on_the_density_matrix() -> None:
+    src = 'OPENQASM 3; include "stdgates.inc"; qubit[4] q; h q; ccx q[0], q[1], q[2]; reset q[1]; cswap q[3], q[0], q[2]; rx(0.3) q[1];'
+    noise = ls.NoiseModel().add_gate_error(ls.depolarizing(0.1, 3), gates="ccx").add_gate_error(ls.amplitude_damping(0.2), gates="rx")
+    want = orc.evolve_dm(ls.parse(src), noise)
+    assert np.abs(ls.density_matrix(src, noise) - want).max() < 1e-10
+
+
+def test_expectations_within_1e10() -> None:
+    src = wl.random_layers(10, 6, 21, two_qubit=("cx", "cz"), measure=False)
+    psi = orc.evolve_sv(ls.parse(src))
+    labels = wl.pauli_strings(10, 16, 3)
+    rng = np.random.default_rng(4)
+    obs = {lbl: float(w) for lbl, w in zip(labels, rng.uniform(-1, 1, size=16))}
+    want = sum(w * orc.expectation_sv(psi, lbl, 10) for lbl, w in obs.items())
+    assert abs(ls.expectation(src, obs) - want) < 1e-10
+    noise = wl.noise_c3()
+    rho = orc.evolve_dm(ls.parse(src), noise)
+    want_dm = sum(w * orc.expectation_dm(rho, lbl, 10) for lbl, w in obs.items())
+    assert abs(ls.expectation(src, obs, noise) - want_dm) < 1e-10
+
+
+def test_expectation_with_noise_is_exact() -> None:
+    source = 'OPENQASM 3; include "stdgates.inc"; qubit[2] q; h q[0]; cx q[0], q[1];'
+    noise = ls.NoiseModel().add_gate_error(ls.depolarizing(0.2, 2), gates="cx")
+    assert ls.expectation(source, "XX", noise) == pytest.approx(0.8)
+    assert ls.expectation(source, {"ZZ": 1.0, "ZI": 3.0}) == pytest.approx(1.0)
+    with pytest.raises(ValueError, match="Pauli string of length 2"):
+        ls.expectation(source, "ZZZ")
+
+
+def test_thermal_relaxation_decays_one_to_zero() -> None:
+    noise = ls.NoiseModel().add_gate_error(ls.thermal_relaxation(t1=10.0, t2=8.0, time=3.0), gates="x")
+    probs = ls.probabilities("OPENQASM 3; qubit q; x q;", noise)
+    assert probs["1"] == pytest.approx(math.exp(-0.3))
+
+
+def test_bell_counts_are_balanced_and_correlated() -> None:
+    result = ls.run(BELL, shots=20_000, seed=11)
+    assert set(result.counts) == {"00", "11"}
+    assert tvd(result.counts, {"00": 0.5, "11": 0.5}) < 0.01
+    assert result.method == "statevector" and result.statevector is not None
+
+
+def test_noisy_counts_match_exact_noisy_probabilities() -> None:
+    noise = ls.NoiseModel.uniform(single=0.02, two=0.06)
+    exact = ls.probabilities(BELL, noise)
+    sampled = ls.run(BELL, shots=40_000, seed=5, noise=noise)
+    assert sampled.method == "density_matrix"
+    assert tvd(sampled.counts, exact) < 0.01
+
+
+def test_trajectories_agree_with_the_density_matrix() -> None:
+    noise = ls.NoiseModel().add_gate_error(ls.amplitude_damping(0.15), gates="h").add_gate_error(ls.depolarizing(0.1, 2), gates="cx")
+    exact = ls.probabilities(BELL, noise)
+    traj = ls.run(BELL, shots=20_000, seed=9, noise=noise, method="trajectories")
+    assert tvd(traj.counts, exact) < 0.015
+
+
+def test_readout_error_flips_the_record() -> None:
+    noise = ls.NoiseModel().add_readout_error(ls.ReadoutError(p01=0.1, p10=0.3))
+    result = ls.run("OPENQASM 3; qubit[2] q; bit[2] c; x q[1]; c = measure q;", shots=40_000, seed=2, noise=noise)
+    expected = {"10": 0.9 * 0.7, "00": 0.9 * 0.3, "11": 0.1 * 0.7, "01": 0.1 * 0.3}
+    assert result.method == "statevector"
+    assert tvd(result.counts, expected) < 0.01
+
+
+def test_teleportation_with_mid_circuit_measurement_and_conditions() -> None:
+    theta = 1.234
+    source = f"""
+    OPENQASM 3;
+    include "stdgates.inc";
+    qubit[3] q;
+    bit[2] m;
+    bit[1] out;
+    ry({theta}) q[0];
+    h q[1]; cx q[1], q[2];
+    cx q[0], q[1]; h q[0];
+    m[0] = measure q[0];
+    m[1] = measure q[1];
+    if (m[1]) x q[2];
+    if (m[0] == 1) {{ z q[2]; }}
+    out[0] = measure q[2];
+    """
+    result = ls.run(source, shots=20_000, seed=4)
+    assert result.method == "trajectories"
+    ones = sum(n for k, n in result.counts.items() if k.split(" ")[0] == "1")
+    assert ones / 20_000 == pytest.approx(math.sin(theta / 2) ** 2, abs=0.012)
+
+
+def test_reset_and_if_else_exact_counts() -> None:
+    assert ls.run("OPENQASM 3; qubit q; bit c; x q; reset q; c = measure q;", shots=100, seed=1).counts == {"0": 100}
+    source = """
+    OPENQASM 3; include "stdgates.inc";
+    qubit[2] q; bit[2] c;
+    x q[0];
+    c[0] = measure q[0];
+    if (c[0] == 1) { c[0] = measure q[1]; } else { x q[1]; }
+    c[1] = measure q[1];
+    """
+    assert ls.run(source, shots=50, seed=1).counts == {"00": 50}
+
+
+def test_outcome_probabilities_include_readout_error() -> None:
+    noise = ls.NoiseModel.uniform(two=0.05, readout=0.1)
+    exact = ls.outcome_probabilities(BELL, noise)
+    assert sum(exact.values()) == pytest.approx(1)
+    assert tvd(ls.run(BELL, shots=40_000, seed=8, noise=noise).counts, exact) < 0.01
+    several = "OPENQASM 2.0; qreg q[2]; creg a[1]; creg b[1]; x q[1]; measure q[0] -> a[0]; measure q[1] -> b[0];"
+    assert ls.outcome_probabilities(several) == {"1 0": 1.0}
+
+
+def test_reproducibility_and_verification(tmp_path) -> None:  # type: ignore[no-untyped-def]
+    noise = ls.NoiseModel.uniform(single=0.01, two=0.03, readout=0.01)
+    a = ls.run(BELL, shots=3000, seed=42, noise=noise)
+    b = ls.run(BELL, shots=3000, seed=42, noise=noise)
+    assert a.counts == b.counts and a.fingerprint == b.fingerprint
+    assert ls.run(BELL, shots=3000, seed=43, noise=noise).counts != a.counts
+    saved = tmp_path / "result.json"
+    saved.write_text(json.dumps(a.to_dict()))
+    assert ls.verify(saved, BELL, noise)
+    assert not ls.verify(saved, BELL, None)
+    assert not ls.verify(a.to_dict() | {"counts": {"00": 1}}, BELL, noise)
+
+
+def test_limits_and_physical_qubits() -> None:
+    with pytest.raises(ls.SimulationError, match="up to 26"):
+        ls.run("OPENQASM 3; qubit[30] q; h q; t q;", shots=1)
+    sv = ls.statevector("OPENQASM 3; x $2;")
+    assert sv.shape == (8,) and abs(sv[4]) == pytest.approx(1)
+    assert np.allclose(ls.statevector("OPENQASM 3;\npragma something\nqubit q;\ngphase(pi);\nx q;"), [0, -1])
+
+
+def test_batched_sequential_and_oracle_trajectories_agree_exactly() -> None:
+    src = wl.teleport_chain(9, 3)
+    circuit = ls.parse(src)
+    noise = wl.noise_c4()
+    shots = 300
+    batched = run_batched(circuit, noise, shots, np.random.default_rng(17))
+    sequential = run_sequential(circuit, noise, shots, np.random.default_rng(17))
+    assert [int(v) for v in batched] == sequential
+    want = orc.trajectory_counts(circuit, noise, shots, np.random.default_rng(17))
+    uniq, cnt = np.unique(batched, return_counts=True)
+    assert {circuit.format_bits(int(v)): int(c) for v, c in zip(uniq, cnt)} == want
+
+
+def test_complex64_trajectories_stay_close_to_complex128() -> None:
+    src = wl.teleport_chain(9, 3)
+    circuit = ls.parse(src)
+    noise = wl.noise_c4()
+    a = run_batched(circuit, noise, 2000, np.random.default_rng(3), dtype=nat.C128)
+    b = run_batched(circuit, noise, 2000, np.random.default_rng(3), dtype=nat.C64)
+    # same draws: shots can only differ where a draw lies within float32 rounding of a boundary
+    assert np.mean(a != b) < 0.01

@@ -1,0 +1,212 @@
+"""Parity of each gfx950 kernel, called through the C ABI, against the oracle.
+
+complex128 results must agree with the oracle to 1e-12 absolute (the 1e-10 budget of the
+reference's complex128 contract, with margin); complex64 to 1e-5. Integer/index work (which
+amplitudes move, selectors, skipped states) is exact by construction of these checks.
+"""
+
+from __future__ import annotations
+
+from itertools import permutations
+
+import numpy as np
+import pytest
+import torch
+
+from lucivant_sim import _native as nat
+from lucivant_sim.engines import WORKSPACE, pauli_masks, ptr, stream_handle
+from oracle import engine as orc
+
+pytestmark = pytest.mark.gpu
+
+TOL = {nat.C128: 1e-12, nat.C64: 1e-5}
+NPDT = {nat.C128: np.complex128, nat.C64: np.complex64}
+
+
+def dev(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def states(n: int, batch: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(batch, 2**n)) + 1j * rng.normal(size=(batch, 2**n))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def unitary(k: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.normal(size=(2**k, 2**k)) + 1j * rng.normal(size=(2**k, 2**k)))
+    return q
+
+
+def apply_dense(psi: np.ndarray, n: int, mats: np.ndarray, targets: tuple[int, ...], dtype: int, sel=None) -> np.ndarray:  # type: ignore[no-untyped-def]
+    s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+    m = dev(mats.astype(NPDT[dtype]))
+    sd = dev(np.asarray(sel, dtype=np.int32)) if sel is not None else None
+    batch = psi.shape[0]
+    nat.call("lsim_sv_apply_dense", ptr(s), dtype, n, batch, len(targets), nat.int_array(targets), ptr(m), ptr(sd) if sd is not None else None, stream_handle())
+    return s.cpu().numpy().reshape(batch, -1)
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_dense_every_placement(k: int, dtype: int) -> None:
+    n = 6
+    for i, targets in enumerate(permutations(range(n), k)):
+        psi = states(n, 1, 100 + i)
+        u = unitary(k, 7 + i)
+        got = apply_dense(psi, n, u[None], targets, dtype)[0]
+        want = orc.apply_sv(psi[0], u, targets, n)
+        assert np.abs(got - want).max() < TOL[dtype], targets
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize("k", [4, 5])
+def test_dense_wide_operators(k: int, dtype: int) -> None:
+    n = 12
+    rng = np.random.default_rng(k)
+    placements = [tuple(range(k)), tuple(range(n - k, n)), tuple(reversed(range(k)))]
+    placements += [tuple(int(x) for x in rng.choice(n, size=k, replace=False)) for _ in range(6)]
+    for i, targets in enumerate(placements):
+        psi = states(n, 1, 200 + i)
+        u = unitary(k, 30 + i)
+        got = apply_dense(psi, n, u[None], targets, dtype)[0]
+        want = orc.apply_sv(psi[0], u, targets, n)
+        assert np.abs(got - want).max() < TOL[dtype] * 4, targets
+
+
+def test_dense_non_unitary_and_large_state() -> None:
+    n = 18
+    psi = states(n, 1, 5)
+    lower = np.array([[1, 0, 0, 0], [0.5, 1, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], dtype=np.complex128)
+    for targets in [(0, 17), (17, 3), (9, 10)]:
+        got = apply_dense(psi, n, lower[None], targets, nat.C128)[0]
+        assert np.abs(got - orc.apply_sv(psi[0], lower, targets, n)).max() < 1e-12
+
+
+def test_batched_selection() -> None:
+    n, batch = 8, 7
+    psi = states(n, batch, 9)
+    mats = np.stack([unitary(2, s) for s in range(3)])
+    sel = [0, 2, -1, 1, 1, -1, 0]
+    got = apply_dense(psi, n, mats, (5, 1), nat.C128, sel)
+    for b in range(batch):
+        want = psi[b] if sel[b] < 0 else orc.apply_sv(psi[b], mats[sel[b]], (5, 1), n)
+        assert np.abs(got[b] - want).max() < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+def test_diagonal(dtype: int) -> None:
+    n, batch = 9, 3
+    psi = states(n, batch, 3)
+    rng = np.random.default_rng(0)
+    for targets in [(0,), (8, 2), (4, 0, 7), (1, 2, 3, 4, 5)]:
+        d = np.exp(1j * rng.uniform(0, 6.28, size=(2, 2 ** len(targets))))
+        s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+        dd = dev(d.astype(NPDT[dtype]))
+        sel = dev(np.array([1, -1, 0], dtype=np.int32))
+        nat.call("lsim_sv_apply_diag", ptr(s), dtype, n, batch, len(targets), nat.int_array(targets), ptr(dd), ptr(sel), stream_handle())
+        got = s.cpu().numpy().reshape(batch, -1)
+        for b, m in enumerate([1, -1, 0]):
+            want = psi[b] if m < 0 else orc.apply_sv(psi[b], np.diag(d[m]), targets, n)
+            assert np.abs(got[b] - want).max() < TOL[dtype]
+
+
+def test_scale_basis_probabilities_axpy() -> None:
+    n, batch = 10, 2
+    s = torch.empty(batch << n, dtype=torch.complex128, device="cuda")
+    nat.call("lsim_sv_set_basis", ptr(s), nat.C128, n, batch, 37, stream_handle())
+    host = s.cpu().numpy().reshape(batch, -1)
+    want = np.zeros((batch, 2**n))
+    want[:, 37] = 1
+    assert np.array_equal(host, want)
+    psi = states(n, batch, 1)
+    s = dev(psi.reshape(-1))
+    nat.call("lsim_sv_scale", ptr(s), nat.C128, n, batch, 0.3, -0.4, stream_handle())
+    assert np.abs(s.cpu().numpy().reshape(batch, -1) - psi * (0.3 - 0.4j)).max() < 1e-15
+    p = torch.empty(batch << n, dtype=torch.float64, device="cuda")
+    nat.call("lsim_sv_probabilities", ptr(s), nat.C128, n, batch, ptr(p), stream_handle())
+    assert np.abs(p.cpu().numpy() - np.abs(psi.reshape(-1) * (0.3 - 0.4j)) ** 2).max() < 1e-15
+    y = dev(psi.reshape(-1))
+    x = dev(psi[::-1].reshape(-1).copy())
+    nat.call("lsim_sv_axpy", ptr(y), ptr(x), nat.C128, n, batch, 0.5, 2.0, stream_handle())
+    assert np.abs(y.cpu().numpy() - (psi.reshape(-1) + (0.5 + 2j) * psi[::-1].reshape(-1))).max() < 1e-14
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize("targets", [(0,), (9,), (3, 7), (7, 3), (0, 9)])
+def test_reduced_density_matrix(targets: tuple[int, ...], dtype: int) -> None:
+    n, batch = 10, 3
+    psi = states(n, batch, 4)
+    s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+    k = len(targets)
+    out = torch.empty(batch << (2 * k), dtype=torch.complex128, device="cuda")
+    ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(n, batch, k))
+    nat.call("lsim_sv_reduced_dm", ptr(s), dtype, n, batch, k, nat.int_array(targets), ptr(out), ptr(ws), stream_handle())
+    got = out.cpu().numpy().reshape(batch, 2**k, 2**k)
+    for b in range(batch):
+        v = psi[b].astype(NPDT[dtype]).astype(np.complex128)
+        full = np.outer(v, v.conj())
+        # reduce with the oracle's index convention: row index bits = targets (first = MSB)
+        idx = np.arange(2**n)
+        r = np.zeros(2**n, dtype=np.int64)
+        for i, q in enumerate(targets):
+            r |= ((idx >> q) & 1) << (k - 1 - i)
+        rest = idx & ~sum(1 << q for q in targets)
+        want = np.zeros((2**k, 2**k), dtype=np.complex128)
+        same = rest[:, None] == rest[None, :]
+        np.add.at(want, (r[:, None].repeat(2**n, 1)[same], r[None, :].repeat(2**n, 0)[same]), full[same])
+        assert np.abs(got[b] - want).max() < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+def test_pauli_expectations_statevector(dtype: int) -> None:
+    n = 3
+    psi = states(n, 2, 8)
+    labels = orc.all_labels(n)
+    s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+    masks = [pauli_masks(lbl) for lbl in labels]
+    out = torch.empty(2 * len(labels), dtype=torch.float64, device="cuda")
+    ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(n, 2, len(labels)))
+    nat.call(
+        "lsim_sv_pauli_expect",
+        ptr(s), dtype, n, 2, len(labels),
+        nat.ull_array([m[0] for m in masks]), nat.ull_array([m[1] for m in masks]),
+        ptr(out), ptr(ws), stream_handle(),
+    )  # fmt: skip
+    got = out.cpu().numpy().reshape(2, -1)
+    for b in range(2):
+        v = psi[b].astype(NPDT[dtype]).astype(np.complex128)
+        for t, lbl in enumerate(labels):
+            assert abs(got[b, t] - orc.expectation_sv(v, lbl, n)) < TOL[dtype], lbl
+
+
+def test_pauli_expectations_wide() -> None:
+    from lucivant_sim import workloads as wl
+    from lucivant_sim.engines import Statevector
+
+    n = 14
+    psi = states(n, 1, 2)[0]
+    labels = wl.pauli_strings(n, 16, 5) + ["I" * n, "Z" * n]
+    sv = Statevector(n)
+    sv.tensor = psi
+    got = sv.expectations(labels)
+    for t, lbl in enumerate(labels):
+        assert abs(got[t] - orc.expectation_sv(psi, lbl, n)) < 1e-12, lbl
+
+
+def test_density_matrix_diagonal_and_pauli() -> None:
+    from lucivant_sim.engines import DensityMatrix
+
+    n = 3
+    rng = np.random.default_rng(6)
+    a = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    rho = a @ a.conj().T
+    rho /= np.trace(rho)
+    dm = DensityMatrix(n)
+    dm.tensor = rho
+    assert np.abs(dm.probabilities() - orc.probabilities_dm(rho)).max() < 1e-15
+    labels = orc.all_labels(n)
+    got = dm.expectations(labels)
+    for t, lbl in enumerate(labels):
+        assert abs(got[t] - orc.expectation_dm(rho, lbl, n)) < 1e-12, lbl

@@ -1,0 +1,67 @@
+"""Full-size configurations on the MI355X, checked through size-independent properties (the
+oracle would take tens of minutes at these sizes): unitarity round trips, norms, traces,
+probability sums and expectations of diagonal observables against the probabilities."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import lucivant_sim as ls
+from lucivant_sim import workloads as wl
+from lucivant_sim.engines import DensityMatrix, Statevector
+from lucivant_sim.plan import plan_density, plan_statevector
+
+pytestmark = pytest.mark.gpu
+
+
+def _inverse(circuit: ls.Circuit) -> ls.Circuit:
+    from lucivant_sim.circuit import Gate
+
+    ops = tuple(Gate(g.name + "_dg", g.qubits, np.asarray(g.matrix).conj().T) for g in reversed(circuit.operations) if isinstance(g, Gate))
+    return ls.Circuit(circuit.num_qubits, 0, ops)
+
+
+def test_26_qubit_circuit_round_trip_and_observables() -> None:
+    src, labels = wl.config_c2()
+    c = ls.parse(src)
+    sv = Statevector(26)
+    sv.run(plan_statevector(c.operations, 26, 4))
+    probs = sv.probabilities()
+    assert abs(probs.sum() - 1) < 1e-10
+    # <Z...Z> on a diagonal string equals the signed sum of probabilities
+    zlabel = "Z" * 13 + "I" * 13
+    idx = np.arange(2**26, dtype=np.int64)
+    sign = 1 - 2 * (np.bitwise_count((idx & ((1 << 13) - 1)).astype(np.uint64)) & 1).astype(np.float64)
+    e = sv.expectations([zlabel] + labels)
+    assert abs(e[0] - float(np.dot(sign, probs))) < 1e-10
+    assert np.all(np.abs(e[1:]) <= 1 + 1e-12)
+    sv.run(plan_statevector(_inverse(c).operations, 26, 5))
+    amp0 = complex(sv.buffer[0].item())
+    rest = float(torch.linalg.vector_norm(sv.buffer[1:]).item())
+    assert abs(amp0 - 1) < 1e-10 and rest < 1e-10
+
+
+def test_13_qubit_density_matrix_trace_and_round_trip() -> None:
+    src = wl.config_c3(13, 6)
+    c = ls.parse(src)
+    dm = DensityMatrix(13)
+    items = plan_density(c.operations, 13, wl.noise_c3())
+    dm.run([p for p in items if not isinstance(p, tuple)])
+    p = dm.probabilities()
+    assert abs(p.sum() - 1) < 1e-10 and p.min() >= 0
+    ideal = DensityMatrix(13)
+    ideal.run(plan_density(c.operations, 13, None))
+    ideal.run(plan_density(_inverse(c).operations, 13, None))
+    d = ideal.probabilities()
+    assert abs(d[0] - 1) < 1e-10
+
+
+def test_20_qubit_trajectory_batch_is_normalised_and_reproducible() -> None:
+    from lucivant_sim.trajectories import run_batched
+
+    c = ls.parse(wl.teleport_chain(20, 3))
+    a = run_batched(c, wl.noise_c4(), 64, np.random.default_rng(5))
+    b = run_batched(c, wl.noise_c4(), 64, np.random.default_rng(5), batch_bytes=16 << 20 << 4)
+    assert np.array_equal(a, b)

@@ -1,0 +1,300 @@
+"""Host logic on the CPU: the OpenQASM reader, noise models, method selection, digests and
+fingerprints against the reference's own outputs, and the fusion planner against the oracle."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import lucivant_sim as ls
+from lucivant_sim import workloads as wl
+from lucivant_sim.plan import Pass, plan_density, plan_statevector
+from lucivant_sim.simulator import Result, choose_method
+from lucivant_sim.trajectories import _kraus_choice, _mixture_choice, compile_program, draw_count
+from oracle import engine as orc
+
+# ------------------------------------------------------------------------------------ reader
+
+
+def test_reader_digests_and_labels_equal_the_reference(golden_host: dict) -> None:
+    for name, rec in golden_host["reader"].items():
+        c = ls.parse(rec["source"], rec["params"])
+        assert c.digest() == rec["digest"], name
+        assert (c.num_qubits, c.num_clbits) == (rec["num_qubits"], rec["num_clbits"]), name
+        labels = [[getattr(op, "name", op.kind), list(getattr(op, "qubits", ()))] for op in c.operations]
+        assert labels == rec["labels"], name
+
+
+def test_config_programs_have_the_reference_digests(golden_host: dict) -> None:
+    src, labels = wl.config_c2()
+    assert ls.parse(src).digest() == golden_host["config_digests"]["c2"]
+    assert labels == golden_host["c2_labels"]
+    assert ls.parse(wl.config_c3()).digest() == golden_host["config_digests"]["c3"]
+    assert ls.parse(wl.teleport_chain(20, 3)).digest() == golden_host["config_digests"]["c4"]
+
+
+def test_broadcast_slices_and_sets() -> None:
+    c = ls.parse("OPENQASM 3; qubit[4] q; qubit[4] r; h q; cx q, r; x q[1:2]; z q[{0, 3}]; y q[-1];")
+    names = [(op.name, op.qubits) for op in c.operations]
+    assert names[:4] == [("h", (0,)), ("h", (1,)), ("h", (2,)), ("h", (3,))]
+    assert names[4:8] == [("cx", (0, 4)), ("cx", (1, 5)), ("cx", (2, 6)), ("cx", (3, 7))]
+    assert names[8:] == [("x", (1,)), ("x", (2,)), ("z", (0,)), ("z", (3,)), ("y", (3,))]
+
+
+def test_missing_and_unknown_inputs_are_reported() -> None:
+    with pytest.raises(ls.QasmError, match="needs a value for its input 'theta'"):
+        ls.parse("OPENQASM 3; input float theta; qubit q; rx(theta) q;")
+    with pytest.raises(ls.QasmError, match="no input named phi"):
+        ls.parse("OPENQASM 3; qubit q; h q;", params={"phi": 1.0})
+
+
+@pytest.mark.parametrize(
+    ("source", "message"),
+    [
+        ("OPENQASM 3; qubit q; foo q;", "line 1, column 22: unknown gate 'foo'"),
+        ("OPENQASM 3;\nqubit[2] q;\ncx q[0], q[0];", "line 3, column 1: cx is given the same qubit twice"),
+        ("OPENQASM 3; qubit[2] q; h q[2];", "index 2 is outside 'q'"),
+        ("OPENQASM 3; qubit q; for uint i in [0:3] { h q; }", "'for' isn't supported"),
+        ("OPENQASM 3; qubit q; rx q;", "rx takes 1 parameter"),
+        ("OPENQASM 3; qubit[2] q; qubit[3] r; cx q, r;", "different sizes"),
+        ('OPENQASM 2.0; include "mine.inc"; qreg q[1];', "only qelib1.inc and stdgates.inc"),
+        ("OPENQASM 4; qubit q;", "only OpenQASM 2 and 3"),
+        ("OPENQASM 3; qubit q; h q", "expected ';'"),
+    ],
+)
+def test_errors_point_at_the_problem(source: str, message: str) -> None:
+    with pytest.raises(ls.QasmError, match=message.replace("(", r"\(").replace("[", r"\[")):
+        ls.parse(source)
+
+
+def test_includes_resolve_only_inside_the_folder(tmp_path) -> None:  # type: ignore[no-untyped-def]
+    (tmp_path / "mine.inc").write_text("gate flip a { x a; }")
+    (tmp_path / "main.qasm").write_text('OPENQASM 3; include "mine.inc"; qubit q; flip q;')
+    c = ls.load(tmp_path / "main.qasm")
+    assert np.allclose(orc.evolve_sv(c), [0, 1])
+    (tmp_path / "bad.qasm").write_text('OPENQASM 3; include "../../etc/passwd"; qubit q;')
+    with pytest.raises(ls.QasmError, match="outside"):
+        ls.load(tmp_path / "bad.qasm")
+
+
+# ------------------------------------------------------------------------------------- noise
+
+
+def test_noise_digests_equal_the_reference(golden_host: dict) -> None:
+    models = {
+        "c3": wl.noise_c3(),
+        "c4": wl.noise_c4(),
+        "uniform_bell": ls.NoiseModel.uniform(single=0.01, two=0.03, readout=0.01),
+        "uniform_teleport": ls.NoiseModel.uniform(single=0.01, two=0.02, readout=0.01),
+        "calibration": ls.NoiseModel.from_calibration(
+            {
+                "device": "example_device",
+                "single_qubit_error": 0.001,
+                "two_qubit_error": 0.01,
+                "readout_error": 0.02,
+                "qubits": [{"qubit": 1, "single_qubit_error": 0.3, "readout_error": None}],
+                "couplers": [{"qubits": [0, 1], "error": 0.05}],
+            }
+        ),
+    }
+    for name, model in models.items():
+        assert model.digest() == golden_host["noise_digests"][name], name
+
+
+def test_calibration_noise_prefers_per_qubit_values() -> None:
+    cal = {
+        "device": "example_device",
+        "single_qubit_error": 0.001,
+        "two_qubit_error": 0.01,
+        "readout_error": 0.02,
+        "qubits": [{"qubit": 1, "single_qubit_error": 0.3, "readout_error": None}],
+        "couplers": [{"qubits": [0, 1], "error": 0.05}],
+    }
+    model = ls.NoiseModel.from_calibration(cal)
+    assert model.gate_error("sx", (1,)).params[0] == pytest.approx(0.3 * 2)  # type: ignore[union-attr]
+    assert model.gate_error("sx", (0,)).params[0] == pytest.approx(0.001 * 2)  # type: ignore[union-attr]
+    assert model.gate_error("cx", (1, 0)).params[0] == pytest.approx(0.05 * 4 / 3)  # type: ignore[union-attr]
+    assert model.gate_error("rz", (0,)) is None
+    assert model.readout_error(1) == ls.ReadoutError(0.02, 0.02)
+
+
+def test_invalid_channels() -> None:
+    with pytest.raises(ValueError, match="T2 can't exceed"):
+        ls.thermal_relaxation(1.0, 3.0, 0.1)
+    with pytest.raises(ValueError, match="between 0 and"):
+        ls.bit_flip(1.5)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_superoperators_equal_the_oracle_channel(k: int) -> None:
+    n = 3
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    rho = a @ a.conj().T
+    rho /= np.trace(rho)
+    qubits = (2,) if k == 1 else (2, 0)
+    channels = [ls.depolarizing(0.3, k), ls.Channel("generic", k, ls.depolarizing(0.3, k).kraus)]
+    if k == 1:
+        channels += [ls.amplitude_damping(0.2), ls.thermal_relaxation(120, 90, 0.05), ls.phase_damping(0.4)]
+    for ch in channels:
+        want = orc.apply_channel_dm(rho, ch, qubits, n)
+        bits = tuple(n + q for q in qubits) + qubits
+        got = orc.apply_sv(rho.reshape(-1), ch.superoperator(), bits, 2 * n).reshape(8, 8)
+        assert np.allclose(got, want, atol=1e-13), ch.name
+
+
+# ---------------------------------------------------------------------------- method choice
+
+
+def test_method_choice_equals_the_reference(golden_host: dict) -> None:
+    g = golden_host["methods"]
+    c2_src, _ = wl.config_c2()
+    cases = {
+        "c1": (wl.config_c1(), None),
+        "c2": (c2_src, None),
+        "c3_automatic": (wl.config_c3(), wl.noise_c3()),
+        "c4": (wl.teleport_chain(20, 3), wl.noise_c4()),
+        "clifford_12q": ("OPENQASM 3; qubit[12] q; h q; cx q[0], q[1]; s q[3];", None),
+        "clifford_12q_noisy": ("OPENQASM 3; qubit[12] q; h q; cx q[0], q[1];", ls.NoiseModel.uniform(single=0.01)),
+        "clifford_12q_ampdamp": ("OPENQASM 3; qubit[12] q; h q;", ls.NoiseModel().add_gate_error(ls.amplitude_damping(0.1))),
+        "t_12q": ("OPENQASM 3; qubit[12] q; h q; t q[0];", None),
+        "noisy_11q": ("OPENQASM 3; qubit[11] q; h q;", ls.NoiseModel.uniform(single=0.01)),
+        "noisy_10q": ("OPENQASM 3; qubit[10] q; h q;", ls.NoiseModel.uniform(single=0.01)),
+    }
+    for name, (src, noise) in cases.items():
+        assert choose_method(ls.parse(src), noise) == g[name], name
+
+
+def test_explicit_method_refusals() -> None:
+    bell = ls.parse('OPENQASM 3; include "stdgates.inc"; qubit[2] q; bit[2] c; h q[0]; cx q[0], q[1]; c = measure q;')
+    with pytest.raises(ls.SimulationError, match="statevector method"):
+        choose_method(bell, ls.NoiseModel.uniform(single=0.01), "statevector")
+    with pytest.raises(ls.SimulationError, match="up to 13"):
+        choose_method(ls.parse("OPENQASM 3; qubit[14] q; h q;"), None, "density_matrix")
+    with pytest.raises(ValueError, match="method must be one of"):
+        choose_method(bell, None, "magic")
+
+
+def test_fingerprints_equal_the_reference(golden_runs: dict) -> None:
+    for name, rec in golden_runs.items():
+        r = Result(
+            counts=rec["counts"],
+            shots=rec["shots"],
+            seed=rec["seed"],
+            method=rec["method"],
+            num_qubits=rec["num_qubits"],
+            num_clbits=rec["num_clbits"],
+            circuit_digest=rec["circuit_digest"],
+            noise_digest=rec["noise_digest"],
+            version=ls.__version__,
+            elapsed_seconds=0.0,
+        )
+        assert r.fingerprint == rec["fingerprint"], name
+        assert r.counts_digest == rec["counts_digest"], name
+
+
+def test_program_digests_equal_the_golden_runs(golden_runs: dict, golden_programs: dict) -> None:
+    for name, rec in golden_runs.items():
+        assert ls.parse(golden_programs[name]).digest() == rec["circuit_digest"], name
+
+
+# ------------------------------------------------------------------------------------ planner
+
+
+def _run_passes_on_cpu(psi: np.ndarray, passes: list[Pass], nbits: int) -> np.ndarray:
+    for p in passes:
+        if p.kind == "scale":
+            psi = psi * p.matrix[0]
+        elif p.kind == "diag":
+            psi = orc.apply_sv(psi, np.diag(p.matrix), p.bits, nbits)
+        else:
+            psi = orc.apply_sv(psi, p.matrix, p.bits, nbits)
+    return psi
+
+
+@pytest.mark.parametrize("width", [1, 2, 3, 4, 5])
+def test_fused_statevector_passes_equal_sequential_gates(width: int) -> None:
+    src = wl.random_layers(7, 6, 11, two_qubit=("cx", "cz", "swap", "iswap"), measure=True)
+    src = src.replace("c = measure q;", "gphase(0.3); ccx q[0], q[3], q[6]; c = measure q;")
+    c = ls.parse(src)
+    passes = plan_statevector(c.operations, c.num_qubits, width)
+    assert all(p.k <= max(width, 3) for p in passes)
+    psi0 = np.zeros(2**7, dtype=np.complex128)
+    psi0[0] = 1
+    assert np.allclose(_run_passes_on_cpu(psi0, passes, 7), orc.evolve_sv(c), atol=1e-12)
+    assert len(passes) < sum(1 for op in c.operations if isinstance(op, ls.circuit.Gate))
+
+
+@pytest.mark.parametrize("width", [1, 2])
+def test_fused_density_passes_equal_the_oracle(width: int) -> None:
+    n = 4
+    src = wl.random_layers(n, 4, 5, two_qubit=("cx", "cz")) + "reset q[2];\nccx q[0], q[1], q[3];\n"
+    c = ls.parse(src)
+    noise = wl.noise_c3().add_gate_error(ls.depolarizing(0.05, 3), gates="ccx")
+    items = plan_density(c.operations, n, noise, width)
+    rho = np.zeros(4**n, dtype=np.complex128)
+    rho[0] = 1
+    for item in items:
+        if isinstance(item, Pass):
+            rho = _run_passes_on_cpu(rho, [item], 2 * n)
+        else:
+            _, (ch, qubits) = item
+            rho = orc.apply_channel_dm(rho.reshape(2**n, 2**n), ch, qubits, n).reshape(-1)
+    want = orc.evolve_dm(c, noise)
+    assert np.allclose(rho.reshape(2**n, 2**n), want, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------ trajectories
+
+
+def test_draw_ledger() -> None:
+    c4 = ls.parse(wl.teleport_chain(20, 3))
+    noise = wl.noise_c4()
+    steps, d, nflags = compile_program(c4, noise)
+    gates = sum(1 for s in steps if s.kind == "gate" and not s.guard)
+    # every unconditioned gate is noisy (depolarising or amplitude damping), plus 6 measurements,
+    # 3 final measurements, and per if/else one noisy one-qubit gate
+    assert d == gates + 6 + 3 + 6
+    assert nflags == 6
+    assert draw_count(c4.operations, noise) == d
+    teleport = ls.parse(
+        'OPENQASM 3; include "stdgates.inc"; qubit[3] q; bit[2] m; h q[0]; m[0] = measure q[0]; if (m[0]) x q[2];'
+    )
+    assert draw_count(teleport.operations, None) == 1
+    assert draw_count(teleport.operations, ls.NoiseModel.uniform(single=0.01)) is None
+
+
+def test_batched_decisions_equal_the_scalar_rule() -> None:
+    rng = np.random.default_rng(1)
+    ch = ls.depolarizing(0.4, 2)
+    r = rng.random(2000)
+    sel, table = _mixture_choice(ch, r)
+    for i in range(len(r)):
+        acc, want = 0.0, -1
+        for p, u in ch.mixture:  # type: ignore[union-attr]
+            acc += p
+            if r[i] < acc:
+                want = -1 if np.allclose(u, np.eye(4)) else 1
+                if want == 1:
+                    got = table[sel[i]]
+                    assert np.array_equal(got, u)
+                break
+        assert (sel[i] >= 0) == (want == 1)
+    amp = ls.thermal_relaxation(120, 90, 30.0)
+    a = rng.normal(size=(500, 2, 2)) + 1j * rng.normal(size=(500, 2, 2))
+    rho = a @ np.conj(np.transpose(a, (0, 2, 1)))
+    rho /= np.trace(rho, axis1=1, axis2=2)[:, None, None]
+    r = rng.random(500)
+    mats, sel = _kraus_choice(amp, rho, r)
+    for i in range(500):
+        acc, pick, w = 0.0, None, 0.0
+        for j, k in enumerate(amp.kraus):
+            w = float(np.trace(k @ rho[i] @ k.conj().T).real)
+            acc += w
+            if r[i] < acc and w > 0:
+                pick = (j, w)
+                break
+        if pick is None:
+            pick = (len(amp.kraus) - 1, w)
+        assert sel[i] == i
+        assert np.allclose(mats[i], amp.kraus[pick[0]] / np.sqrt(pick[1]))
