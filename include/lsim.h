@@ -108,6 +108,53 @@ int lsim_dm_pauli_expect(const void* rho, int dtype, int n, long long batch, int
                          const unsigned long long* xmask, const unsigned long long* zmask,
                          double* out, double* workspace, void* stream);
 
+/* ------------------------------------------------------------------------------------------
+ * Whole-circuit programs on LDS-resident states (one workgroup per circuit instance).
+ *
+ * Replaces, for states of up to lsim_program_max_qubits() qubits (13 for complex128 on
+ * gfx950's 160 KiB of LDS), the per-shot loop of the trajectory method (simulator.py:287-317,
+ * 364-371, with engines.py:159-205 per operation) and the per-operation loop of static
+ * evolution (simulator.py:229-251) for a batch of circuits. Every instance starts in |0...0>.
+ *
+ * An op runs only when (flags & guard_mask) == guard_value; COND sets flag bit `flag` to
+ * ((clbits & cmask) == cpattern) != negate. Matrices are complex128 in `mats` (also for
+ * LSIM_C64 states); `reals` holds float64 side data (see kinds below).
+ */
+#define LSIM_OP_DENSE 0   /* mats[mat]: 2^k x 2^k matrix */
+#define LSIM_OP_DIAG 1    /* mats[mat]: 2^k diagonal */
+#define LSIM_OP_MIX 2     /* reals[aux..aux+nb): cumulative probabilities; reals[aux+nb+j]: the mats
+                             offset of branch j or -1 for the identity; one draw */
+#define LSIM_OP_KRAUS 3   /* k <= 2; mats[mat + j*4^k]: the nb Kraus operators; one draw */
+#define LSIM_OP_MEASURE 4 /* qubit t[0] into clbit (or none if clbit < 0); one draw, plus one more
+                             for a readout flip when aux >= 0 (reals[aux] = p01, reals[aux+1] = p10) */
+#define LSIM_OP_RESET 5   /* qubit t[0]; one draw */
+#define LSIM_OP_COND 6    /* sets flag bit `flag` */
+#define LSIM_OP_PHASE 7   /* mats[mat]: global phase factor */
+
+typedef struct lsim_prog_op {
+  unsigned char kind, k, flag, negate;
+  unsigned char t[5]; /* target qubits, first = most significant matrix bit */
+  unsigned char pad[3];
+  int mat, nb, aux, draw, clbit;
+  unsigned int guard_mask, guard_value;
+  unsigned long long cmask, cpattern;
+} lsim_prog_op;
+
+/* Widest state the program kernel holds in one workgroup's LDS for `dtype` (0 on error). */
+int lsim_program_max_qubits(int dtype);
+
+/* Run `instances` programs: instance b executes ops[op_start[b] .. op_start[b]+op_count[b])
+ * with draws from draws[draw_start[b] ..] (draws/draw_start may be NULL when no op draws).
+ * Optional outputs (NULL to skip): clbits[b]; probs[b][2^n] = |psi|^2; state[b][2^n] in dtype;
+ * expect[b][t] = <psi_b|P_bt|psi_b> for the nterms Pauli strings of each instance, given by the
+ * DEVICE arrays pauli_x[b][t] / pauli_z[b][t] (bit masks as for lsim_sv_pauli_expect). */
+int lsim_program_run(int dtype, int n, long long instances, const lsim_prog_op* ops,
+                     const int* op_start, const int* op_count, const void* mats,
+                     const double* reals, const double* draws, const int* draw_start,
+                     unsigned long long* clbits, double* probs, void* state, int nterms,
+                     const unsigned long long* pauli_x, const unsigned long long* pauli_z,
+                     double* expect, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

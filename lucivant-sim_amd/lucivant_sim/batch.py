@@ -1,0 +1,156 @@
+"""Many circuits at once: every small circuit of a batch in one launch of the LDS program kernel.
+
+:func:`run_many` returns exactly what ``[run(p, shots, seed=s, noise=noise) for p, s in ...]``
+returns (same counts, fingerprints and digests; only ``elapsed_seconds`` differs), and
+:func:`expectation_many` what ``[expectation(p, obs) ...]`` returns. Circuits that fit the
+program kernel (statevector or trajectory method, at most :func:`program.max_qubits` qubits) of
+the same width run together, one workgroup per circuit (per shot for trajectories); anything else
+runs through the single-circuit path.
+"""
+
+from __future__ import annotations
+
+import secrets
+import time
+from collections.abc import Mapping, Sequence
+from typing import Any
+
+import numpy as np
+
+from lucivant_sim import __version__
+from lucivant_sim import _native as nat
+from lucivant_sim import program as pg
+from lucivant_sim.circuit import Circuit
+from lucivant_sim.engines import SimulationError, pauli_masks
+from lucivant_sim.noise import NoiseModel
+from lucivant_sim.simulator import MAX_SHOTS, Result, _sample, _static, choose_method, expectation, run, to_circuit
+
+
+def _result(circuit: Circuit, counts: dict[str, int], shots: int, seed: int, method: str, noise: NoiseModel | None, t0: float, vector=None) -> Result:  # type: ignore[no-untyped-def]
+    return Result(
+        counts=dict(sorted(counts.items())),
+        shots=shots,
+        seed=seed,
+        method=method,
+        num_qubits=circuit.num_qubits,
+        num_clbits=circuit.num_clbits,
+        circuit_digest=circuit.digest(),
+        noise_digest=noise.digest() if noise is not None and not noise.is_empty() else None,
+        version=__version__,
+        elapsed_seconds=round(time.perf_counter() - t0, 6),
+        statevector=vector,
+    )
+
+
+def run_many(
+    programs: Sequence[Any],
+    shots: int = 1024,
+    *,
+    seeds: Sequence[int] | None = None,
+    noise: NoiseModel | None = None,
+    method: str = "automatic",
+    params: Mapping[str, float] | None = None,
+    dtype: int = nat.C128,
+    keep_statevector: bool = False,
+) -> list[Result]:
+    """``run`` for every program; same-width small circuits share one LDS-program launch."""
+    if not 0 <= shots <= MAX_SHOTS:
+        raise ValueError(f"shots must be between 0 and {MAX_SHOTS:,}")
+    t0 = time.perf_counter()
+    circuits = [to_circuit(p, params) for p in programs]
+    seeds = [secrets.randbelow(2**63) for _ in circuits] if seeds is None else [int(s) for s in seeds]
+    if len(seeds) != len(circuits) or any(s < 0 for s in seeds):
+        raise ValueError("give one non-negative seed per program")
+    methods = [choose_method(c, noise, method) for c in circuits]
+    results: list[Result | None] = [None] * len(circuits)
+    groups: dict[tuple[str, int], list[int]] = {}
+    for i, (c, m) in enumerate(zip(circuits, methods)):
+        fits = 1 <= c.num_qubits <= pg.max_qubits(dtype)
+        if fits and m in ("statevector", "trajectories"):
+            groups.setdefault((m, c.num_qubits), []).append(i)
+        else:
+            results[i] = run(c, shots, seed=seeds[i], noise=noise, method=m)
+    for (m, _n), idx in groups.items():
+        if m == "statevector":
+            progs = [pg.compile_static(circuits[i]) for i in idx]
+            out = pg.run_programs(progs, dtype=dtype, want_probs=True, want_state=keep_statevector)
+            assert out.probs is not None
+            for j, i in enumerate(idx):
+                rng = np.random.default_rng(seeds[i])
+                counts = _sample(circuits[i], out.probs[j], shots, noise, rng)
+                vec = out.state[j].astype(np.complex128) if out.state is not None else None
+                results[i] = _result(circuits[i], counts, shots, seeds[i], m, noise, t0, vec)
+        else:
+            _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0)
+    return [r for r in results if r is not None]
+
+
+def _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0) -> None:  # type: ignore[no-untyped-def]
+    from lucivant_sim.trajectories import run_trajectories
+
+    progs, ok = [], []
+    for i in idx:
+        try:
+            progs.append(pg.compile_trajectory(circuits[i], noise))
+            ok.append(i)
+        except pg.Unsupported:
+            results[i] = run(circuits[i], shots, seed=seeds[i], noise=noise, method="trajectories")
+    if not ok or shots == 0:
+        for i in ok:
+            results[i] = _result(circuits[i], {}, shots, seeds[i], "trajectories", noise, t0)
+        return
+    width = max(p.draws for p in progs)
+    draws = np.zeros((len(ok) * shots, max(width, 1)))
+    for j, (i, p) in enumerate(zip(ok, progs)):
+        if p.draws:
+            draws[j * shots : (j + 1) * shots, : p.draws] = np.random.default_rng(seeds[i]).random(shots * p.draws).reshape(shots, p.draws)
+    out = pg.run_programs(progs, repeat=shots, dtype=dtype, draws=draws, want_clbits=True)
+    assert out.clbits is not None
+    for j, i in enumerate(ok):
+        c = circuits[i]
+        bits = out.clbits[j * shots : (j + 1) * shots]
+        uniq, cnt = np.unique(bits, return_counts=True)
+        counts = {c.format_bits(int(v)): int(k) for v, k in zip(uniq, cnt)} if c.num_clbits else {}
+        results[i] = _result(c, counts, shots, seeds[i], "trajectories", noise, t0)
+    del run_trajectories
+
+
+def expectation_many(
+    programs: Sequence[Any],
+    observables: Sequence[str | Mapping[str, float]],
+    *,
+    params: Mapping[str, float] | None = None,
+    dtype: int = nat.C128,
+) -> list[float]:
+    """``expectation`` of each ideal program's observable; small circuits share one launch."""
+    circuits = [to_circuit(p, params) for p in programs]
+    if len(observables) != len(circuits):
+        raise ValueError("give one observable per program")
+    out: list[float | None] = [None] * len(circuits)
+    groups: dict[tuple[int, int], list[int]] = {}
+    terms: list[dict[str, float]] = []
+    for i, (c, obs) in enumerate(zip(circuits, observables)):
+        _static(c, "an expectation value")
+        t = {obs: 1.0} if isinstance(obs, str) else dict(obs)
+        for label in t:
+            if len(label) != c.num_qubits or set(label.upper()) - set("IXYZ"):
+                raise ValueError(f"{label!r} must be a Pauli string of length {c.num_qubits} (qubit 0 rightmost)")
+        terms.append(t)
+        if 1 <= c.num_qubits <= pg.max_qubits(dtype) and 0 < len(t) <= nat.MAX_TERMS:
+            groups.setdefault((c.num_qubits, len(t)), []).append(i)
+        else:
+            out[i] = expectation(c, t)
+    for (_n, _nt), idx in groups.items():
+        progs = [pg.compile_static(circuits[i]) for i in idx]
+        masks = [[pauli_masks(lbl) for lbl in terms[i]] for i in idx]
+        res = pg.run_programs(progs, dtype=dtype, pauli=masks)
+        assert res.expect is not None
+        for j, i in enumerate(idx):
+            total = 0.0
+            for w, v in zip(terms[i].values(), res.expect[j]):
+                total += w * float(v)
+            out[i] = total
+    return [float(v) for v in out]  # type: ignore[arg-type]
+
+
+__all__ = ["SimulationError", "expectation_many", "run_many"]

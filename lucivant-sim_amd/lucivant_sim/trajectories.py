@@ -330,6 +330,34 @@ def run_batched(
     return np.concatenate(out) if out else np.zeros(0, dtype=np.int64)
 
 
+#: Shots per program launch (bounds the draw buffer and output sizes of one launch).
+LDS_CHUNK = 1 << 16
+
+
+def run_lds(
+    circuit: Circuit, noise: NoiseModel | None, shots: int, rng: np.random.Generator, *, dtype: int = nat.C128
+) -> np.ndarray | None:
+    """All shots as LDS programs, one workgroup per shot, when the state fits a workgroup's LDS
+    and the circuit fits the program kernel; None otherwise (the caller uses :func:`run_batched`).
+    Draws are taken from ``rng`` exactly as :func:`run_batched` takes them."""
+    from lucivant_sim import program as pg
+
+    if circuit.num_qubits < 1 or circuit.num_qubits > pg.max_qubits(dtype) or circuit.num_clbits > 62:
+        return None
+    try:
+        prog = pg.compile_trajectory(circuit, noise)
+    except pg.Unsupported:
+        return None
+    out = []
+    for start in range(0, shots, LDS_CHUNK):
+        b = min(LDS_CHUNK, shots - start)
+        draws = rng.random(b * prog.draws).reshape(b, prog.draws) if prog.draws else None
+        res = pg.run_programs([prog], repeat=b, dtype=dtype, draws=draws, want_clbits=True)
+        assert res.clbits is not None
+        out.append(res.clbits.astype(np.int64))
+    return np.concatenate(out)
+
+
 def run_sequential(circuit: Circuit, noise: NoiseModel | None, shots: int, rng: np.random.Generator) -> list[int]:
     """One device statevector per shot, in the reference's exact draw order (any circuit)."""
     results = []
@@ -375,7 +403,9 @@ def run_trajectories(
         for v in bits:
             tally[v] = tally.get(v, 0) + 1
         return tally
-    values = run_batched(circuit, noise, shots, rng, dtype=dtype)
+    values = run_lds(circuit, noise, shots, rng, dtype=dtype)
+    if values is None:
+        values = run_batched(circuit, noise, shots, rng, dtype=dtype)
     uniq, counts = np.unique(values, return_counts=True)
     return {int(v): int(c) for v, c in zip(uniq, counts)}
 

@@ -298,3 +298,30 @@ def test_batched_decisions_equal_the_scalar_rule() -> None:
             pick = (len(amp.kraus) - 1, w)
         assert sel[i] == i
         assert np.allclose(mats[i], amp.kraus[pick[0]] / np.sqrt(pick[1]))
+
+
+# ----------------------------------------------------------------------- program validation
+
+
+def test_program_tables_are_bounds_checked_before_launch() -> None:
+    from lucivant_sim import program as pg
+
+    c = ls.parse(wl.teleport_chain(9, 3))
+    prog = pg.compile_trajectory(c, wl.noise_c4())
+    table = np.array(prog.ops, dtype=pg.OP_DTYPE)
+    reals = np.asarray(prog.reals)
+    pg._validate(table, 9, prog.mat_size, reals, prog.draws)
+    planted = [
+        ("t", lambda t: t["t"].__setitem__((0, 0), 9), "beyond the state"),
+        ("mat", lambda t: t["mat"].__setitem__(0, prog.mat_size), "matrix pool"),
+        ("draw", lambda t: t["draw"].__setitem__(np.flatnonzero(t["kind"] == pg.OP_MEASURE)[0], prog.draws), "draw"),
+        ("k", lambda t: t["k"].__setitem__(0, 0), "no target"),
+        ("kind", lambda t: t["kind"].__setitem__(0, 9), "unknown op"),
+    ]
+    for name, plant, message in planted:
+        bad = table.copy()
+        plant(bad)
+        with pytest.raises(ValueError, match=message):
+            pg._validate(bad, 9, prog.mat_size, reals, prog.draws)
+    with pytest.raises(ValueError, match="draw"):
+        pg._validate(table, 9, prog.mat_size, reals, 0)
