@@ -66,6 +66,145 @@ def cpu_baseline(circuit, budget_s: float = 15.0) -> dict:  # type: ignore[no-un
     }
 
 
+def plain_counts(circuit, shots: int, seed: int):  # type: ignore[no-untyped-def]
+    """The plain GPU path for one circuit: a fresh HBM statevector, one kernel launch per gate
+    (matrices uploaded once), probabilities back to the host, the reference's sampling rule."""
+    import numpy as np
+
+    from lucivant_sim.circuit import Gate
+    from lucivant_sim.engines import Statevector, run_passes
+    from lucivant_sim.plan import Pass, is_diagonal
+    from lucivant_sim.simulator import _sample
+
+    passes = []
+    for op in circuit.operations:
+        if isinstance(op, Gate):
+            m = np.asarray(op.matrix)
+            passes.append(Pass("diag", op.qubits, np.diag(m)) if is_diagonal(m) else Pass("dense", op.qubits, m))
+    sv = Statevector(circuit.num_qubits)
+    run_passes(sv.buffer, sv.dtype, sv.nbits, 1, passes)
+    return _sample(circuit, sv.probabilities(), shots, None, np.random.default_rng(seed))
+
+
+def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
+    """12-qubit circuits of the 12-qubit configuration's shape (20 layers of u3 + cx, 1000 shots
+    each, distinct seeds): plain = one circuit at a time, one launch per gate; batched = run_many,
+    one LDS-program launch for all. Both start from parsed circuits (parsing is host work common
+    to both and excluded)."""
+    import numpy as np
+
+    import lucivant_sim as ls
+    from lucivant_sim import workloads
+    from lucivant_sim.batch import run_many
+
+    circuits = [ls.parse(workloads.random_layers(12, 20, 5000 + i)) for i in range(count)]
+    seeds = list(range(count))
+    run_many(circuits[:8], 1000, seeds=seeds[:8])  # warm-up (kernel load)
+    plain_counts(circuits[0], 1000, 0)
+    t0 = time.perf_counter()
+    lat = []
+    for i in range(plain_count):
+        a = time.perf_counter()
+        plain_counts(circuits[i], 1000, seeds[i])
+        lat.append(time.perf_counter() - a)
+    plain = plain_count / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    res = run_many(circuits, 1000, seeds=seeds)
+    dt = time.perf_counter() - t0
+    batched = count / dt
+    same = all(res[i].counts == plain_counts(circuits[i], 1000, seeds[i]) for i in range(4))
+    return {
+        "workload": f"{count} distinct 12-qubit circuits, 20 layers u3 + cx, 1000 shots each, complex128",
+        "plain_circuits_per_s": round(plain, 1),
+        "plain_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "plain_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "batched_circuits_per_s": round(batched, 1),
+        "batched_latency_ms": round(dt * 1e3, 2),
+        "speedup": round(batched / plain, 2),
+        "counts_equal_plain": same,
+    }
+
+
+def bench_serving(qps: float, count: int = 48, seed: int = 1005) -> dict:
+    """The serving stream: Poisson arrivals at ``qps``; widths uniform in 10..26, depths in
+    10..100; half hardware-efficient ansatz expectation requests (16-term Pauli sums, weights in
+    [-1, 1]), half random u3 + cx count requests (1000 shots); count requests of at most 12
+    qubits carry the uniform noise model (1e-3 / 1e-2) and run as trajectories (wider noisy
+    requests would be 1000 full simulations each, so noise is capped there)."""
+    import numpy as np
+
+    from lucivant_sim import workloads
+    from lucivant_sim.noise import NoiseModel
+    from lucivant_sim.serving import Request, Server, plain_stream, serve_stream
+
+    rng = np.random.default_rng(seed)
+    noise = NoiseModel.uniform(single=1e-3, two=1e-2)
+
+    def make():  # type: ignore[no-untyped-def]
+        reqs = []
+        for i in range(count):
+            n = int(rng.integers(10, 27))
+            depth = int(rng.integers(10, 101))
+            if i % 2 == 0:
+                src = workloads.hea_ansatz(n, depth, 7000 + i)
+                obs = {lbl: float(w) for lbl, w in zip(workloads.pauli_strings(n, 16, 8000 + i), rng.uniform(-1, 1, 16))}
+                reqs.append(Request(src, observable=obs))
+            else:
+                src = workloads.random_layers(n, depth, 9000 + i)
+                noisy = n <= 12
+                reqs.append(Request(src, shots=1000, seed=i, noise=noise if noisy else None, method="trajectories" if noisy else "automatic"))
+        return reqs
+
+    reqs = make()
+    arrivals = list(np.cumsum(rng.exponential(1.0 / qps, size=count)))
+    served = serve_stream(Server(), reqs, arrivals)
+    plain_reqs = [Request(r.program, observable=r.observable, shots=r.shots, seed=r.seed, noise=r.noise, method=r.method) for r in reqs]
+    plain = plain_stream(plain_reqs, arrivals)
+    agree = all(
+        (abs(a.result - b.result) < 1e-9) if a.observable is not None else a.result == b.result for a, b in zip(reqs, plain_reqs)
+    )
+    return {
+        "qps": qps,
+        "requests": count,
+        "served": {k: round(v, 3) if isinstance(v, float) else v for k, v in served.items()},
+        "plain": {k: round(v, 3) if isinstance(v, float) else v for k, v in plain.items()},
+        "results_equal_plain": agree,
+    }
+
+
+def bench_trajectories(shots: int = 4096) -> dict:
+    """The 20-qubit trajectory configuration (teleport chain, 6 mid-circuit measurements,
+    uniform depolarising 1e-3/1e-2 and amplitude damping 0.01 on h) in complex128 and complex64."""
+    import numpy as np
+
+    import lucivant_sim as ls
+    from lucivant_sim import _native as nat
+    from lucivant_sim import workloads
+    from lucivant_sim.trajectories import run_batched
+
+    c = ls.parse(workloads.teleport_chain(20, 3))
+    noise = workloads.noise_c4()
+    out = {}
+    bits = {}
+    for name, dt in (("complex128", nat.C128), ("complex64", nat.C64)):
+        t0 = time.perf_counter()
+        bits[name] = run_batched(c, noise, shots, np.random.default_rng(7), dtype=dt)
+        el = time.perf_counter() - t0
+        out[name] = {"seconds": round(el, 3), "shots_per_s": round(shots / el, 1)}
+    a, b = bits["complex128"], bits["complex64"]
+    ua, ca = np.unique(a, return_counts=True)
+    ub, cb = np.unique(b, return_counts=True)
+    pa = dict(zip(ua.tolist(), (ca / shots).tolist()))
+    pb = dict(zip(ub.tolist(), (cb / shots).tolist()))
+    keys = set(pa) | set(pb)
+    out["workload"] = f"20-qubit teleport chain, {shots} shots in one batch ({shots} x 16 MiB complex128 states)"
+    out["complex64_vs_complex128"] = {
+        "shots_with_different_bits": int(np.sum(a != b)),
+        "tvd": round(0.5 * sum(abs(pa.get(k, 0) - pb.get(k, 0)) for k in keys), 6),
+    }
+    return out
+
+
 def read_traffic() -> float | None:
     """HBM bytes per dense launch from the committed rocprofv3 counter summary, if present."""
     path = ROOT / "profiles" / "dense_gate_26q_traffic.json"
@@ -84,6 +223,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=0, help="fusion width (0: the tuned default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and trajectory legs")
+    ap.add_argument(
+        "--profile",
+        action="store_true",
+        help="only run the passes (and one k_scale pass, a read+write of the 1 GiB state used to "
+        "calibrate HBM counters) for rocprofv3; prints no JSON",
+    )
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(
@@ -126,6 +272,14 @@ def main() -> int:
         for fn, _kind, k, bits, addr in calls:
             rc = fn(state, nat.C128, N, 1, k, bits, addr, None, hs)
             assert rc == 0, lib.lsim_last_error()
+
+    if args.profile:
+        for _ in range(max(args.steps, 1)):
+            step()
+            lib.lsim_sv_scale(state, nat.C128, N, 1, 1.0, 0.0, hs)
+        torch.cuda.synchronize()
+        print(f"profiled {args.steps} steps of {len(passes)} passes ({dense} dense) plus one scale pass each", flush=True)
+        return 0
 
     for _ in range(args.warmup):
         step()
@@ -202,6 +356,12 @@ def main() -> int:
             "per_k_avg_ms": {str(k): round(float(np.mean(v)) * 1e3, 4) for k, v in durations.items() if v},
         },
     }
+    del sv
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        out["batched"] = bench_batched()
+        out["serving"] = [bench_serving(q) for q in (5.0, 20.0)]
+        out["trajectories_20q"] = bench_trajectories()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(circuit)
     print(json.dumps(out), flush=True)

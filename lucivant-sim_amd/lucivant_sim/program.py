@@ -102,6 +102,9 @@ class Program:
     reals: list[float] = field(default_factory=list)
     mat_size: int = 0
     draws: int = 0
+    #: prebuilt op table and matrix pool (set by compile_static's vectorised path)
+    table: np.ndarray | None = None
+    pool: np.ndarray | None = None
 
     def add_matrix(self, m: np.ndarray) -> int:
         off = self.mat_size
@@ -126,7 +129,12 @@ def _gate(prog: Program, matrix: np.ndarray, qubits: tuple[int, ...], guard: tup
 
 
 def compile_static(circuit: Circuit, fuse_width: int = 0) -> Program:
-    """An ideal static circuit (measurements deferred): gates only, optionally fused."""
+    """An ideal static circuit (measurements deferred): gates only, optionally fused.
+
+    Unfused programs are built with array operations over all gates at once (a few dozen NumPy
+    calls per circuit whatever its length), which keeps host time per batched circuit small."""
+    if not fuse_width:
+        return _compile_static_vectorised(circuit)
     prog = Program(circuit.num_qubits)
     if fuse_width:
         from lucivant_sim.plan import plan_statevector
@@ -146,6 +154,65 @@ def compile_static(circuit: Circuit, fuse_width: int = 0) -> Program:
             _gate(prog, op.matrix, op.qubits, (0, 0))
         elif isinstance(op, GlobalPhase):
             prog.op(OP_PHASE, (), mat=prog.add_matrix(np.array([np.exp(1j * op.angle)])))
+    return prog
+
+
+def _compile_static_vectorised(circuit: Circuit) -> Program:
+    from lucivant_sim.circuit import GlobalPhase
+
+    gates = []
+    phase = 1.0 + 0.0j
+    for op in circuit.operations:
+        if isinstance(op, Gate):
+            gates.append(op)
+        elif isinstance(op, GlobalPhase):
+            phase *= np.exp(1j * op.angle)
+    prog = Program(circuit.num_qubits)
+    count = len(gates) + (phase != 1)
+    table = np.zeros(count, dtype=OP_DTYPE)
+    if not gates and phase == 1:
+        prog.table, prog.pool = table, np.zeros(0, dtype=np.complex128)
+        return prog
+    ks = np.fromiter((len(g.qubits) for g in gates), dtype=np.int64, count=len(gates))
+    if np.any(ks > 5):
+        raise Unsupported("operators on more than 5 qubits")
+    sizes = np.zeros(len(gates), dtype=np.int64)
+    kinds = np.zeros(len(gates), dtype=np.uint8)
+    groups = []
+    for k in np.unique(ks):
+        idx = np.flatnonzero(ks == k)
+        d = 1 << int(k)
+        mats = np.stack([np.asarray(gates[i].matrix, dtype=np.complex128) for i in idx])
+        off = ~np.eye(d, dtype=bool)
+        diag = ~np.any(mats[:, off], axis=1)
+        sizes[idx] = np.where(diag, d, d * d)
+        kinds[idx] = np.where(diag, OP_DIAG, OP_DENSE)
+        groups.append((idx, mats, diag, d))
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    pool = np.empty(int(offsets[-1]) + (phase != 1), dtype=np.complex128)
+    for idx, mats, diag, d in groups:
+        if np.any(diag):
+            sel = idx[diag]
+            pool[offsets[sel][:, None] + np.arange(d)[None, :]] = np.diagonal(mats[diag], axis1=1, axis2=2)
+        if np.any(~diag):
+            sel = idx[~diag]
+            pool[offsets[sel][:, None] + np.arange(d * d)[None, :]] = mats[~diag].reshape(-1, d * d)
+    n = len(gates)
+    table["kind"][:n] = kinds
+    table["k"][:n] = ks
+    t = np.zeros((n, 5), dtype=np.uint8)
+    for j, g in enumerate(gates):
+        t[j, : len(g.qubits)] = g.qubits
+    table["t"][:n] = t
+    table["mat"][:n] = offsets[:-1]
+    table["aux"] = -1
+    table["clbit"] = -1
+    if phase != 1:
+        pool[-1] = phase
+        table["kind"][n] = OP_PHASE
+        table["mat"][n] = pool.size - 1
+    prog.table, prog.pool = table, pool
+    prog.mat_size = pool.size
     return prog
 
 
@@ -247,11 +314,22 @@ def run_programs(
     if n > max_qubits(dtype):
         raise Unsupported(f"{n} qubits don't fit one workgroup's LDS")
     device = require_device()
-    table = np.zeros(sum(len(p.ops) for p in programs), dtype=OP_DTYPE)
+    table = np.zeros(sum(len(p.table) if p.table is not None else len(p.ops) for p in programs), dtype=OP_DTYPE)
     starts, counts = [], []
     mats, reals = [], []
     mat_base = real_base = row = 0
     for p in programs:
+        if p.table is not None:
+            arr = p.table.copy()
+            arr["mat"] += mat_base
+            table[row : row + len(arr)] = arr
+            starts.append(row)
+            counts.append(len(arr))
+            row += len(arr)
+            mats.append(p.pool)
+            reals.append(np.zeros(0))
+            mat_base += p.mat_size
+            continue
         if p.ops:
             arr = np.array(p.ops, dtype=OP_DTYPE)
             # rebase offsets into the concatenated pools

@@ -19,7 +19,7 @@ import re
 from collections.abc import Callable, Mapping
 from dataclasses import dataclass
 from pathlib import Path
-from typing import Any
+from typing import Any, NamedTuple
 
 import numpy as np
 
@@ -66,8 +66,7 @@ class QasmError(ValueError):
 # ----------------------------------------------------------------------------------- lexing
 
 
-@dataclass(frozen=True)
-class Tok:
+class Tok(NamedTuple):
     kind: str  # id, num, str, phys, op, eof
     text: str
     line: int
@@ -75,13 +74,14 @@ class Tok:
 
 
 _LEX = re.compile(
-    r"(?P<nl>\n)|(?P<ws>[ \t\r\f\v]+)|(?P<line_comment>//[^\n]*)|(?P<block_comment>/\*.*?\*/)"
+    r"(?P<ws>\s+)|(?P<line_comment>//[^\n]*)|(?P<block_comment>/\*.*?\*/)"
     r"|(?P<num>0[bB][01_]+|0[xX][0-9a-fA-F_]+|0[oO][0-7_]+|(?:\d[\d_]*\.?[\d_]*|\.\d[\d_]*)(?:[eE][+-]?\d+)?)"
     r"|(?P<str>\"[^\"\n]*\"|'[^'\n]*')|(?P<phys>\$\d+)|(?P<id>[^\W\d]\w*)"
     r"|(?P<op>->|==|!=|<=|>=|&&|\|\||\*\*|<<|>>|[\[\](){};,:=+\-*/^%<>!~@&|.])",
     re.DOTALL,
 )
 _PRAGMA = re.compile(r"(?m)^[ \t]*(?:#?pragma|@)[^\n]*")
+_SKIP = frozenset({"ws", "line_comment", "block_comment"})
 
 
 def tokenize(source: str) -> list[Tok]:
@@ -89,23 +89,25 @@ def tokenize(source: str) -> list[Tok]:
     annotation lines are dropped (they don't change the circuit)."""
     source = _PRAGMA.sub("", source)
     out: list[Tok] = []
+    append = out.append
     line, line_start, pos = 1, 0, 0
-    while pos < len(source):
-        m = _LEX.match(source, pos)
-        if m is None:
+    for m in _LEX.finditer(source):
+        start = m.start()
+        if start != pos:
             raise QasmError(f"unexpected character {source[pos]!r}", line, pos - line_start + 1)
-        kind = m.lastgroup or ""
-        text = m.group()
-        if kind == "nl":
-            line, line_start = line + 1, m.end()
-        elif kind == "block_comment":
-            if "\n" in text:
-                line += text.count("\n")
-                line_start = pos + text.rfind("\n") + 1
-        elif kind not in ("ws", "line_comment"):
-            out.append(Tok(kind, text, line, pos - line_start + 1))
+        kind = m.lastgroup
         pos = m.end()
-    out.append(Tok("eof", "", line, pos - line_start + 1))
+        if kind in _SKIP:
+            text = m.group()
+            nl = text.count("\n")
+            if nl:
+                line += nl
+                line_start = start + text.rfind("\n") + 1
+            continue
+        append(Tok(kind, m.group(), line, start - line_start + 1))  # type: ignore[arg-type]
+    if pos != len(source):
+        raise QasmError(f"unexpected character {source[pos]!r}", line, pos - line_start + 1)
+    append(Tok("eof", "", line, pos - line_start + 1))
     return out
 
 
@@ -423,8 +425,71 @@ class _Reader:
             self.expect(";")
             return self.measures(qubits, bits)
         if t.kind == "id":
+            fast = self.simple_gate_statement()
+            if fast is not None:
+                return fast
             return self.gate_statement()
         raise self.err(f"unexpected {t.text!r}" if t.kind != "eof" else "unexpected end of program")
+
+    def simple_gate_statement(self) -> list[Operation] | None:
+        """``name(num, -num, ...) reg[int], reg[int], ...;`` for a built-in gate, read straight
+        from the token list; None (nothing consumed) for anything else, which then takes the
+        general path and its error messages. Produces exactly what :meth:`gate_statement`
+        produces for such a statement."""
+        toks = self.toks
+        j = self.i
+        name = toks[j].text
+        spec = GATES.get(name)
+        if spec is None or name in self.gates:
+            return None
+        j += 1
+        params: list[float] = []
+        tok = toks[j]
+        if tok.text == "(" and tok.kind == "op":
+            j += 1
+            while True:
+                tok = toks[j]
+                negative = tok.text == "-" and tok.kind == "op"
+                if negative:
+                    j += 1
+                    tok = toks[j]
+                if tok.kind != "num":
+                    return None
+                v = _number(tok.text)
+                params.append(-v if negative else v)
+                j += 1
+                sep = toks[j]
+                j += 1
+                if sep.text == ")" and sep.kind == "op":
+                    break
+                if sep.text != "," or sep.kind != "op":
+                    return None
+        qubits: list[int] = []
+        while True:
+            reg_tok = toks[j]
+            reg = self.qregs.get(reg_tok.text) if reg_tok.kind == "id" else None
+            if reg is None or toks[j + 1].text != "[" or toks[j + 2].kind != "num" or toks[j + 3].text != "]":
+                return None
+            text = toks[j + 2].text
+            if not text.isdigit():
+                return None
+            index = int(text)
+            if index >= reg.size:
+                return None
+            qubits.append(reg.offset + index)
+            sep = toks[j + 4]
+            j += 5
+            if sep.kind != "op":
+                return None
+            if sep.text == ";":
+                break
+            if sep.text != ",":
+                return None
+        if len(params) != spec.params or len(qubits) != spec.qubits or len(set(qubits)) != len(qubits):
+            return None
+        self.i = j
+        pt = tuple(params)
+        return [Gate(canonical_name(name), tuple(qubits), gate_matrix(name, pt), pt)]
 
     def include_statement(self) -> list[Operation]:
         self.take()
@@ -777,13 +842,17 @@ class _Reader:
 
     def define_prelude(self) -> None:
         """The relative-phase Toffolis qelib1.inc defines by decomposition (not user gates, so a
-        program may redefine them)."""
-        saved = (self.toks, self.i)
-        self.toks, self.i = tokenize(QELIB_DECOMPOSED), 0
-        while self.peek().kind != "eof":
-            self.gate_definition()
-        self.toks, self.i = saved
-        self.user_defined.clear()
+        program may redefine them); parsed once per process."""
+        global _PRELUDE_GATES
+        if _PRELUDE_GATES is None:
+            saved = (self.toks, self.i)
+            self.toks, self.i = tokenize(QELIB_DECOMPOSED), 0
+            while self.peek().kind != "eof":
+                self.gate_definition()
+            self.toks, self.i = saved
+            self.user_defined.clear()
+            _PRELUDE_GATES = dict(self.gates)
+        self.gates.update(_PRELUDE_GATES)
 
 
 def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
@@ -803,6 +872,8 @@ def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
     same_rest = rest[:, None] == rest[None, :]
     return np.where(same_rest, sub[sub_index[:, None], sub_index[None, :]], 0).astype(np.complex128)
 
+
+_PRELUDE_GATES: dict[str, _GateDef] | None = None
 
 # --------------------------------------------------------------------------------------- API
 

@@ -51,6 +51,7 @@ class Request:
     shots: int = 1000
     seed: int = 0
     noise: NoiseModel | None = None
+    method: str = "automatic"
     id: int = field(default_factory=lambda: next(_ids))
     arrival: float = 0.0
     # filled in by the server
@@ -119,8 +120,15 @@ class Server:
     # -- submission ---------------------------------------------------------------------
 
     def submit(self, req: Request) -> Request:
+        """Queue a request. Its program is parsed here, in the caller's thread, so the host work
+        of arriving requests overlaps what the scheduler has on the GPU."""
         if not req.arrival:
             req.arrival = time.perf_counter()
+        try:
+            req.program = to_circuit(req.program)
+        except BaseException as exc:  # noqa: BLE001 - reported to the requester
+            self._finish_error(req, exc)
+            return req
         self.inbox.put(req)
         return req
 
@@ -156,10 +164,12 @@ class Server:
             method = "statevector"
         else:
             terms = None
-            method = choose_method(circuit, req.noise)
-            if method not in ("statevector", "trajectories"):
-                raise ValueError(f"the server runs statevector and trajectory requests, not {method!r}")
+            method = choose_method(circuit, req.noise, req.method)
+            if method not in ("statevector", "trajectories", "density_matrix"):
+                raise ValueError(f"the server runs statevector, density-matrix and trajectory requests, not {method!r}")
         size = 16 << circuit.num_qubits
+        if method == "density_matrix":
+            size = 16 << (2 * circuit.num_qubits)
         footprint = size * (req.shots if method == "trajectories" and circuit.num_qubits > self.max_lds else 1)
         return _Job(req, circuit, method, footprint, terms)
 
@@ -189,7 +199,11 @@ class Server:
         if not self.resident:
             return False
         self.ticks += 1
-        small = [j for j in self.resident if j.state is None and j.circuit.num_qubits <= self.max_lds and j.circuit.num_qubits >= 1]
+        small = [
+            j
+            for j in self.resident
+            if j.state is None and j.method != "density_matrix" and 1 <= j.circuit.num_qubits <= self.max_lds
+        ]
         self._run_small(small)
         wide = [j for j in self.resident if j not in small]
         for job in wide:
@@ -235,6 +249,17 @@ class Server:
                     self._release(j)
 
     def _advance_wide(self, job: _Job) -> None:
+        if job.method == "density_matrix":
+            from lucivant_sim.simulator import run
+
+            try:
+                res = run(job.circuit, job.req.shots, seed=job.req.seed, noise=job.req.noise, method="density_matrix")
+                job.passes, job.next_pass = [], 0
+                self._finish(job, res.counts)
+            except BaseException as exc:  # noqa: BLE001
+                self._finish_error(job.req, exc)
+                self._release(job)
+            return
         if job.method == "trajectories":
             from lucivant_sim.trajectories import run_trajectories
 
@@ -366,7 +391,7 @@ def plain_stream(requests: list[Request], arrivals: list[float]) -> dict[str, fl
             if req.observable is not None:
                 req.result = simulator.expectation(req.program, req.observable)
             else:
-                res = simulator.run(req.program, req.shots, seed=req.seed, noise=req.noise)
+                res = simulator.run(req.program, req.shots, seed=req.seed, noise=req.noise, method=req.method)
                 req.result = res.counts
             req.done = time.perf_counter()
             lat.append(req.done - arrival)
