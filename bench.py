@@ -95,7 +95,7 @@ def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
 
     import lucivant_sim as ls
     from lucivant_sim import workloads
-    from lucivant_sim.batch import run_many
+    from lucivant_sim.batch import counts_many, run_many
 
     circuits = [ls.parse(workloads.random_layers(12, 20, 5000 + i)) for i in range(count)]
     seeds = list(range(count))
@@ -109,10 +109,11 @@ def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
         lat.append(time.perf_counter() - a)
     plain = plain_count / (time.perf_counter() - t0)
     t0 = time.perf_counter()
-    res = run_many(circuits, 1000, seeds=seeds)
+    res = counts_many(circuits, 1000, seeds)
     dt = time.perf_counter() - t0
     batched = count / dt
-    same = all(res[i].counts == plain_counts(circuits[i], 1000, seeds[i]) for i in range(4))
+    same = all(res[i] == plain_counts(circuits[i], 1000, seeds[i]) for i in range(4))
+    same = same and run_many(circuits[:2], 1000, seeds=seeds[:2])[1].counts == res[1]
     return {
         "workload": f"{count} distinct 12-qubit circuits, 20 layers u3 + cx, 1000 shots each, complex128",
         "plain_circuits_per_s": round(plain, 1),

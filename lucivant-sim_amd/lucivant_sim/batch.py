@@ -42,6 +42,21 @@ def _result(circuit: Circuit, counts: dict[str, int], shots: int, seed: int, met
     )
 
 
+def counts_many(
+    circuits: Sequence[Circuit],
+    shots: int,
+    seeds: Sequence[int],
+    noise: NoiseModel | None = None,
+    dtype: int = nat.C128,
+) -> list[dict[str, int]]:
+    """Counts of ideal static circuits of one width (each with its own seed) from one
+    LDS-program launch: the counts ``run`` would return, without building Results."""
+    progs = [pg.compile_static(c) for c in circuits]
+    out = pg.run_programs(progs, dtype=dtype, want_probs=True)
+    assert out.probs is not None
+    return [_sample(c, out.probs[j], shots, noise, np.random.default_rng(s)) for j, (c, s) in enumerate(zip(circuits, seeds))]
+
+
 def run_many(
     programs: Sequence[Any],
     shots: int = 1024,

