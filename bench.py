@@ -206,6 +206,35 @@ def bench_trajectories(shots: int = 4096) -> dict:
     return out
 
 
+def bench_density_matrix() -> dict:
+    """The 13-qubit density-matrix configuration: depth 30 (seed 1004) under the uniform noise
+    model with thermal relaxation on u3 and readout error; exact probabilities, then 4000 seeded
+    shots with the density-matrix method. The 4^13-entry matrix is 1 GiB complex128."""
+    import lucivant_sim as ls
+    from lucivant_sim import workloads
+    from lucivant_sim.plan import Pass, plan_density
+
+    src = workloads.config_c3()
+    noise = workloads.noise_c3()
+    circuit = ls.parse(src)
+    ls.probabilities(workloads.config_c3(4, 2), noise)  # warm-up
+    t0 = time.perf_counter()
+    probs = ls.probabilities(circuit, noise)
+    t_probs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = ls.run(circuit, 4000, seed=7, noise=noise, method="density_matrix")
+    t_run = time.perf_counter() - t0
+    passes = [p for p in plan_density(circuit.operations, 13, noise) if isinstance(p, Pass)]
+    return {
+        "workload": "13-qubit density matrix, depth 30 u3 + cx/cz, depolarising 1e-3/1e-2, thermal relaxation on u3, readout 0.01/0.03",
+        "superoperator_passes": len(passes),
+        "probabilities_seconds": round(t_probs, 3),
+        "run_4000_shots_seconds": round(t_run, 3),
+        "distinct_outcomes": len(res.counts),
+        "probability_sum": round(sum(probs.values()), 12),
+    }
+
+
 def read_traffic() -> float | None:
     """HBM bytes per dense launch from the committed rocprofv3 counter summary, if present."""
     path = ROOT / "profiles" / "dense_gate_26q_traffic.json"
@@ -363,6 +392,7 @@ def main() -> int:
         out["batched"] = bench_batched()
         out["serving"] = [bench_serving(q) for q in (5.0, 20.0)]
         out["trajectories_20q"] = bench_trajectories()
+        out["density_matrix_13q"] = bench_density_matrix()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(circuit)
     print(json.dumps(out), flush=True)

@@ -61,6 +61,11 @@ int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long 
 int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int k,
                         const int* targets, const void* mats, const int* sel, void* stream);
 
+/* Select the dense kernel variant: with tiling on (the default), operators whose targets include
+ * low index bits run through LDS-staged tiles; off, every placement uses the direct kernel.
+ * Returns the previous setting. No reference counterpart (a tuning switch). */
+int lsim_set_dense_tiling(int enabled);
+
 /* psi <- diag(d) psi on k <= 5 target bits: the diagonal case of _apply_slices
  * (engines.py:91-104; rz, s, t, cz, cp). `diags`: M diagonals of 2^k entries; `sel` as above. */
 int lsim_sv_apply_diag(void* states, int dtype, int nbits, long long batch, int k,

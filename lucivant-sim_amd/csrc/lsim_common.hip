@@ -56,6 +56,51 @@ int make_targets(int nbits, int k, const int* targets, Targets* out) {
   return LSIM_OK;
 }
 
+bool make_tiling(int nbits, int k, const int* targets, Tiling* t) {
+  uint32_t sorted[kMaxK];
+  for (int i = 0; i < k; ++i) sorted[i] = (uint32_t)targets[i];
+  for (int i = 1; i < k; ++i)
+    for (int j = i; j > 0 && sorted[j - 1] > sorted[j]; --j) {
+      const uint32_t x = sorted[j];
+      sorted[j] = sorted[j - 1];
+      sorted[j - 1] = x;
+    }
+  // low targets: the longest prefix with sorted[j] < 6 + (j + 1); then b = 6 + (#low)
+  int low = 0;
+  while (low < k && sorted[low] < (uint32_t)(6 + low + 1)) ++low;
+  // lanes stride only when at least one target sits below bit 6, and striding hurts from two
+  // low targets (or one low target of a wider operator)
+  int below6 = 0;
+  for (int i = 0; i < k; ++i) below6 += sorted[i] < 6;
+  if (below6 == 0 || (below6 == 1 && k <= 2)) return false;
+  if ((int)(k + 6) > nbits) return false;
+  t->b = 6 + low;
+  t->h = k - low;
+  for (int j = 0; j < kMaxK; ++j) t->high[j] = j < (int)t->h ? sorted[low + j] : 0;
+  for (int s = 0; s < (1 << kMaxK); ++s) {
+    uint32_t off = 0;
+    if (s < (1 << t->h))
+      for (int j = 0; j < (int)t->h; ++j)
+        if ((s >> j) & 1) off |= 1u << t->high[j];
+    t->seg[s] = off;
+  }
+  auto local = [&](uint32_t g) -> uint32_t {
+    if (g < t->b) return g;
+    for (int j = 0; j < (int)t->h; ++j)
+      if (t->high[j] == g) return t->b + j;
+    return 0;
+  };
+  for (int i = 0; i < kMaxK; ++i) t->lsorted[i] = i < k ? local(sorted[i]) : 0;
+  for (int c = 0; c < (1 << kMaxK); ++c) {
+    uint32_t off = 0;
+    if (c < (1 << k))
+      for (int i = 0; i < k; ++i)
+        if ((c >> (k - 1 - i)) & 1) off |= 1u << local((uint32_t)targets[i]);
+    t->loffs[c] = off;
+  }
+  return true;
+}
+
 }  // namespace lsim
 
 extern "C" {

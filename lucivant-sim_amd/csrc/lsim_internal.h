@@ -53,6 +53,20 @@ struct Targets {
 
 int make_targets(int nbits, int k, const int* targets, Targets* out);
 
+// LDS-tiled placement of a k-qubit operator: a tile is 2^(k+6) amplitudes, the `b` lowest
+// index bits (contiguous, coalesced) times every combination of the `h` targets at or above b.
+// Local tile index bits: [0, b) are the global bits [0, b); b + j is high[j].
+struct Tiling {
+  uint32_t b, h;
+  uint32_t high[kMaxK];          // targets >= b, ascending
+  uint32_t seg[1 << kMaxK];      // global offset of tile segment s
+  uint32_t lsorted[kMaxK];       // targets in local bits, ascending
+  uint32_t loffs[1 << kMaxK];    // local offset of matrix index c
+};
+
+// True when the operator's low targets make lanes stride (the tiled kernel pays off); fills t.
+bool make_tiling(int nbits, int k, const int* targets, Tiling* t);
+
 __device__ __forceinline__ uint32_t spread(uint32_t g, const uint32_t* sorted, int k) {
 #pragma unroll
   for (int j = 0; j < kMaxK; ++j) {

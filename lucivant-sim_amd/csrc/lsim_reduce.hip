@@ -10,11 +10,12 @@
 
 namespace lsim {
 
-// Workgroups per state for a reduction over `items` work items (about 8 items per thread).
+// Workgroups per state for a reduction over `items` work items: about 8 items per thread, and
+// up to 4096 workgroups (16 waves per CU on 256 CUs) so one large state keeps HBM busy.
 static uint32_t reduce_blocks(uint64_t items) {
   uint64_t nb = items / (kBlock * 8);
   if (nb < 1) nb = 1;
-  if (nb > 256) nb = 256;
+  if (nb > 4096) nb = 4096;
   return (uint32_t)nb;
 }
 

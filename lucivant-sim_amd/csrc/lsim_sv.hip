@@ -53,6 +53,86 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// LDS-tiled dense operator for placements with low target bits. One wavefront owns a tile of
+// 2^(K+6) amplitudes: 2^h segments of 2^b contiguous amplitudes (b low bits, h high targets).
+// Segments are loaded with consecutive lanes on consecutive amplitudes (full 128 B lines), kept
+// in LDS under an XOR swizzle of the 16-byte bank group, each lane then applies U to one group
+// of 2^K amplitudes, and the tile is stored back the same coalesced way. HBM sees each line once.
+__device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ ((i >> 4) & 15u); }
+
+template <int K, typename R>
+__global__ void __launch_bounds__(64)
+    k_dense_tiled(typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t tiles,
+                  const typename Cx<R>::V* __restrict__ mats, const int* __restrict__ sel,
+                  uint32_t batch_base, Tiling t) {
+  using V = typename Cx<R>::V;
+  constexpr int D = 1 << K;
+  constexpr uint32_t M = 1u << (K + 6);
+  __shared__ V tile[M];
+  const uint32_t b = batch_base + blockIdx.y;
+  const V* __restrict__ U = mats;
+  if (sel != nullptr) {
+    const int m = sel[b];
+    if (m < 0) return;
+    U += (size_t)m * (D * D);
+  }
+  const uint32_t lane = threadIdx.x;
+  V* __restrict__ st = states + ((size_t)b << nbits);
+  const uint32_t seglen = 1u << t.b, nseg = 1u << t.h;
+  for (uint32_t tid = blockIdx.x; tid < tiles; tid += gridDim.x) {
+    // global base of the tile: tile number above bit b, zeros spread in at the high targets
+    uint32_t base = tid << t.b;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (j < (int)t.h) {
+        const uint32_t s = t.high[j];
+        base = ((base >> s) << (s + 1)) | (base & ((1u << s) - 1u));
+      }
+    // every lane loads M / 64 = D amplitudes, 64 consecutive per instruction
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      const uint32_t i = (uint32_t)r * 64u + lane;  // tile-local linear index
+      const uint32_t seg = i >> t.b, off = i & (seglen - 1u);
+      tile[swz(i)] = st[base + t.seg[seg] + off];
+    }
+    (void)nseg;
+    __syncthreads();
+    uint32_t g = lane;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const uint32_t s = t.lsorted[j];
+      g = ((g >> s) << (s + 1)) | (g & ((1u << s) - 1u));
+    }
+    V in[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) in[c] = tile[swz(g + t.loffs[c])];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      R re = 0, im = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const V u = U[r * D + c];
+        re = fma(u.x, in[c].x, re);
+        re = fma(-u.y, in[c].y, re);
+        im = fma(u.x, in[c].y, im);
+        im = fma(u.y, in[c].x, im);
+      }
+      V o;
+      o.x = re;
+      o.y = im;
+      tile[swz(g + t.loffs[r])] = o;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      const uint32_t i = (uint32_t)r * 64u + lane;
+      const uint32_t seg = i >> t.b, off = i & (seglen - 1u);
+      st[base + t.seg[seg] + off] = tile[swz(i)];
+    }
+    __syncthreads();
+  }
+}
+
 // Diagonal operator: every amplitude is scaled by the diagonal entry its target bits select.
 // The (at most 32-entry) diagonal is staged in LDS; four amplitudes per thread.
 template <typename R>
@@ -161,6 +241,29 @@ __global__ void __launch_bounds__(kBlock)
 // ----------------------------------------------------------------------------- launchers
 
 template <typename R>
+static int launch_dense_tiled(void* states, int nbits, long long batch, int k, const Tiling& t,
+                              const void* mats, const int* sel, hipStream_t s) {
+  using V = typename Cx<R>::V;
+  const uint32_t tiles = 1u << (nbits - k - 6);
+  // one tile per wavefront up to ~8 waves' worth per CU, then grid-stride
+  const uint32_t gx = tiles < 32768u ? tiles : 32768u;
+  for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
+    dim3 grid(gx, gy), block(64);
+    V* st = (V*)states;
+    const V* m = (const V*)mats;
+    switch (k) {
+      case 2: k_dense_tiled<2, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
+      case 3: k_dense_tiled<3, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
+      case 4: k_dense_tiled<4, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
+      case 5: k_dense_tiled<5, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
+      default: return fail(LSIM_EINVAL, "tiled dense operator needs k between 2 and 5");
+    }
+  }
+  return check_launch("tiled dense operator");
+}
+
+template <typename R>
 static int launch_dense(void* states, int nbits, long long batch, int k, const Targets& t,
                         const void* mats, const int* sel, hipStream_t s) {
   using V = typename Cx<R>::V;
@@ -195,11 +298,19 @@ static unsigned blocks_for(uint64_t total) {
   return (unsigned)((total + kBlock * 4 - 1) / (kBlock * 4));
 }
 
+static int g_tiling_enabled = 1;
+
 }  // namespace lsim
 
 using namespace lsim;
 
 extern "C" {
+
+int lsim_set_dense_tiling(int enabled) {
+  const int old = g_tiling_enabled;
+  g_tiling_enabled = enabled != 0;
+  return old;
+}
 
 int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,
                       void* stream) {
@@ -222,6 +333,12 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
   LSIM_REQUIRE(mats != nullptr, "the matrix pointer is NULL");
   Targets t;
   if (int rc = make_targets(nbits, k, targets, &t)) return rc;
+  Tiling tl;
+  if (k >= 2 && g_tiling_enabled && make_tiling(nbits, k, targets, &tl)) {
+    if (dtype == LSIM_C128)
+      return launch_dense_tiled<double>(states, nbits, batch, k, tl, mats, sel, as_stream(stream));
+    return launch_dense_tiled<float>(states, nbits, batch, k, tl, mats, sel, as_stream(stream));
+  }
   if (dtype == LSIM_C128)
     return launch_dense<double>(states, nbits, batch, k, t, mats, sel, as_stream(stream));
   return launch_dense<float>(states, nbits, batch, k, t, mats, sel, as_stream(stream));

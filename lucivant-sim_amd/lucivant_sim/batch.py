@@ -125,7 +125,7 @@ def _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0) ->
         c = circuits[i]
         bits = out.clbits[j * shots : (j + 1) * shots]
         uniq, cnt = np.unique(bits, return_counts=True)
-        counts = {c.format_bits(int(v)): int(k) for v, k in zip(uniq, cnt)} if c.num_clbits else {}
+        counts = dict(zip(c.format_many(uniq), cnt.tolist())) if c.num_clbits else {}
         results[i] = _result(c, counts, shots, seeds[i], "trajectories", noise, t0)
     del run_trajectories
 

@@ -95,6 +95,7 @@ class Register:
 
 
 _JSON_CACHE: dict[bytes, list[list[list[float]]]] = {}
+_FORMAT_TABLES: dict[tuple, np.ndarray] = {}
 
 
 def matrix_json(matrix: np.ndarray) -> list[list[list[float]]]:
@@ -189,6 +190,19 @@ class Circuit:
     def digest(self) -> str:
         data = json.dumps(self.canonical(), separators=(",", ":"), sort_keys=True).encode()
         return "sha256:" + hashlib.sha256(data).hexdigest()
+
+    def format_many(self, values: "np.ndarray") -> list[str]:
+        """``format_bits`` of many integers; layouts of at most 16 classical bits use a cached
+        table of every pattern (one lookup per value)."""
+        if 0 < self.num_clbits <= 16:
+            key = (self.num_clbits, tuple((r.offset, r.size) for r in self.cregs))
+            table = _FORMAT_TABLES.get(key)
+            if table is None:
+                table = np.array([self.format_bits(v) for v in range(1 << self.num_clbits)], dtype=object)
+                if len(_FORMAT_TABLES) < 64:
+                    _FORMAT_TABLES[key] = table
+            return table[np.asarray(values, dtype=np.int64)].tolist()
+        return [self.format_bits(int(v)) for v in values]
 
     def format_bits(self, bits: int) -> str:
         """Registers last-declared first, space-separated, each with its bit 0 on the right."""

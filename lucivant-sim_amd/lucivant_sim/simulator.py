@@ -231,7 +231,9 @@ def _sample(
         else:
             bits |= value << c
     values, counts = np.unique(bits, return_counts=True)
-    return {circuit.format_bits(int(v)): int(n) for v, n in zip(values, counts)}
+    if wide:
+        return {circuit.format_bits(int(v)): int(n) for v, n in zip(values, counts)}
+    return dict(zip(circuit.format_many(values), counts.tolist()))
 
 
 def run(

@@ -210,3 +210,27 @@ def test_density_matrix_diagonal_and_pauli() -> None:
     got = dm.expectations(labels)
     for t, lbl in enumerate(labels):
         assert abs(got[t] - orc.expectation_dm(rho, lbl, n)) < 1e-12, lbl
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+def test_dense_tiled_placements_equal_the_oracle_and_the_direct_kernel(k: int, dtype: int) -> None:
+    """Placements with low target bits take the LDS-tiled kernel; it must agree with the oracle
+    and with the direct kernel (tiling switched off) on every kind of placement."""
+    n = 12
+    rng = np.random.default_rng(40 + k)
+    placements = [tuple(range(k)), tuple(reversed(range(k))), (0,) + tuple(range(n - k + 1, n)), (1, 0) + tuple(range(7, 7 + k - 2))]
+    placements += [tuple(int(x) for x in rng.choice(n, size=k, replace=False)) for _ in range(10)]
+    lib = nat.load()
+    for i, targets in enumerate(placements):
+        psi = states(n, 2, 300 + i)
+        u = unitary(k, 60 + i)
+        want = orc.apply_sv(psi[1], u, targets, n)
+        tiled = apply_dense(psi, n, np.stack([u, u]), targets, dtype, [1, 0])[1]
+        old = lib.lsim_set_dense_tiling(0)
+        try:
+            direct = apply_dense(psi, n, np.stack([u, u]), targets, dtype, [1, 0])[1]
+        finally:
+            lib.lsim_set_dense_tiling(old)
+        assert np.abs(tiled - want).max() < TOL[dtype] * 4, targets
+        assert np.abs(tiled - direct).max() < TOL[dtype] * 4, targets
