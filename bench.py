@@ -126,6 +126,40 @@ def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
     }
 
 
+def bench_batched_shots(shots: int = 20000, plain_shots: int = 40) -> dict:
+    """Trajectory shots of the 20-qubit configuration's recipe at 9 qubits (teleport chain with
+    noise): plain = one shot at a time, one launch per operation (the reference's per-shot loop
+    on the device); batched = every shot in one LDS-program launch (one workgroup per shot), in
+    complex128 and complex64 (FP32 arithmetic, FP64 reductions and renormalisation)."""
+    import numpy as np
+
+    import lucivant_sim as ls
+    from lucivant_sim import _native as nat
+    from lucivant_sim import workloads
+    from lucivant_sim.trajectories import run_lds, run_sequential
+
+    c = ls.parse(workloads.teleport_chain(9, 3))
+    noise = workloads.noise_c4()
+    run_sequential(c, noise, 2, np.random.default_rng(0))
+    t0 = time.perf_counter()
+    plain_bits = run_sequential(c, noise, plain_shots, np.random.default_rng(7))
+    plain = plain_shots / (time.perf_counter() - t0)
+    out: dict = {"workload": f"9-qubit teleport chain with noise, {shots} shots", "plain_shots_per_s": round(plain, 1)}
+    for name, dt in (("complex128", nat.C128), ("complex64", nat.C64)):
+        run_lds(c, noise, 64, np.random.default_rng(0), dtype=dt)
+        t0 = time.perf_counter()
+        bits = run_lds(c, noise, shots, np.random.default_rng(7), dtype=dt)
+        el = time.perf_counter() - t0
+        assert bits is not None
+        out[name] = {"shots_per_s": round(shots / el, 1), "seconds": round(el, 4), "speedup_vs_plain": round(shots / el / plain, 1)}
+        if name == "complex128":
+            ref_bits = bits
+            out[name]["first_shots_equal_plain"] = [int(v) for v in bits[:plain_shots]] == plain_bits
+        else:
+            out[name]["shots_differing_from_complex128"] = int(np.sum(bits != ref_bits))
+    return out
+
+
 def bench_serving(qps: float, count: int = 48, seed: int = 1005) -> dict:
     """The serving stream: Poisson arrivals at ``qps``; widths uniform in 10..26, depths in
     10..100; half hardware-efficient ansatz expectation requests (16-term Pauli sums, weights in
@@ -278,9 +312,19 @@ def main() -> int:
     from lucivant_sim.engines import Statevector, upload
     from lucivant_sim.plan import plan_statevector
 
-    width = args.width or simulator.FUSION_WIDTH
     src, _labels = workloads.config_c2()
     circuit = ls.parse(src)
+    tuned = None
+    if args.width:
+        width = args.width
+    elif args.profile:
+        width = simulator.FUSION_WIDTH
+    else:
+        from lucivant_sim import autotune
+
+        tuned = autotune.tune(circuit, store=False)
+        width = tuned["width"]
+        nat.load().lsim_set_dense_tiling(tuned["tiled"])
     gates = sum(1 for op in circuit.operations if isinstance(op, ls.circuit.Gate))
     passes = plan_statevector(circuit.operations, N, width)
     dense = sum(1 for p in passes if p.kind == "dense")
@@ -373,6 +417,7 @@ def main() -> int:
             "dense_passes": dense,
             "bytes_per_pass": BYTES_PER_PASS,
             "effective_unfused_GBps": round(gates * BYTES_PER_PASS / (elapsed / args.steps) / 1e9, 1),
+            "autotune_ms": {k: round(v * 1e3, 2) for k, v in tuned["timings"].items()} if tuned else None,
         },
         "roofline": {
             "bound": "hbm",
@@ -390,6 +435,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     if not args.no_extras:
         out["batched"] = bench_batched()
+        out["batched_shots"] = bench_batched_shots()
         out["serving"] = [bench_serving(q) for q in (5.0, 20.0)]
         out["trajectories_20q"] = bench_trajectories()
         out["density_matrix_13q"] = bench_density_matrix()

@@ -68,11 +68,16 @@ bool make_tiling(int nbits, int k, const int* targets, Tiling* t) {
   // low targets: the longest prefix with sorted[j] < 6 + (j + 1); then b = 6 + (#low)
   int low = 0;
   while (low < k && sorted[low] < (uint32_t)(6 + low + 1)) ++low;
-  // lanes stride only when at least one target sits below bit 6, and striding hurts from two
-  // low targets (or one low target of a wider operator)
+  // The direct kernel's lanes read runs of 2^(lowest target) consecutive amplitudes; from 8
+  // (a full 128 B line) up they stay efficient. Tile when the lowest target is below bit 3, or
+  // when a 3- or 4-qubit operator has two targets below bit 6 (measured on gfx950; 5-qubit
+  // operators keep the direct kernel there).
   int below6 = 0;
   for (int i = 0; i < k; ++i) below6 += sorted[i] < 6;
-  if (below6 == 0 || (below6 == 1 && k <= 2)) return false;
+  if (k > 4) return false;
+  const bool split_low = sorted[0] < 3;
+  const bool crowded = below6 >= 2 && k >= 3 && k <= 4;
+  if (!split_low && !crowded) return false;
   if ((int)(k + 6) > nbits) return false;
   t->b = 6 + low;
   t->h = k - low;

@@ -75,9 +75,21 @@ __global__ void __launch_bounds__(kBlock)
   block_sum_store<NV>(acc, partial + ((size_t)b * nb + blockIdx.x) * NV);
 }
 
-// Sum the partials of each state and expand to a full Hermitian 2^K x 2^K complex matrix.
+// Sum, in a fixed order, the nb partials of every value: one workgroup per (state, value), a
+// strided per-thread sum then a wavefront/LDS tree. out_sum[b * nv + v].
+__global__ void __launch_bounds__(kBlock)
+    k_sum_values(const double* __restrict__ partial, uint32_t nb, uint32_t nv,
+                 double* __restrict__ out_sum) {
+  const uint32_t v = blockIdx.x, b = blockIdx.y;
+  const double* p = partial + ((size_t)b * nb) * nv + v;
+  double acc[1] = {0};
+  for (uint32_t j = threadIdx.x; j < nb; j += kBlock) acc[0] += p[(size_t)j * nv];
+  block_sum_store<1>(acc, out_sum + (size_t)b * nv + v);
+}
+
+// Expand the summed values of each state to a full Hermitian 2^K x 2^K complex matrix.
 template <int K>
-__global__ void k_rdm_finish(const double* __restrict__ partial, uint32_t nb, uint64_t batch,
+__global__ void k_rdm_finish(const double* __restrict__ sums, uint64_t batch,
                              double2* __restrict__ out) {
   constexpr int D = 1 << K;
   constexpr int NV = D * D;
@@ -85,24 +97,17 @@ __global__ void k_rdm_finish(const double* __restrict__ partial, uint32_t nb, ui
   if (id >= batch * NV) return;
   const uint64_t b = id / NV;
   const int rc = (int)(id % NV), r = rc / D, c = rc % D;
-  const double* p = partial + b * nb * NV;
+  const double* p = sums + b * NV;
   double2 v;
   if (r == c) {
-    double s = 0;
-    for (uint32_t j = 0; j < nb; ++j) s += p[(size_t)j * NV + r];
-    v.x = s;
+    v.x = p[r];
     v.y = 0;
   } else {
     const int lo = r < c ? r : c, hi = r < c ? c : r;
     // position of pair (lo, hi) among the pairs r < c in row-major order
     const int pair = lo * D - lo * (lo + 1) / 2 + (hi - lo - 1);
-    double sx = 0, sy = 0;
-    for (uint32_t j = 0; j < nb; ++j) {
-      sx += p[(size_t)j * NV + D + 2 * pair];
-      sy += p[(size_t)j * NV + D + 2 * pair + 1];
-    }
-    v.x = sx;
-    v.y = r < c ? sy : -sy;
+    v.x = p[D + 2 * pair];
+    v.y = r < c ? p[D + 2 * pair + 1] : -p[D + 2 * pair + 1];
   }
   out[b * NV + rc] = v;
 }
@@ -180,14 +185,6 @@ __global__ void __launch_bounds__(kBlock)
   block_sum_store<1>(acc, partial + ((size_t)b * nterms + t) * nb + blockIdx.x);
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partial, uint32_t nb, uint64_t count,
-                               double* __restrict__ out) {
-  const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= count) return;
-  double s = 0;
-  for (uint32_t j = 0; j < nb; ++j) s += partial[id * nb + j];
-  out[id] = s;
-}
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
@@ -199,6 +196,15 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t b = id / dim, i = id % dim;
   const double v = (double)rho[(b << (2 * n)) + i * dim + i].x;
   probs[id] = v > 0.0 ? v : 0.0;
+}
+
+// out[r] = sum of the nb consecutive partials of row r, one workgroup per row (fixed order).
+static void sum_rows(const double* partial, uint32_t nb, uint64_t rows, double* out,
+                     hipStream_t s) {
+  for (uint64_t r0 = 0; r0 < rows; r0 += kMaxGridY) {
+    const uint32_t gy = (uint32_t)(rows - r0 < kMaxGridY ? rows - r0 : kMaxGridY);
+    k_sum_values<<<dim3(1, gy), kBlock, 0, s>>>(partial + r0 * nb, nb, 1, out + r0);
+  }
 }
 
 static int check_common(const void* states, int dtype, int nbits, long long batch) {
@@ -238,7 +244,9 @@ size_t lsim_reduce_workspace_bytes(int nbits, long long batch, int width) {
   const uint64_t nb = reduce_blocks(1ull << (nbits - 1));
   const uint64_t per = width <= 2 ? (uint64_t)(1 << (2 * width)) : (uint64_t)width;
   const uint64_t per_rdm = 16;
-  return (size_t)(batch * nb * (per > per_rdm ? per : per_rdm) * sizeof(double));
+  const uint64_t per_state = per > per_rdm ? per : per_rdm;
+  // partials, then one summed value per (state, value) for the reduced density matrices
+  return (size_t)((batch * nb * per_state + batch * per_rdm) * sizeof(double));
 }
 
 int lsim_sv_reduced_dm(const void* states, int dtype, int nbits, long long batch, int k,
@@ -270,12 +278,19 @@ int lsim_sv_reduced_dm(const void* states, int dtype, int nbits, long long batch
                                                 workspace, (uint32_t)b0);
     }
   }
+  const uint32_t nv = 1u << (2 * k);
+  double* sums = workspace + (size_t)batch * nb * nv;
+  for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
+    k_sum_values<<<dim3(nv, gy), kBlock, 0, s>>>(workspace + (size_t)b0 * nb * nv, nb, nv,
+                                               sums + (size_t)b0 * nv);
+  }
   const uint64_t count = (uint64_t)batch << (2 * k);
   const unsigned fb = (unsigned)((count + kBlock - 1) / kBlock);
   if (k == 1)
-    k_rdm_finish<1><<<fb, kBlock, 0, s>>>(workspace, nb, (uint64_t)batch, (double2*)out);
+    k_rdm_finish<1><<<fb, kBlock, 0, s>>>(sums, (uint64_t)batch, (double2*)out);
   else
-    k_rdm_finish<2><<<fb, kBlock, 0, s>>>(workspace, nb, (uint64_t)batch, (double2*)out);
+    k_rdm_finish<2><<<fb, kBlock, 0, s>>>(sums, (uint64_t)batch, (double2*)out);
   return check_launch("reduced density matrix");
 }
 
@@ -299,9 +314,7 @@ int lsim_sv_pauli_expect(const void* states, int dtype, int nbits, long long bat
       k_pauli_sv<float><<<grid, kBlock, 0, s>>>((const float2*)states, nbits, nb, nterms, terms,
                                                 workspace, (uint32_t)b0);
   }
-  const uint64_t count = (uint64_t)batch * nterms;
-  k_sum_partials<<<(unsigned)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(workspace, nb, count,
-                                                                           out);
+  sum_rows(workspace, nb, (uint64_t)batch * nterms, out, s);
   return check_launch("Pauli expectation");
 }
 
@@ -340,9 +353,7 @@ int lsim_dm_pauli_expect(const void* rho, int dtype, int n, long long batch, int
       k_pauli_dm<float><<<grid, kBlock, 0, s>>>((const float2*)rho, n, nb, nterms, terms,
                                                 workspace, (uint32_t)b0);
   }
-  const uint64_t count = (uint64_t)batch * nterms;
-  k_sum_partials<<<(unsigned)((count + kBlock - 1) / kBlock), kBlock, 0, s>>>(workspace, nb, count,
-                                                                           out);
+  sum_rows(workspace, nb, (uint64_t)batch * nterms, out, s);
   return check_launch("density-matrix Pauli expectation");
 }
 

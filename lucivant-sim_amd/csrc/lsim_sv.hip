@@ -88,14 +88,17 @@ __global__ void __launch_bounds__(64)
         const uint32_t s = t.high[j];
         base = ((base >> s) << (s + 1)) | (base & ((1u << s) - 1u));
       }
-    // every lane loads M / 64 = D amplitudes, 64 consecutive per instruction
+    // every lane loads M / 64 = D amplitudes, 64 consecutive per instruction. Row r of 64
+    // lies in segment r >> (b - 6) at offset (r & (2^(b-6) - 1)) * 64, a wavefront-uniform
+    // address: each load is a scalar base plus the lane number.
+    const uint32_t rsh = t.b - 6u, rmask = (1u << rsh) - 1u;
 #pragma unroll
     for (int r = 0; r < D; ++r) {
-      const uint32_t i = (uint32_t)r * 64u + lane;  // tile-local linear index
-      const uint32_t seg = i >> t.b, off = i & (seglen - 1u);
-      tile[swz(i)] = st[base + t.seg[seg] + off];
+      const uint32_t row = base + t.seg[(uint32_t)r >> rsh] + (((uint32_t)r & rmask) << 6);
+      tile[swz((uint32_t)r * 64u + lane)] = st[row + lane];
     }
     (void)nseg;
+    (void)seglen;
     __syncthreads();
     uint32_t g = lane;
 #pragma unroll
@@ -125,9 +128,8 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < D; ++r) {
-      const uint32_t i = (uint32_t)r * 64u + lane;
-      const uint32_t seg = i >> t.b, off = i & (seglen - 1u);
-      st[base + t.seg[seg] + off] = tile[swz(i)];
+      const uint32_t row = base + t.seg[(uint32_t)r >> rsh] + (((uint32_t)r & rmask) << 6);
+      st[row + lane] = tile[swz((uint32_t)r * 64u + lane)];
     }
     __syncthreads();
   }
@@ -256,8 +258,7 @@ static int launch_dense_tiled(void* states, int nbits, long long batch, int k, c
       case 2: k_dense_tiled<2, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
       case 3: k_dense_tiled<3, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
       case 4: k_dense_tiled<4, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
-      case 5: k_dense_tiled<5, R><<<grid, block, 0, s>>>(st, nbits, tiles, m, sel, (uint32_t)b0, t); break;
-      default: return fail(LSIM_EINVAL, "tiled dense operator needs k between 2 and 5");
+      default: return fail(LSIM_EINVAL, "tiled dense operator needs k between 2 and 4");
     }
   }
   return check_launch("tiled dense operator");

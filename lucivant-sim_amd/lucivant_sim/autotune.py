@@ -1,0 +1,132 @@
+"""Autotuning of fusion width and kernel variant, measured on the device, cached on disk.
+
+Two choices set the cost of a statevector simulation on the MI355X:
+
+- the fusion width (qubits per dense pass): wider passes mean fewer HBM sweeps but more FP64
+  work per amplitude (8 * 2^k flop per 32 B; k = 5 sits at the FP64 ridge of ~9.8 flop/B);
+- the dense-kernel variant for placements with low target bits: the LDS-tiled kernel or the
+  direct one.
+
+:func:`tune` plans a circuit at each candidate width, runs every candidate (with the tiled kernel
+on and off) on a scratch state of the same size, and times it with HIP events; the winner is stored
+per (gfx architecture, qubit count) in ``~/.cache/lucivant_sim/autotune.json`` (or
+``$LSIM_AUTOTUNE``) and applied by :func:`apply`, which ``simulator`` calls on first use. The LDS
+program path has its own knob, the fusion width of compiled programs (0 = one op per gate), tuned
+the same way by :func:`tune_program_width`.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+import torch
+
+from lucivant_sim import _native as nat
+from lucivant_sim.circuit import Circuit
+from lucivant_sim.engines import Statevector, run_passes
+from lucivant_sim.plan import plan_statevector
+
+CACHE = Path(os.environ.get("LSIM_AUTOTUNE", str(Path.home() / ".cache" / "lucivant_sim" / "autotune.json")))
+
+
+def _arch() -> str:
+    return str(nat.device_info()["arch"]).split(":")[0]
+
+
+def load_cache() -> dict[str, Any]:
+    try:
+        return json.loads(CACHE.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(data: dict[str, Any]) -> None:
+    CACHE.parent.mkdir(parents=True, exist_ok=True)
+    tmp = CACHE.with_suffix(".tmp")
+    tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+    tmp.replace(CACHE)
+
+
+def _time_passes(sv: Statevector, passes: list, reps: int) -> float:
+    stream = torch.cuda.current_stream()
+    run_passes(sv.buffer, sv.dtype, sv.nbits, 1, passes)  # warm-up
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        run_passes(sv.buffer, sv.dtype, sv.nbits, 1, passes)
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) * 1e-3 / reps
+
+
+def tune(circuit: Circuit, widths: tuple[int, ...] = (2, 3, 4, 5), reps: int = 2, store: bool = True) -> dict[str, Any]:
+    """Time ``circuit`` at every fusion width, tiled kernel on and off; returns (and by default
+    stores) the timings and the winner for this circuit's width."""
+    lib = nat.load()
+    sv = Statevector(circuit.num_qubits)
+    timings: dict[str, float] = {}
+    plans = {w: plan_statevector(circuit.operations, circuit.num_qubits, w) for w in widths}
+    old = lib.lsim_set_dense_tiling(1)
+    try:
+        for tiled in (1, 0):
+            lib.lsim_set_dense_tiling(tiled)
+            for w, passes in plans.items():
+                timings[f"w{w}_tiled{tiled}"] = _time_passes(sv, passes, reps)
+    finally:
+        lib.lsim_set_dense_tiling(old)
+    best = min(timings, key=timings.__getitem__)
+    width = int(best[1 : best.index("_")])
+    tiled = int(best[-1])
+    rec = {"width": width, "tiled": tiled, "seconds": timings[best], "timings": timings, "passes": {str(w): len(p) for w, p in plans.items()}}
+    if store:
+        data = load_cache()
+        data.setdefault(_arch(), {})[str(circuit.num_qubits)] = rec
+        save_cache(data)
+    return rec
+
+
+def best_for(num_qubits: int) -> dict[str, Any] | None:
+    """The stored choice for this device and width, or the nearest stored width."""
+    table = load_cache().get(_arch()) if torch.cuda.is_available() else None
+    if not table:
+        return None
+    if str(num_qubits) in table:
+        return table[str(num_qubits)]
+    nearest = min(table, key=lambda k: abs(int(k) - num_qubits))
+    return table[nearest]
+
+
+def apply(num_qubits: int) -> int:
+    """Fusion width to use for ``num_qubits`` (and set the tiling switch accordingly)."""
+    from lucivant_sim import simulator
+
+    rec = best_for(num_qubits)
+    if rec is None:
+        return simulator.FUSION_WIDTH
+    nat.load().lsim_set_dense_tiling(int(rec.get("tiled", 1)))
+    return int(rec["width"])
+
+
+def tune_program_width(circuits: list[Circuit], widths: tuple[int, ...] = (0, 2, 3), reps: int = 3) -> dict[str, float]:
+    """Seconds per batch of ``circuits`` as LDS programs compiled at each fusion width."""
+    from lucivant_sim import program as pg
+
+    out: dict[str, float] = {}
+    for w in widths:
+        progs = [pg.compile_static(c, w) for c in circuits]
+        pg.run_programs(progs, want_probs=True)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            pg.run_programs(progs, want_probs=True)
+            end.record()
+            torch.cuda.synchronize()
+            times.append(start.elapsed_time(end) * 1e-3)
+        out[f"w{w}"] = float(np.median(times))
+    return out

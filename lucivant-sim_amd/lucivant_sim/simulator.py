@@ -188,7 +188,7 @@ def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseMo
     """Static evolution (simulator.py:229-251): gates, their noise, resets of untouched qubits;
     measurements deferred. Runs as fused device passes."""
     if isinstance(state, Statevector):
-        state.run(plan_statevector(circuit.operations, circuit.num_qubits, width or FUSION_WIDTH))
+        state.run(plan_statevector(circuit.operations, circuit.num_qubits, width or _tuned_width(circuit.num_qubits)))
         return
     group: list[Pass] = []
     for item in plan_density(circuit.operations, circuit.num_qubits, noise):
@@ -200,6 +200,23 @@ def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseMo
         _, (channel, qubits) = item  # type: ignore[misc]
         state.run_kraus(channel, qubits)
     state.run(group)
+
+
+_TUNED: dict[int, int] = {}
+
+
+def _tuned_width(num_qubits: int) -> int:
+    """The autotuner's fusion width for this size when one is stored (see autotune.py), else
+    FUSION_WIDTH."""
+    if num_qubits not in _TUNED:
+        from lucivant_sim import autotune
+
+        try:
+            rec = autotune.best_for(num_qubits)
+        except Exception:  # noqa: BLE001 - an unreadable cache means the default
+            rec = None
+        _TUNED[num_qubits] = int(rec["width"]) if rec else 0
+    return _TUNED[num_qubits] or FUSION_WIDTH
 
 
 def _sample(
