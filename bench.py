@@ -326,7 +326,9 @@ def main() -> int:
         width = tuned["width"]
         nat.load().lsim_set_dense_tiling(tuned["tiled"])
     gates = sum(1 for op in circuit.operations if isinstance(op, ls.circuit.Gate))
-    passes = plan_statevector(circuit.operations, N, width)
+    low_bits = tuned["low_bits"] if tuned else 0
+    low_width = tuned["low_width"] if tuned else 0
+    passes = plan_statevector(circuit.operations, N, width, low_bits, low_width)
     dense = sum(1 for p in passes if p.kind == "dense")
 
     sv = Statevector(N)
@@ -413,6 +415,7 @@ def main() -> int:
             "qubits": N,
             "gates": gates,
             "fusion_width": width,
+            "low_qubit_width": low_width or None,
             "passes": len(passes),
             "dense_passes": dense,
             "bytes_per_pass": BYTES_PER_PASS,

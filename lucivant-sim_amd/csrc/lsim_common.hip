@@ -68,16 +68,14 @@ bool make_tiling(int nbits, int k, const int* targets, Tiling* t) {
   // low targets: the longest prefix with sorted[j] < 6 + (j + 1); then b = 6 + (#low)
   int low = 0;
   while (low < k && sorted[low] < (uint32_t)(6 + low + 1)) ++low;
-  // The direct kernel's lanes read runs of 2^(lowest target) consecutive amplitudes; from 8
-  // (a full 128 B line) up they stay efficient. Tile when the lowest target is below bit 3, or
-  // when a 3- or 4-qubit operator has two targets below bit 6 (measured on gfx950; 5-qubit
-  // operators keep the direct kernel there).
-  int below6 = 0;
-  for (int i = 0; i < k; ++i) below6 += sorted[i] < 6;
-  if (k > 4) return false;
-  const bool split_low = sorted[0] < 3;
-  const bool crowded = below6 >= 2 && k >= 3 && k <= 4;
-  if (!split_low && !crowded) return false;
+  // With targets listed from the highest bit down (the planner's order), the direct kernel
+  // fetches the amplitudes that share a 128 B line with consecutive instructions and stays near
+  // copy speed unless three or more targets crowd the lowest 32 amplitudes (512 B), where every
+  // lane of an instruction hits its own line: tile those 3- and 4-qubit operators (measured on
+  // gfx950; a 5-qubit tile does not fit the register budget).
+  int below5 = 0;
+  for (int i = 0; i < k; ++i) below5 += sorted[i] < 5;
+  if (k < 3 || k > 4 || below5 < 3) return false;
   if ((int)(k + 6) > nbits) return false;
   t->b = 6 + low;
   t->h = k - low;

@@ -28,6 +28,13 @@ __global__ void __launch_bounds__(kBlock)
     if (m < 0) return;
     U += (size_t)m * (D * D);
   }
+  // A 5-qubit operator is 16 KiB: staged once per workgroup in LDS and read as a wavefront-
+  // uniform broadcast, instead of streaming it through the scalar cache for every group.
+  __shared__ V ulds[K == 5 ? D * D : 1];
+  if constexpr (K == 5) {
+    for (int e = threadIdx.x; e < D * D; e += kBlock) ulds[e] = U[e];
+    __syncthreads();
+  }
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   if (g >= groups) return;
   V* __restrict__ st = states + ((size_t)b << nbits);
@@ -40,7 +47,7 @@ __global__ void __launch_bounds__(kBlock)
     R re = 0, im = 0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const V u = U[r * D + c];
+      const V u = K == 5 ? ulds[r * D + c] : U[r * D + c];
       re = fma(u.x, in[c].x, re);
       re = fma(-u.y, in[c].y, re);
       im = fma(u.x, in[c].y, im);

@@ -63,25 +63,38 @@ def _time_passes(sv: Statevector, passes: list, reps: int) -> float:
     return start.elapsed_time(end) * 1e-3 / reps
 
 
-def tune(circuit: Circuit, widths: tuple[int, ...] = (2, 3, 4, 5), reps: int = 2, store: bool = True) -> dict[str, Any]:
-    """Time ``circuit`` at every fusion width, tiled kernel on and off; returns (and by default
-    stores) the timings and the winner for this circuit's width."""
+#: (width, low_width) candidates: blocks touching qubits 0..2 may be held narrower (0: no cap)
+CANDIDATES = ((3, 0), (4, 0), (5, 0), (4, 3), (5, 3), (5, 4))
+LOW_BITS = 3
+
+
+def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES, reps: int = 2, store: bool = True) -> dict[str, Any]:
+    """Time ``circuit`` for every (width, low_width) candidate, tiled kernel on and off; returns
+    (and by default stores) the timings and the winner for this circuit's width."""
     lib = nat.load()
     sv = Statevector(circuit.num_qubits)
     timings: dict[str, float] = {}
-    plans = {w: plan_statevector(circuit.operations, circuit.num_qubits, w) for w in widths}
+    plans = {c: plan_statevector(circuit.operations, circuit.num_qubits, c[0], LOW_BITS, c[1]) for c in candidates}
     old = lib.lsim_set_dense_tiling(1)
     try:
         for tiled in (1, 0):
             lib.lsim_set_dense_tiling(tiled)
-            for w, passes in plans.items():
-                timings[f"w{w}_tiled{tiled}"] = _time_passes(sv, passes, reps)
+            for (w, lw), passes in plans.items():
+                timings[f"w{w}_low{lw}_tiled{tiled}"] = _time_passes(sv, passes, reps)
     finally:
         lib.lsim_set_dense_tiling(old)
     best = min(timings, key=timings.__getitem__)
-    width = int(best[1 : best.index("_")])
-    tiled = int(best[-1])
-    rec = {"width": width, "tiled": tiled, "seconds": timings[best], "timings": timings, "passes": {str(w): len(p) for w, p in plans.items()}}
+    w_s, lw_s, t_s = best.split("_")
+    width, low_width, tiled = int(w_s[1:]), int(lw_s[3:]), int(t_s[5:])
+    rec = {
+        "width": width,
+        "low_width": low_width,
+        "low_bits": LOW_BITS,
+        "tiled": tiled,
+        "seconds": timings[best],
+        "timings": timings,
+        "passes": {f"w{w}_low{lw}": len(p) for (w, lw), p in plans.items()},
+    }
     if store:
         data = load_cache()
         data.setdefault(_arch(), {})[str(circuit.num_qubits)] = rec
@@ -100,15 +113,16 @@ def best_for(num_qubits: int) -> dict[str, Any] | None:
     return table[nearest]
 
 
-def apply(num_qubits: int) -> int:
-    """Fusion width to use for ``num_qubits`` (and set the tiling switch accordingly)."""
+def apply(num_qubits: int) -> tuple[int, int, int]:
+    """(width, low_bits, low_width) to plan ``num_qubits``-qubit circuits with; also sets the
+    tiling switch to the stored choice."""
     from lucivant_sim import simulator
 
     rec = best_for(num_qubits)
     if rec is None:
-        return simulator.FUSION_WIDTH
+        return simulator.FUSION_WIDTH, 0, 0
     nat.load().lsim_set_dense_tiling(int(rec.get("tiled", 1)))
-    return int(rec["width"])
+    return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0))
 
 
 def tune_program_width(circuits: list[Circuit], widths: tuple[int, ...] = (0, 2, 3), reps: int = 3) -> dict[str, float]:

@@ -25,7 +25,7 @@ from numpy.typing import NDArray
 from lucivant_sim import _native as nat
 from lucivant_sim.gates import Matrix
 from lucivant_sim.noise import Channel
-from lucivant_sim.plan import RESET_CHANNEL, Pass, is_diagonal, superop_of_gate
+from lucivant_sim.plan import RESET_CHANNEL, Pass, descending, is_diagonal, superop_of_gate
 
 #: Memory guards of the reference (engines.py:29-31): 2^26 amplitudes and 4^13 entries, 1 GiB each.
 MAX_STATEVECTOR_QUBITS = 26
@@ -107,7 +107,7 @@ def run_passes(buf: torch.Tensor, dtype: int, nbits: int, batch: int, passes: Se
 
 
 def _single(matrix: Matrix, bits: tuple[int, ...]) -> Pass:
-    m = np.asarray(matrix, dtype=np.complex128)
+    m, bits = descending(np.asarray(matrix, dtype=np.complex128), tuple(bits))
     if is_diagonal(m):
         return Pass("diag", bits, np.ascontiguousarray(np.diag(m)))
     return Pass("dense", bits, np.ascontiguousarray(m))

@@ -82,12 +82,19 @@ class _Block:
         self.gates += 1
 
 
-def _greedy(items: list[tuple[Matrix, tuple[int, ...]]], width: int) -> list[_Block]:
+def _greedy(
+    items: list[tuple[Matrix, tuple[int, ...]]], width: int, low_bits: int = 0, low_width: int = 0
+) -> list[_Block]:
+    """Merge each item into the latest block touching its qubits while the union has at most
+    ``width`` qubits -- at most ``low_width`` when the union includes a qubit below ``low_bits``
+    (operators on the lowest index bits cost more per pass; the autotuner sets both)."""
     blocks: list[_Block] = []
     last: dict[int, int] = {}  # qubit -> index of the latest block touching it
     for matrix, qubits in items:
         j = max((last.get(q, -1) for q in qubits), default=-1)
-        if j >= 0 and len(set(blocks[j].qubits) | set(qubits)) <= width:
+        union = set(blocks[j].qubits) | set(qubits) if j >= 0 else set()
+        limit = low_width if low_width and low_bits and min(union, default=low_bits) < low_bits else width
+        if j >= 0 and len(union) <= limit:
             blocks[j].absorb(matrix, qubits)
         else:
             blk = _Block(list(qubits), np.eye(1 << len(qubits), dtype=np.complex128))
@@ -99,9 +106,12 @@ def _greedy(items: list[tuple[Matrix, tuple[int, ...]]], width: int) -> list[_Bl
     return blocks
 
 
-def plan_statevector(ops: tuple[Operation, ...], num_qubits: int, width: int = 4) -> list[Pass]:
+def plan_statevector(
+    ops: tuple[Operation, ...], num_qubits: int, width: int = 4, low_bits: int = 0, low_width: int = 0
+) -> list[Pass]:
     """Passes for an ideal static circuit on a statevector (measurements deferred, resets of
-    untouched qubits and barriers are no-ops, as in simulator.py:229-251)."""
+    untouched qubits and barriers are no-ops, as in simulator.py:229-251). ``low_bits`` and
+    ``low_width`` optionally narrow the blocks that touch the lowest qubits."""
     if not 1 <= width <= KERNEL_MAX_BITS:
         raise ValueError(f"fusion width must be between 1 and {KERNEL_MAX_BITS}")
     items: list[tuple[Matrix, tuple[int, ...]]] = []
@@ -118,7 +128,7 @@ def plan_statevector(ops: tuple[Operation, ...], num_qubits: int, width: int = 4
         else:
             raise ValueError(f"a static statevector can't run {type(op).__name__}")
     passes: list[Pass] = []
-    for blk in _greedy(items, width):
+    for blk in _greedy(items, width, low_bits, low_width):
         passes.append(_as_pass(blk.matrix, tuple(blk.qubits), blk.gates))
     if phase != 1:
         if passes:
@@ -129,7 +139,24 @@ def plan_statevector(ops: tuple[Operation, ...], num_qubits: int, width: int = 4
     return passes
 
 
+def descending(matrix: Matrix, bits: tuple[int, ...]) -> tuple[Matrix, tuple[int, ...]]:
+    """The same operator with its targets listed from the highest bit down (the lowest bit is
+    then the matrix's least significant one). The dense kernel loads the amplitudes of a group in
+    matrix-index order, so amplitudes that share a 128-byte line -- those differing only in the
+    lowest target bits -- are then fetched by consecutive instructions."""
+    k = len(bits)
+    order = sorted(range(k), key=lambda i: -bits[i])
+    if order == list(range(k)):
+        return matrix, bits
+    idx = np.arange(1 << k)
+    old = np.zeros(1 << k, dtype=np.int64)
+    for j, i in enumerate(order):
+        old |= ((idx >> (k - 1 - j)) & 1) << (k - 1 - i)
+    return np.ascontiguousarray(matrix[np.ix_(old, old)]), tuple(bits[i] for i in order)
+
+
 def _as_pass(matrix: Matrix, bits: tuple[int, ...], gates: int) -> Pass:
+    matrix, bits = descending(matrix, bits)
     if is_diagonal(matrix):
         return Pass("diag", bits, np.ascontiguousarray(np.diag(matrix)), gates)
     return Pass("dense", bits, np.ascontiguousarray(matrix), gates)

@@ -182,7 +182,7 @@ def _compile_static_vectorised(circuit: Circuit) -> Program:
     for k in np.unique(ks):
         idx = np.flatnonzero(ks == k)
         d = 1 << int(k)
-        mats = np.stack([np.asarray(gates[i].matrix, dtype=np.complex128) for i in idx])
+        mats = np.array([gates[i].matrix for i in idx], dtype=np.complex128)
         off = ~np.eye(d, dtype=bool)
         diag = ~np.any(mats[:, off], axis=1)
         sizes[idx] = np.where(diag, d, d * d)

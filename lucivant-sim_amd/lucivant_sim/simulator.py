@@ -188,7 +188,11 @@ def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseMo
     """Static evolution (simulator.py:229-251): gates, their noise, resets of untouched qubits;
     measurements deferred. Runs as fused device passes."""
     if isinstance(state, Statevector):
-        state.run(plan_statevector(circuit.operations, circuit.num_qubits, width or _tuned_width(circuit.num_qubits)))
+        if width:
+            state.run(plan_statevector(circuit.operations, circuit.num_qubits, width))
+        else:
+            w, low_bits, low_width = _tuned(circuit.num_qubits)
+            state.run(plan_statevector(circuit.operations, circuit.num_qubits, w, low_bits, low_width))
         return
     group: list[Pass] = []
     for item in plan_density(circuit.operations, circuit.num_qubits, noise):
@@ -202,21 +206,21 @@ def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseMo
     state.run(group)
 
 
-_TUNED: dict[int, int] = {}
+_TUNED: dict[int, tuple[int, int, int]] = {}
 
 
-def _tuned_width(num_qubits: int) -> int:
-    """The autotuner's fusion width for this size when one is stored (see autotune.py), else
-    FUSION_WIDTH."""
+def _tuned(num_qubits: int) -> tuple[int, int, int]:
+    """(width, low_bits, low_width) from the autotuner's stored choice for this size (see
+    autotune.py), else (FUSION_WIDTH, 0, 0)."""
     if num_qubits not in _TUNED:
         from lucivant_sim import autotune
 
         try:
-            rec = autotune.best_for(num_qubits)
-        except Exception:  # noqa: BLE001 - an unreadable cache means the default
-            rec = None
-        _TUNED[num_qubits] = int(rec["width"]) if rec else 0
-    return _TUNED[num_qubits] or FUSION_WIDTH
+            _TUNED[num_qubits] = autotune.apply(num_qubits)
+        except Exception:  # noqa: BLE001 - no device facts or an unreadable cache: the default
+            _TUNED[num_qubits] = (0, 0, 0)
+    w, low_bits, low_width = _TUNED[num_qubits]
+    return (w or FUSION_WIDTH, low_bits, low_width)
 
 
 def _sample(
@@ -236,6 +240,12 @@ def _sample(
     outcomes = np.searchsorted(cumulative, rng.random(shots), side="right").astype(np.int64)
     outcomes = np.minimum(outcomes, len(dist) - 1)
     wide = circuit.num_clbits > 62
+    flips = noise is not None and any((e := noise.readout_error(q)) is not None and (e.p01 or e.p10) for q, _ in pairs)
+    if not wide and not flips and len(dist) <= 1 << 16:
+        # no readout draws: the bit pattern is a fixed function of the outcome, tabulated once
+        bits = _outcome_table(tuple(pairs), tuple(measured), len(dist))[outcomes]
+        values, counts = np.unique(bits, return_counts=True)
+        return dict(zip(circuit.format_many(values), counts.tolist()))
     bits = np.zeros(shots, dtype=object if wide else np.int64)
     for q, c in pairs:
         value = (outcomes >> position[q]) & 1
@@ -251,6 +261,24 @@ def _sample(
     if wide:
         return {circuit.format_bits(int(v)): int(n) for v, n in zip(values, counts)}
     return dict(zip(circuit.format_many(values), counts.tolist()))
+
+
+_OUTCOME_TABLES: dict[tuple, np.ndarray] = {}
+
+
+def _outcome_table(pairs: tuple[tuple[int, int], ...], measured: tuple[int, ...], size: int) -> np.ndarray:
+    """Classical bits for every marginal outcome index (bit position[q] of the index -> clbit c)."""
+    key = (pairs, measured, size)
+    table = _OUTCOME_TABLES.get(key)
+    if table is None:
+        position = {q: i for i, q in enumerate(measured)}
+        idx = np.arange(size, dtype=np.int64)
+        table = np.zeros(size, dtype=np.int64)
+        for q, c in pairs:
+            table |= ((idx >> position[q]) & 1) << c
+        if len(_OUTCOME_TABLES) < 256:
+            _OUTCOME_TABLES[key] = table
+    return table
 
 
 def run(

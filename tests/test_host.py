@@ -212,13 +212,16 @@ def _run_passes_on_cpu(psi: np.ndarray, passes: list[Pass], nbits: int) -> np.nd
     return psi
 
 
+@pytest.mark.parametrize("low_width", [0, 2, 3])
 @pytest.mark.parametrize("width", [1, 2, 3, 4, 5])
-def test_fused_statevector_passes_equal_sequential_gates(width: int) -> None:
+def test_fused_statevector_passes_equal_sequential_gates(width: int, low_width: int) -> None:
     src = wl.random_layers(7, 6, 11, two_qubit=("cx", "cz", "swap", "iswap"), measure=True)
     src = src.replace("c = measure q;", "gphase(0.3); ccx q[0], q[3], q[6]; c = measure q;")
     c = ls.parse(src)
-    passes = plan_statevector(c.operations, c.num_qubits, width)
+    passes = plan_statevector(c.operations, c.num_qubits, width, 3, low_width)
     assert all(p.k <= max(width, 3) for p in passes)
+    if low_width:
+        assert all(p.k <= max(low_width, 3) for p in passes if min(p.bits, default=9) < 3)
     psi0 = np.zeros(2**7, dtype=np.complex128)
     psi0[0] = 1
     assert np.allclose(_run_passes_on_cpu(psi0, passes, 7), orc.evolve_sv(c), atol=1e-12)
