@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -270,14 +271,13 @@ def bench_density_matrix() -> dict:
 
 
 def read_traffic() -> float | None:
-    """HBM bytes per dense launch from the committed rocprofv3 counter summary, if present."""
-    path = ROOT / "profiles" / "dense_gate_26q_traffic.json"
+    """HBM bytes per dense launch from the committed rocprofv3 counter summary, if present
+    (its ``dense_bytes_per_launch:`` line: FETCH_SIZE and WRITE_SIZE scaled by a plain copy)."""
+    path = ROOT / "profiles" / "dense_gate_26q_traffic.txt"
     if not path.exists():
         return None
-    try:
-        return float(json.loads(path.read_text())["bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+    m = re.search(r"^dense_bytes_per_launch:\s*([0-9.]+)", path.read_text(), re.MULTILINE)
+    return float(m.group(1)) if m else None
 
 
 def main() -> int:
@@ -285,7 +285,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--width", type=int, default=0, help="fusion width (0: the tuned default)")
+    ap.add_argument("--width", type=int, default=0, help="fusion width (0: run the autotuner and use its choice)")
+    ap.add_argument("--tiling", type=int, choices=(0, 1), default=None, help="force the LDS-tiled dense kernel off/on")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and trajectory legs")
     ap.add_argument(
@@ -315,6 +316,8 @@ def main() -> int:
     src, _labels = workloads.config_c2()
     circuit = ls.parse(src)
     tuned = None
+    if args.tiling is not None:
+        nat.load().lsim_set_dense_tiling(args.tiling)
     if args.width:
         width = args.width
     elif args.profile:

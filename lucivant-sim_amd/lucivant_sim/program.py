@@ -25,59 +25,22 @@ from lucivant_sim.engines import ptr, require_device, stream_handle
 from lucivant_sim.noise import NoiseModel
 from lucivant_sim.plan import is_diagonal
 
-OP_DENSE, OP_DIAG, OP_MIX, OP_KRAUS, OP_MEASURE, OP_RESET, OP_COND, OP_PHASE = range(8)
-
-MAX_FLAGS = 32
-MAX_CLBITS = 64
-
-
-class ProgOp(ctypes.Structure):
-    """Mirror of ``lsim_prog_op`` in include/lsim.h."""
-
-    _fields_ = [
-        ("kind", ctypes.c_ubyte),
-        ("k", ctypes.c_ubyte),
-        ("flag", ctypes.c_ubyte),
-        ("negate", ctypes.c_ubyte),
-        ("t", ctypes.c_ubyte * 5),
-        ("pad", ctypes.c_ubyte * 3),
-        ("mat", ctypes.c_int),
-        ("nb", ctypes.c_int),
-        ("aux", ctypes.c_int),
-        ("draw", ctypes.c_int),
-        ("clbit", ctypes.c_int),
-        ("guard_mask", ctypes.c_uint),
-        ("guard_value", ctypes.c_uint),
-        ("cmask", ctypes.c_ulonglong),
-        ("cpattern", ctypes.c_ulonglong),
-    ]
-
-
-OP_DTYPE = np.dtype(
-    [
-        ("kind", "u1"),
-        ("k", "u1"),
-        ("flag", "u1"),
-        ("negate", "u1"),
-        ("t", "u1", (5,)),
-        ("pad", "u1", (3,)),
-        ("mat", "<i4"),
-        ("nb", "<i4"),
-        ("aux", "<i4"),
-        ("draw", "<i4"),
-        ("clbit", "<i4"),
-        ("guard_mask", "<u4"),
-        ("guard_value", "<u4"),
-        ("cmask", "<u8"),
-        ("cpattern", "<u8"),
-    ],
-    align=True,
+from lucivant_sim.optable import (  # noqa: F401  (re-exported)
+    MAX_CLBITS,
+    MAX_FLAGS,
+    OP_COND,
+    OP_DENSE,
+    OP_DIAG,
+    OP_DTYPE,
+    OP_KRAUS,
+    OP_MEASURE,
+    OP_MIX,
+    OP_PHASE,
+    OP_RESET,
+    ProgOp,
+    Unsupported,
+    static_table,
 )
-assert OP_DTYPE.itemsize == ctypes.sizeof(ProgOp)
-
-
-class Unsupported(ValueError):
-    """The circuit needs something the LDS program kernel doesn't do (it runs elsewhere)."""
 
 
 _MAX_QUBITS: dict[int, int] = {}
@@ -158,59 +121,16 @@ def compile_static(circuit: Circuit, fuse_width: int = 0) -> Program:
 
 
 def _compile_static_vectorised(circuit: Circuit) -> Program:
-    from lucivant_sim.circuit import GlobalPhase
-
-    gates = []
-    phase = 1.0 + 0.0j
-    for op in circuit.operations:
-        if isinstance(op, Gate):
-            gates.append(op)
-        elif isinstance(op, GlobalPhase):
-            phase *= np.exp(1j * op.angle)
+    table, pool = static_table(circuit)
     prog = Program(circuit.num_qubits)
-    count = len(gates) + (phase != 1)
-    table = np.zeros(count, dtype=OP_DTYPE)
-    if not gates and phase == 1:
-        prog.table, prog.pool = table, np.zeros(0, dtype=np.complex128)
-        return prog
-    ks = np.fromiter((len(g.qubits) for g in gates), dtype=np.int64, count=len(gates))
-    if np.any(ks > 5):
-        raise Unsupported("operators on more than 5 qubits")
-    sizes = np.zeros(len(gates), dtype=np.int64)
-    kinds = np.zeros(len(gates), dtype=np.uint8)
-    groups = []
-    for k in np.unique(ks):
-        idx = np.flatnonzero(ks == k)
-        d = 1 << int(k)
-        mats = np.array([gates[i].matrix for i in idx], dtype=np.complex128)
-        off = ~np.eye(d, dtype=bool)
-        diag = ~np.any(mats[:, off], axis=1)
-        sizes[idx] = np.where(diag, d, d * d)
-        kinds[idx] = np.where(diag, OP_DIAG, OP_DENSE)
-        groups.append((idx, mats, diag, d))
-    offsets = np.concatenate([[0], np.cumsum(sizes)])
-    pool = np.empty(int(offsets[-1]) + (phase != 1), dtype=np.complex128)
-    for idx, mats, diag, d in groups:
-        if np.any(diag):
-            sel = idx[diag]
-            pool[offsets[sel][:, None] + np.arange(d)[None, :]] = np.diagonal(mats[diag], axis1=1, axis2=2)
-        if np.any(~diag):
-            sel = idx[~diag]
-            pool[offsets[sel][:, None] + np.arange(d * d)[None, :]] = mats[~diag].reshape(-1, d * d)
-    n = len(gates)
-    table["kind"][:n] = kinds
-    table["k"][:n] = ks
-    pad = (0, 0, 0, 0, 0)
-    table["t"][:n] = np.array([g.qubits + pad[len(g.qubits) :] for g in gates], dtype=np.uint8)
-    table["mat"][:n] = offsets[:-1]
-    table["aux"] = -1
-    table["clbit"] = -1
-    if phase != 1:
-        pool[-1] = phase
-        table["kind"][n] = OP_PHASE
-        table["mat"][n] = pool.size - 1
-    prog.table, prog.pool = table, pool
-    prog.mat_size = pool.size
+    prog.table, prog.pool, prog.mat_size = table, pool, pool.size
+    return prog
+
+
+def from_table(num_qubits: int, table: np.ndarray, pool: np.ndarray) -> Program:
+    """A Program around an op table and matrix pool built elsewhere (e.g. in a worker process)."""
+    prog = Program(num_qubits)
+    prog.table, prog.pool, prog.mat_size = table, pool, pool.size
     return prog
 
 

@@ -83,30 +83,92 @@ _LEX = re.compile(
 _PRAGMA = re.compile(r"(?m)^[ \t]*(?:#?pragma|@)[^\n]*")
 _SKIP = frozenset({"ws", "line_comment", "block_comment"})
 
+# A whole gate-call statement with literal-number arguments and indexed qubits, e.g.
+# ``u3(0.1, -2e-3, 3) q[0];`` or ``cx q[0], q[11];``: lexed as one ``simple`` token when it starts a
+# statement, so the common case costs one regex match instead of a dozen tokens. Statement
+# keywords are excluded; the parser reads such a token straight into a Gate, or lexes its text
+# into ordinary tokens when it is anything else (a user-defined gate, an error).
+_SNUM = r"-?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?"
+_SARG = r"([A-Za-z_]\w*)\s*\[\s*(\d+)\s*\]"
+_SARG_NC = r"[A-Za-z_]\w*\s*\[\s*\d+\s*\]"
+_SIMPLE = re.compile(
+    r"(?!(?:measure|reset|barrier|qreg|creg|qubit|bit|include|gate|opaque|if|else|for|while|box|delay"
+    r"|gphase|input|output|const|float|angle|int|uint|bool|def|defcal|cal|return|let|extern|ctrl|negctrl"
+    r"|inv|pow)\b)"
+    rf"([A-Za-z_]\w*)(?:\s*\(\s*({_SNUM}(?:\s*,\s*{_SNUM})*)\s*\)\s*|\s+)"
+    rf"({_SARG_NC}(?:\s*,\s*{_SARG_NC})*)\s*;"
+)
+_SIMPLE_ARG = re.compile(_SARG)
+
+
+class _SimpleTok(NamedTuple):
+    """A ``simple`` token with its regex groups (name, parameter text, qubit text)."""
+
+    kind: str
+    text: str
+    line: int
+    col: int
+    groups: tuple[str, str | None, str]
+_BOUNDARY = frozenset({";", "{", "}"})
+#: lex simple gate calls as single tokens (False: every statement takes the general path)
+FAST_STATEMENTS = True
+
+
+def _lex_span(text: str, line: int, col: int) -> list[Tok]:
+    """Ordinary tokens of ``text``, which starts at ``line``/``col`` of the source."""
+    out: list[Tok] = []
+    line_start = -(col - 1)
+    for m in _LEX.finditer(text):
+        kind = m.lastgroup
+        start = m.start()
+        if kind in _SKIP:
+            part = m.group()
+            nl = part.count("\n")
+            if nl:
+                line += nl
+                line_start = start + part.rfind("\n") + 1
+            continue
+        out.append(Tok(kind, m.group(), line, start - line_start + 1))  # type: ignore[arg-type]
+    return out
+
 
 def tokenize(source: str) -> list[Tok]:
     """Tokens of ``source`` with 1-based line/column, ending in one ``eof`` token. Pragma and
-    annotation lines are dropped (they don't change the circuit)."""
+    annotation lines are dropped (they don't change the circuit). Simple gate calls at the start
+    of a statement come as single ``simple`` tokens (see ``_SIMPLE``)."""
     source = _PRAGMA.sub("", source)
     out: list[Tok] = []
     append = out.append
-    line, line_start, pos = 1, 0, 0
-    for m in _LEX.finditer(source):
-        start = m.start()
-        if start != pos:
+    lex = _LEX.match
+    simple = _SIMPLE.match
+    line, line_start, pos, end = 1, 0, 0, len(source)
+    boundary = FAST_STATEMENTS
+    while pos < end:
+        if boundary:
+            m = simple(source, pos)
+            if m is not None:
+                text = m.group()
+                append(_SimpleTok("simple", text, line, pos - line_start + 1, m.groups()))
+                nl = text.count("\n")
+                if nl:
+                    line += nl
+                    line_start = pos + text.rfind("\n") + 1
+                pos = m.end()
+                continue
+        m = lex(source, pos)
+        if m is None:
             raise QasmError(f"unexpected character {source[pos]!r}", line, pos - line_start + 1)
         kind = m.lastgroup
-        pos = m.end()
+        start, pos = pos, m.end()
+        text = m.group()
         if kind in _SKIP:
-            text = m.group()
             nl = text.count("\n")
             if nl:
                 line += nl
                 line_start = start + text.rfind("\n") + 1
             continue
-        append(Tok(kind, m.group(), line, start - line_start + 1))  # type: ignore[arg-type]
-    if pos != len(source):
-        raise QasmError(f"unexpected character {source[pos]!r}", line, pos - line_start + 1)
+        append(Tok(kind, text, line, start - line_start + 1))  # type: ignore[arg-type]
+        boundary = FAST_STATEMENTS and kind == "op" and text in _BOUNDARY
     append(Tok("eof", "", line, pos - line_start + 1))
     return out
 
@@ -333,6 +395,8 @@ class _Reader:
 
     def statement(self) -> list[Operation]:
         t = self.peek()
+        if t.kind == "simple":
+            return self.simple_statement(t)
         word = t.text if t.kind == "id" else ""
         if t.kind == "op" and t.text == ";":
             self.take()
@@ -430,6 +494,41 @@ class _Reader:
                 return fast
             return self.gate_statement()
         raise self.err(f"unexpected {t.text!r}" if t.kind != "eof" else "unexpected end of program")
+
+    def simple_statement(self, tok: Tok) -> list[Operation]:
+        """A ``simple`` token: a built-in gate on in-range, distinct qubits becomes its Gate
+        directly (the same Gate :meth:`gate_statement` makes); anything else is lexed into
+        ordinary tokens and read by the general path, with its usual errors."""
+        name, params_text, args_text = tok.groups  # type: ignore[attr-defined]
+        spec = GATES.get(name)
+        if spec is not None and name not in self.gates:
+            # _SNUM admits only plain decimal literals, which float() reads exactly as _number does
+            params = [float(x) for x in params_text.split(",")] if params_text else []
+            qubits: list[int] = []
+            for reg_name, index_text in _SIMPLE_ARG.findall(args_text):
+                reg = self.qregs.get(reg_name)
+                index = int(index_text)
+                if reg is None or index >= reg.size:
+                    qubits = []
+                    break
+                qubits.append(reg.offset + index)
+            if qubits and len(params) == spec.params and len(qubits) == spec.qubits and len(set(qubits)) == len(qubits):
+                self.i += 1
+                pt = tuple(params)
+                return [Gate(canonical_name(name), tuple(qubits), gate_matrix(name, pt), pt)]
+        # the general path over this statement's ordinary tokens
+        sub = _lex_span(tok.text, tok.line, tok.col)
+        last = sub[-1]
+        saved, saved_i = self.toks, self.i
+        self.toks, self.i = sub + [Tok("eof", "", last.line, last.col + len(last.text))], 0
+        try:
+            ops = self.statement()
+            if self.peek().kind != "eof":
+                raise self.err(f"unexpected {self.peek().text!r}")
+        finally:
+            self.toks, self.i = saved, saved_i
+        self.i += 1
+        return ops
 
     def simple_gate_statement(self) -> list[Operation] | None:
         """``name(num, -num, ...) reg[int], reg[int], ...;`` for a built-in gate, read straight
