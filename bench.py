@@ -87,43 +87,67 @@ def plain_counts(circuit, shots: int, seed: int):  # type: ignore[no-untyped-def
     return _sample(circuit, sv.probabilities(), shots, None, np.random.default_rng(seed))
 
 
+def plain_run(text: str, shots: int, seed: int):  # type: ignore[no-untyped-def]
+    """``run`` on the plain GPU path: read the text, one launch per gate, sample, build the
+    Result (with its circuit digest), one circuit at a time."""
+    import time as _time
+
+    import lucivant_sim as ls
+    from lucivant_sim.batch import _result
+
+    t0 = _time.perf_counter()
+    c = ls.parse(text)
+    return _result(c, plain_counts(c, shots, seed), shots, seed, "statevector", None, t0)
+
+
 def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
-    """12-qubit circuits of the 12-qubit configuration's shape (20 layers of u3 + cx, 1000 shots
-    each, distinct seeds): plain = one circuit at a time, one launch per gate; batched = run_many,
-    one LDS-program launch for all. Both start from parsed circuits (parsing is host work common
-    to both and excluded)."""
+    """1024 distinct 12-qubit circuits of the 12-qubit configuration's shape (20 layers of u3 on
+    every qubit + cx on a random pairing, 1000 shots each, distinct seeds), from OpenQASM text to
+    Results. plain = ``plain_run`` per circuit; batched = ``run_many`` over all texts: read in the
+    front-end worker processes, one LDS-program launch, sampling for the whole batch in this
+    process. ``in_process`` = the same launch and sampling from already-read circuits, with no
+    worker processes (host work in one thread)."""
     import numpy as np
 
     import lucivant_sim as ls
-    from lucivant_sim import workloads
+    from lucivant_sim import frontend, workloads
     from lucivant_sim.batch import counts_many, run_many
 
-    circuits = [ls.parse(workloads.random_layers(12, 20, 5000 + i)) for i in range(count)]
+    texts = [workloads.random_layers(12, 20, 5000 + i) for i in range(count)]
     seeds = list(range(count))
-    run_many(circuits[:8], 1000, seeds=seeds[:8])  # warm-up (kernel load)
-    plain_counts(circuits[0], 1000, 0)
+    run_many(texts[:8], 1000, seeds=seeds[:8])  # warm-up (kernel load, worker imports)
+    plain_run(texts[0], 1000, 0)
     t0 = time.perf_counter()
     lat = []
+    plain_results = []
     for i in range(plain_count):
         a = time.perf_counter()
-        plain_counts(circuits[i], 1000, seeds[i])
+        plain_results.append(plain_run(texts[i], 1000, seeds[i]))
         lat.append(time.perf_counter() - a)
     plain = plain_count / (time.perf_counter() - t0)
     t0 = time.perf_counter()
-    res = counts_many(circuits, 1000, seeds)
+    res = run_many(texts, 1000, seeds=seeds)
     dt = time.perf_counter() - t0
     batched = count / dt
-    same = all(res[i] == plain_counts(circuits[i], 1000, seeds[i]) for i in range(4))
-    same = same and run_many(circuits[:2], 1000, seeds=seeds[:2])[1].counts == res[1]
+    same = all(res[i].counts == plain_results[i].counts and res[i].fingerprint == plain_results[i].fingerprint for i in range(plain_count))
+    circuits = [ls.parse(t) for t in texts]
+    counts_many(circuits[:8], 1000, seeds[:8])
+    t0 = time.perf_counter()
+    inproc = counts_many(circuits, 1000, seeds)
+    dt_in = time.perf_counter() - t0
+    same = same and all(inproc[i] == res[i].counts for i in range(count))
     return {
-        "workload": f"{count} distinct 12-qubit circuits, 20 layers u3 + cx, 1000 shots each, complex128",
+        "workload": f"{count} distinct 12-qubit circuits, 20 layers u3 + cx, 1000 shots each, complex128, OpenQASM text to Results",
         "plain_circuits_per_s": round(plain, 1),
         "plain_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "plain_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
         "batched_circuits_per_s": round(batched, 1),
         "batched_latency_ms": round(dt * 1e3, 2),
         "speedup": round(batched / plain, 2),
-        "counts_equal_plain": same,
+        "frontend_workers": frontend.workers(),
+        "in_process_circuits_per_s": round(count / dt_in, 1),
+        "in_process_note": "from already-read circuits, counts only (no reading, digests or Results): one thread of host work",
+        "results_equal_plain": same,
     }
 
 
@@ -287,6 +311,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=0, help="fusion width (0: run the autotuner and use its choice)")
     ap.add_argument("--tiling", type=int, choices=(0, 1), default=None, help="force the LDS-tiled dense kernel off/on")
+    ap.add_argument("--k5-split", type=int, default=None, help="rows of 5-qubit operators read through the scalar cache")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and trajectory legs")
     ap.add_argument(
@@ -304,6 +329,12 @@ def main() -> int:
         )
         return 2
 
+    if not args.no_extras and not args.profile:
+        # the front-end workers are forked before this process first uses the GPU
+        from lucivant_sim import frontend
+
+        frontend.start()
+
     import numpy as np
     import torch
 
@@ -318,6 +349,8 @@ def main() -> int:
     tuned = None
     if args.tiling is not None:
         nat.load().lsim_set_dense_tiling(args.tiling)
+    if args.k5_split is not None:
+        nat.call("lsim_set_dense_k5_split", args.k5_split)
     if args.width:
         width = args.width
     elif args.profile:
@@ -328,6 +361,7 @@ def main() -> int:
         tuned = autotune.tune(circuit, store=False)
         width = tuned["width"]
         nat.load().lsim_set_dense_tiling(tuned["tiled"])
+        nat.call("lsim_set_dense_k5_split", tuned["k5_split"])
     gates = sum(1 for op in circuit.operations if isinstance(op, ls.circuit.Gate))
     low_bits = tuned["low_bits"] if tuned else 0
     low_width = tuned["low_width"] if tuned else 0
@@ -419,6 +453,7 @@ def main() -> int:
             "gates": gates,
             "fusion_width": width,
             "low_qubit_width": low_width or None,
+            "k5_scalar_rows": int(nat.load().lsim_set_dense_k5_split(-1)),
             "passes": len(passes),
             "dense_passes": dense,
             "bytes_per_pass": BYTES_PER_PASS,
@@ -442,12 +477,16 @@ def main() -> int:
     if not args.no_extras:
         out["batched"] = bench_batched()
         out["batched_shots"] = bench_batched_shots()
-        out["serving"] = [bench_serving(q) for q in (5.0, 20.0)]
+        out["serving"] = [bench_serving(q) for q in (5.0, 20.0, 1000.0)]
         out["trajectories_20q"] = bench_trajectories()
         out["density_matrix_13q"] = bench_density_matrix()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(circuit)
     print(json.dumps(out), flush=True)
+    if not args.no_extras:
+        from lucivant_sim import frontend
+
+        frontend.stop()
     return 0
 
 

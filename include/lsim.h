@@ -66,6 +66,13 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
  * Returns the previous setting. No reference counterpart (a tuning switch). */
 int lsim_set_dense_tiling(int enabled);
 
+/* Select how the dense kernel reads a 5-qubit operator: its first `scalar_rows` rows through the
+ * scalar cache, the others from LDS (0, 8, 12, 16, 20, 24 or 32; the default is 0, all from
+ * LDS). Returns LSIM_OK, or LSIM_EINVAL for another value; a negative argument changes nothing
+ * and returns the current setting. A tuning choice only: every split gives the same results.
+ * No reference counterpart (engines.py:38-119 has one NumPy path); the autotuner sets it. */
+int lsim_set_dense_k5_split(int scalar_rows);
+
 /* psi <- diag(d) psi on k <= 5 target bits: the diagonal case of _apply_slices
  * (engines.py:91-104; rz, s, t, cz, cp). `diags`: M diagonals of 2^k entries; `sel` as above. */
 int lsim_sv_apply_diag(void* states, int dtype, int nbits, long long batch, int k,

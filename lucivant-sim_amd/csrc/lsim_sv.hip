@@ -14,13 +14,20 @@
 
 namespace lsim {
 
-template <int K, typename R>
+// rows of a 5-qubit operator the dense kernel reads through the scalar cache (the rest from LDS)
+static int g_k5_scalar_rows = 0;
+
+// SR: for K = 5, how many leading rows of U are read through the scalar cache; the other rows
+// are staged in LDS (see below). Ignored for K < 5 (every row through the scalar cache).
+template <int K, typename R, int SR = 1 << K>
 __global__ void __launch_bounds__(kBlock)
     k_dense(typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t groups,
             const typename Cx<R>::V* __restrict__ mats, const int* __restrict__ sel,
             uint32_t batch_base, Targets t) {
   using V = typename Cx<R>::V;
   constexpr int D = 1 << K;
+  // rows of a 5-qubit operator read from LDS
+  constexpr int LR = K == 5 ? D - SR : 0;
   const uint32_t b = batch_base + blockIdx.y;
   const V* __restrict__ U = mats;
   if (sel != nullptr) {
@@ -28,11 +35,14 @@ __global__ void __launch_bounds__(kBlock)
     if (m < 0) return;
     U += (size_t)m * (D * D);
   }
-  // A 5-qubit operator is 16 KiB: staged once per workgroup in LDS and read as a wavefront-
-  // uniform broadcast, instead of streaming it through the scalar cache for every group.
-  __shared__ V ulds[K == 5 ? D * D : 1];
-  if constexpr (K == 5) {
-    for (int e = threadIdx.x; e < D * D; e += kBlock) ulds[e] = U[e];
+  // A 5-qubit operator is 16 KiB, as large as the scalar cache. Rows SR.. are staged once per
+  // workgroup in LDS and read as wavefront-uniform broadcasts; rows ..SR-1 stream through the
+  // scalar cache. The split divides the operator reads between the two paths: an LDS broadcast
+  // of 16 B still returns 1 KiB per wavefront, so all rows from LDS make LDS the bottleneck,
+  // and all rows through the scalar cache thrash it.
+  __shared__ V ulds[LR > 0 ? LR * D : 1];
+  if constexpr (LR > 0) {
+    for (int e = threadIdx.x; e < LR * D; e += kBlock) ulds[e] = U[SR * D + e];
     __syncthreads();
   }
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
@@ -47,7 +57,7 @@ __global__ void __launch_bounds__(kBlock)
     R re = 0, im = 0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const V u = K == 5 ? ulds[r * D + c] : U[r * D + c];
+      const V u = (LR > 0 && r >= SR) ? ulds[(r - SR) * D + c] : U[r * D + c];
       re = fma(u.x, in[c].x, re);
       re = fma(-u.y, in[c].y, re);
       im = fma(u.x, in[c].y, im);
@@ -287,7 +297,17 @@ static int launch_dense(void* states, int nbits, long long batch, int k, const T
       case 2: k_dense<2, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
       case 3: k_dense<3, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
       case 4: k_dense<4, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-      case 5: k_dense<5, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+      case 5:
+        switch (g_k5_scalar_rows) {
+          case 0: k_dense<5, R, 0><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+          case 8: k_dense<5, R, 8><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+          case 12: k_dense<5, R, 12><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+          case 16: k_dense<5, R, 16><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+          case 20: k_dense<5, R, 20><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+          case 24: k_dense<5, R, 24><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+          default: k_dense<5, R, 32><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
+        }
+        break;
     }
   }
   return check_launch("dense operator");
@@ -318,6 +338,15 @@ int lsim_set_dense_tiling(int enabled) {
   const int old = g_tiling_enabled;
   g_tiling_enabled = enabled != 0;
   return old;
+}
+
+int lsim_set_dense_k5_split(int scalar_rows) {
+  if (scalar_rows < 0) return g_k5_scalar_rows;
+  LSIM_REQUIRE(scalar_rows == 0 || scalar_rows == 8 || scalar_rows == 12 || scalar_rows == 16 ||
+                   scalar_rows == 20 || scalar_rows == 24 || scalar_rows == 32,
+               "the 5-qubit split must be 0, 8, 12, 16, 20, 24 or 32 rows, not %d", scalar_rows);
+  g_k5_scalar_rows = scalar_rows;
+  return LSIM_OK;
 }
 
 int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,

@@ -18,6 +18,8 @@ import torch  # noqa: F401  (loads the HIP runtime the library links against)
 LIB_PATH = Path(os.environ.get("LSIM_LIBRARY", Path(__file__).with_name("liblsim.so")))
 
 C128, C64 = 0, 1
+#: status codes of include/lsim.h
+OK, EINVAL, EHIP = 0, 1, 2
 MAX_K = 5
 MAX_TERMS = 64
 
@@ -36,6 +38,7 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "lsim_sv_set_basis": ([_c_void, _c_int, _c_int, _c_ll, _c_ll, _c_void], _c_int),
     "lsim_sv_apply_dense": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_void, _c_void, _c_void], _c_int),
     "lsim_set_dense_tiling": ([_c_int], _c_int),
+    "lsim_set_dense_k5_split": ([_c_int], _c_int),
     "lsim_sv_apply_diag": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_void, _c_void, _c_void], _c_int),
     "lsim_sv_scale": ([_c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),
     "lsim_sv_axpy": ([_c_void, _c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),
@@ -81,6 +84,11 @@ def load() -> ctypes.CDLL:
         fn.restype = res
     _lib = lib
     return lib
+
+
+def loaded() -> bool:
+    """Whether this process has loaded the library (and so may hold device state)."""
+    return _lib is not None
 
 
 def call(name: str, *args: Any) -> None:

@@ -1,14 +1,16 @@
 """Autotuning of fusion width and kernel variant, measured on the device, cached on disk.
 
-Two choices set the cost of a statevector simulation on the MI355X:
+Three choices set the cost of a statevector simulation on the MI355X:
 
 - the fusion width (qubits per dense pass): wider passes mean fewer HBM sweeps but more FP64
   work per amplitude (8 * 2^k flop per 32 B; k = 5 sits at the FP64 ridge of ~9.8 flop/B);
 - the dense-kernel variant for placements with low target bits: the LDS-tiled kernel or the
-  direct one.
+  direct one;
+- for 5-qubit passes, how the operator's reads are split between the scalar cache and LDS.
 
 :func:`tune` plans a circuit at each candidate width, runs every candidate (with the tiled kernel
-on and off) on a scratch state of the same size, and times it with HIP events; the winner is stored
+on and off) on a scratch state of the same size, and times it with HIP events, then times the
+winner's 5-qubit split candidates; the winner is stored
 per (gfx architecture, qubit count) in ``~/.cache/lucivant_sim/autotune.json`` (or
 ``$LSIM_AUTOTUNE``) and applied by :func:`apply`, which ``simulator`` calls on first use. The LDS
 program path has its own knob, the fusion width of compiled programs (0 = one op per gate), tuned
@@ -66,6 +68,8 @@ def _time_passes(sv: Statevector, passes: list, reps: int) -> float:
 #: (width, low_width) candidates: blocks touching qubits 0..2 may be held narrower (0: no cap)
 CANDIDATES = ((3, 0), (4, 0), (5, 0), (4, 3), (5, 3), (5, 4))
 LOW_BITS = 3
+#: rows of a 5-qubit operator read through the scalar cache (the rest from LDS)
+K5_SPLITS = (0, 8, 16, 24, 32)
 
 
 def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES, reps: int = 2, store: bool = True) -> dict[str, Any]:
@@ -86,12 +90,29 @@ def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES,
     best = min(timings, key=timings.__getitem__)
     w_s, lw_s, t_s = best.split("_")
     width, low_width, tiled = int(w_s[1:]), int(lw_s[3:]), int(t_s[5:])
+    # the winner's 5-qubit passes: how their operator reads are split between scalar cache and LDS
+    split = int(lib.lsim_set_dense_k5_split(-1))
+    split_timings: dict[str, float] = {}
+    winner = plans[(width, low_width)]
+    if any(p.k == 5 for p in winner):
+        old_split = split
+        old_tiling = lib.lsim_set_dense_tiling(tiled)
+        try:
+            for rows in K5_SPLITS:
+                lib.lsim_set_dense_k5_split(rows)
+                split_timings[f"k5split{rows}"] = _time_passes(sv, winner, reps)
+        finally:
+            lib.lsim_set_dense_k5_split(old_split)
+            lib.lsim_set_dense_tiling(old_tiling)
+        split = int(min(split_timings, key=split_timings.__getitem__)[len("k5split") :])
+        timings.update(split_timings)
     rec = {
         "width": width,
         "low_width": low_width,
         "low_bits": LOW_BITS,
         "tiled": tiled,
-        "seconds": timings[best],
+        "k5_split": split,
+        "seconds": min([timings[best], *split_timings.values()]),
         "timings": timings,
         "passes": {f"w{w}_low{lw}": len(p) for (w, lw), p in plans.items()},
     }
@@ -121,7 +142,9 @@ def apply(num_qubits: int) -> tuple[int, int, int]:
     rec = best_for(num_qubits)
     if rec is None:
         return simulator.FUSION_WIDTH, 0, 0
-    nat.load().lsim_set_dense_tiling(int(rec.get("tiled", 1)))
+    lib = nat.load()
+    lib.lsim_set_dense_tiling(int(rec.get("tiled", 1)))
+    lib.lsim_set_dense_k5_split(int(rec.get("k5_split", 0)))
     return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0))
 
 

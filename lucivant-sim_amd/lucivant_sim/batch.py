@@ -17,16 +17,19 @@ from typing import Any
 
 import numpy as np
 
-from lucivant_sim import __version__
+from lucivant_sim import __version__, frontend
 from lucivant_sim import _native as nat
 from lucivant_sim import program as pg
 from lucivant_sim.circuit import Circuit
 from lucivant_sim.engines import SimulationError, pauli_masks
+from lucivant_sim.frontend import Compiled
 from lucivant_sim.noise import NoiseModel
-from lucivant_sim.simulator import MAX_SHOTS, Result, _sample, _static, choose_method, expectation, run, to_circuit
+from lucivant_sim.optable import static_tables
+from lucivant_sim.simulator import METHODS, MAX_SHOTS, Result, _sample_many, _static, choose_method, expectation, run, to_circuit
 
 
-def _result(circuit: Circuit, counts: dict[str, int], shots: int, seed: int, method: str, noise: NoiseModel | None, t0: float, vector=None) -> Result:  # type: ignore[no-untyped-def]
+def _result(circuit: Any, counts: dict[str, int], shots: int, seed: int, method: str, noise: NoiseModel | None, t0: float, vector=None) -> Result:  # type: ignore[no-untyped-def]
+    digest = circuit.digest if isinstance(circuit, Compiled) else circuit.digest()
     return Result(
         counts=dict(sorted(counts.items())),
         shots=shots,
@@ -34,7 +37,7 @@ def _result(circuit: Circuit, counts: dict[str, int], shots: int, seed: int, met
         method=method,
         num_qubits=circuit.num_qubits,
         num_clbits=circuit.num_clbits,
-        circuit_digest=circuit.digest(),
+        circuit_digest=digest,
         noise_digest=noise.digest() if noise is not None and not noise.is_empty() else None,
         version=__version__,
         elapsed_seconds=round(time.perf_counter() - t0, 6),
@@ -51,10 +54,12 @@ def counts_many(
 ) -> list[dict[str, int]]:
     """Counts of ideal static circuits of one width (each with its own seed) from one
     LDS-program launch: the counts ``run`` would return, without building Results."""
-    progs = [pg.compile_static(c) for c in circuits]
-    out = pg.run_programs(progs, dtype=dtype, want_probs=True)
+    if not circuits:
+        return []
+    table, pool, starts, rows = static_tables(circuits)
+    out = pg.run_table(circuits[0].num_qubits, table, pool, starts, rows, dtype=dtype, want_probs=True)
     assert out.probs is not None
-    return [_sample(c, out.probs[j], shots, noise, np.random.default_rng(s)) for j, (c, s) in enumerate(zip(circuits, seeds))]
+    return _sample_many(circuits, out.probs, shots, noise, seeds)
 
 
 def run_many(
@@ -68,10 +73,14 @@ def run_many(
     dtype: int = nat.C128,
     keep_statevector: bool = False,
 ) -> list[Result]:
-    """``run`` for every program; same-width small circuits share one LDS-program launch."""
+    """``run`` for every program; same-width small circuits share one LDS-program launch. When
+    the front-end pool runs (``frontend.start``) and every program is OpenQASM text, the programs
+    are read in the pool's worker processes."""
     if not 0 <= shots <= MAX_SHOTS:
         raise ValueError(f"shots must be between 0 and {MAX_SHOTS:,}")
     t0 = time.perf_counter()
+    if frontend.active() and programs and all(isinstance(p, str) for p in programs):
+        return _run_many_frontend(list(programs), shots, seeds, noise, method, params, dtype, keep_statevector, t0)
     circuits = [to_circuit(p, params) for p in programs]
     seeds = [secrets.randbelow(2**63) for _ in circuits] if seeds is None else [int(s) for s in seeds]
     if len(seeds) != len(circuits) or any(s < 0 for s in seeds):
@@ -85,18 +94,55 @@ def run_many(
             groups.setdefault((m, c.num_qubits), []).append(i)
         else:
             results[i] = run(c, shots, seed=seeds[i], noise=noise, method=m)
-    for (m, _n), idx in groups.items():
+    for (m, n), idx in groups.items():
         if m == "statevector":
-            progs = [pg.compile_static(circuits[i]) for i in idx]
-            out = pg.run_programs(progs, dtype=dtype, want_probs=True, want_state=keep_statevector)
+            group = [circuits[i] for i in idx]
+            try:
+                table, pool, starts, rows = static_tables(group)
+            except pg.Unsupported:
+                for i in idx:
+                    results[i] = run(circuits[i], shots, seed=seeds[i], noise=noise, method=m)
+                continue
+            out = pg.run_table(n, table, pool, starts, rows, dtype=dtype, want_probs=True, want_state=keep_statevector)
             assert out.probs is not None
+            counts = _sample_many(group, out.probs, shots, noise, [seeds[i] for i in idx])
             for j, i in enumerate(idx):
-                rng = np.random.default_rng(seeds[i])
-                counts = _sample(circuits[i], out.probs[j], shots, noise, rng)
                 vec = out.state[j].astype(np.complex128) if out.state is not None else None
-                results[i] = _result(circuits[i], counts, shots, seeds[i], m, noise, t0, vec)
+                results[i] = _result(circuits[i], counts[j], shots, seeds[i], m, noise, t0, vec)
         else:
             _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0)
+    return [r for r in results if r is not None]
+
+
+def _run_many_frontend(programs, shots, seeds, noise, method, params, dtype, keep_statevector, t0) -> list[Result]:  # type: ignore[no-untyped-def]
+    seeds = [secrets.randbelow(2**63) for _ in programs] if seeds is None else [int(s) for s in seeds]
+    if len(seeds) != len(programs) or any(s < 0 for s in seeds):
+        raise ValueError("give one non-negative seed per program")
+    if method not in METHODS:
+        raise ValueError(f"method must be one of {', '.join(METHODS)}")
+    compiled = frontend.compile_texts(programs, dict(params) if params else None, noise, method, pg.max_qubits(dtype))
+    for c in compiled:
+        if c.error is not None:
+            raise c.error
+    results: list[Result | None] = [None] * len(compiled)
+    tabled: dict[int, list[int]] = {}
+    traj: dict[int, list[int]] = {}
+    for i, c in enumerate(compiled):
+        if c.table is not None:
+            tabled.setdefault(c.num_qubits, []).append(i)
+        elif c.method == "trajectories" and 1 <= c.num_qubits <= pg.max_qubits(dtype):
+            traj.setdefault(c.num_qubits, []).append(i)
+        else:
+            results[i] = run(c.circuit, shots, seed=seeds[i], noise=noise, method=c.method)
+    for n, idx in tabled.items():
+        out = pg.run_tables(n, [compiled[i].table for i in idx], [compiled[i].pool for i in idx], dtype=dtype, want_probs=True, want_state=keep_statevector)
+        counts = _sample_many([compiled[i] for i in idx], out.probs, shots, noise, [seeds[i] for i in idx])
+        for j, i in enumerate(idx):
+            vec = out.state[j].astype(np.complex128) if out.state is not None else None
+            results[i] = _result(compiled[i], counts[j], shots, seeds[i], "statevector", noise, t0, vec)
+    circuits = [c.circuit for c in compiled]
+    for _n, idx in traj.items():
+        _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0)
     return [r for r in results if r is not None]
 
 

@@ -105,7 +105,8 @@ def matrix_json(matrix: np.ndarray) -> list[list[list[float]]]:
     if hit is not None:
         return hit
     rounded = np.round(np.asarray(matrix, dtype=np.complex128), 12) + 0.0
-    out = [[[float(z.real), float(z.imag)] for z in row] for row in rounded]
+    # (re, im) float64 pairs as nested Python floats, row by row
+    out = np.ascontiguousarray(rounded).view(np.float64).reshape(rounded.shape[0], rounded.shape[1], 2).tolist()
     if len(_JSON_CACHE) < 8192:
         _JSON_CACHE[key] = out
     return out

@@ -57,15 +57,20 @@ _NP_DTYPE = {nat.C128: np.complex128, nat.C64: np.complex64}
 
 
 class Workspace:
-    """Device scratch for the reductions, grown on demand and reused."""
+    """Device scratch for the reductions, grown on demand and reused; one buffer per stream, so
+    reductions queued on different streams never share scratch (a buffer is allocated on, and
+    only ever used by, its stream)."""
 
     def __init__(self) -> None:
-        self.buf: torch.Tensor | None = None
+        self.bufs: dict[int, torch.Tensor] = {}
 
     def get(self, nbytes: int) -> torch.Tensor:
-        if self.buf is None or self.buf.numel() * 8 < nbytes:
-            self.buf = torch.empty(max(nbytes // 8 + 1, 1024), dtype=torch.float64, device=require_device())
-        return self.buf
+        key = torch.cuda.current_stream().cuda_stream
+        buf = self.bufs.get(key)
+        if buf is None or buf.numel() * 8 < nbytes:
+            buf = torch.empty(max(nbytes // 8 + 1, 1024), dtype=torch.float64, device=require_device())
+            self.bufs[key] = buf
+        return buf
 
 
 WORKSPACE = Workspace()
@@ -345,18 +350,23 @@ def _expect(fn: str, buf: torch.Tensor, dtype: int, n: int, nbits: int, labels: 
     if n == 0:
         # a 0-qubit state is the scalar 1 (or rho = [[1]]): every (empty) string has value 1
         return np.ones(len(labels))
-    out = np.empty(len(labels))
+    return launch_expectations(fn, buf, dtype, n, nbits, labels).cpu().numpy()
+
+
+def launch_expectations(fn: str, buf: torch.Tensor, dtype: int, n: int, nbits: int, labels: Sequence[str]) -> torch.Tensor:
+    """Queue the Pauli-expectation reductions of ``labels`` on the current stream; returns the
+    device tensor that will hold the values (read it after the stream reaches this point)."""
+    labels = list(labels)
+    res = torch.empty(max(len(labels), 1), dtype=torch.float64, device=buf.device)
     stream = stream_handle()
     for start in range(0, len(labels), nat.MAX_TERMS):
         chunk = labels[start : start + nat.MAX_TERMS]
         masks = [pauli_masks(lbl) for lbl in chunk]
         xs = nat.ull_array([m[0] for m in masks])
         zs = nat.ull_array([m[1] for m in masks])
-        res = torch.empty(len(chunk), dtype=torch.float64, device=buf.device)
         ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(nbits, 1, len(chunk)))
-        nat.call(fn, ptr(buf), dtype, n, 1, len(chunk), xs, zs, ptr(res), ptr(ws), stream)
-        out[start : start + len(chunk)] = res.cpu().numpy()
-    return out
+        nat.call(fn, ptr(buf), dtype, n, 1, len(chunk), xs, zs, ptr(res) + 8 * start, ptr(ws), stream)
+    return res[: len(labels)]
 
 
 def marginal(probabilities: NDArray[np.float64], num_qubits: int, qubits: list[int]) -> NDArray[np.float64]:

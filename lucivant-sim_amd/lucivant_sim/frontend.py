@@ -1,0 +1,195 @@
+"""Front-end worker processes: the host-side reading of circuits, in parallel.
+
+For a batch of small circuits the device side is one LDS-program launch; what remains per
+circuit is host work in Python: reading the OpenQASM text, checking the circuit, choosing its
+method, hashing it for the result digest, and building its op table. :func:`start` forks a pool
+of worker processes that do this work for circuits given as text (workers use NumPy only and
+never touch the GPU); :func:`compile_texts` returns, per program, a :class:`Compiled` summary that
+carries everything the launching process needs: the op table and matrix pool for the LDS program
+kernel, the measured (qubit, clbit) pairs and register layout for sampling, and the digest.
+``batch.run_many`` and ``serving.Server`` use the pool when it is running.
+
+The pool is forked, so it must be started before this process first uses the GPU (children must
+not inherit a live device context): :func:`start` returns 0 and starts nothing when the GPU is
+already in use, and callers then keep the in-process path.
+"""
+
+from __future__ import annotations
+
+import multiprocessing
+import os
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from lucivant_sim.circuit import Circuit, Register
+
+_POOL: Any = None
+_WORKERS = 0
+
+
+@dataclass
+class Compiled:
+    """One program as read by a worker. ``table``/``pool`` are set for ideal static circuits the
+    LDS program kernel can run; ``passes`` for wider statevectors when fusion widths were given;
+    ``circuit`` when the launching process needs the whole circuit (dynamic circuits, noisy runs,
+    wide states otherwise); ``terms`` are an expectation request's checked Pauli terms; ``error``
+    holds the exception reading the program raised, re-raised by the caller."""
+
+    num_qubits: int = 0
+    num_clbits: int = 0
+    cregs: tuple[Register, ...] = ()
+    pairs: tuple[tuple[int, int], ...] = ()
+    digest: str = ""
+    method: str = ""
+    table: np.ndarray | None = None
+    pool: np.ndarray | None = None
+    circuit: Circuit | None = None
+    terms: dict[str, float] | None = None
+    passes: list | None = None
+    error: BaseException | None = None
+    _shell: Circuit | None = field(default=None, repr=False)
+
+    # the circuit interface that sampling and results use
+    def final_measurements(self) -> list[tuple[int, int]]:
+        return list(self.pairs)
+
+    def _formatter(self) -> Circuit:
+        if self._shell is None:
+            self._shell = Circuit(self.num_qubits, self.num_clbits, (), (), self.cregs)
+        return self._shell
+
+    def format_many(self, values: np.ndarray) -> list[str]:
+        return self._formatter().format_many(values)
+
+    def format_bits(self, bits: int) -> str:
+        return self._formatter().format_bits(bits)
+
+
+def default_workers() -> int:
+    """Worker count: the CPUs this process may run on, less one for the launching process, at
+    most 15."""
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover - non-Linux
+        cpus = os.cpu_count() or 2
+    env = os.environ.get("LSIM_FRONTEND_WORKERS")
+    if env:
+        return max(1, int(env))
+    return max(1, min(15, cpus - 1))
+
+
+def start(workers: int | None = None) -> int:
+    """Start the worker pool; returns its size (0: not started because the GPU is in use)."""
+    global _POOL, _WORKERS
+    if _POOL is not None:
+        return _WORKERS
+    import torch
+
+    from lucivant_sim import _native as nat
+
+    if torch.cuda.is_initialized() or (nat.loaded() and torch.cuda.is_available()):
+        return 0
+    n = workers or default_workers()
+    _POOL = multiprocessing.get_context("fork").Pool(n)
+    _WORKERS = n
+    return n
+
+
+def stop() -> None:
+    global _POOL, _WORKERS
+    if _POOL is not None:
+        _POOL.close()
+        _POOL.join()
+    _POOL, _WORKERS = None, 0
+
+
+def active() -> bool:
+    return _POOL is not None
+
+
+def workers() -> int:
+    return _WORKERS
+
+
+def compile_one(
+    src: Any,
+    params: dict | None,
+    noise: Any,
+    method: str,
+    lds_qubits: int,
+    observable: Any = None,
+    widths: dict[int, tuple[int, int, int]] | None = None,
+    digest: bool = True,
+) -> Compiled:
+    """Read one program (runs in a worker; also callable in-process).
+
+    ``observable`` set: an expectation request (labels checked here, statevector method).
+    ``widths`` set: statevectors wider than ``lds_qubits`` also get their fused passes, planned
+    with ``widths[num_qubits]`` = (fusion width, low bits, low width). ``digest`` False skips the
+    circuit digest (callers that build no Result)."""
+    from lucivant_sim.optable import Unsupported, static_table
+    from lucivant_sim.plan import plan_statevector
+    from lucivant_sim.simulator import choose_method, to_circuit
+
+    try:
+        c = to_circuit(src, params)
+        terms = None
+        if observable is not None:
+            terms = {observable: 1.0} if isinstance(observable, str) else {str(k): float(v) for k, v in dict(observable).items()}
+            for label in terms:
+                if len(label) != c.num_qubits or set(label.upper()) - set("IXYZ"):
+                    raise ValueError(f"{label!r} must be a Pauli string of length {c.num_qubits} (qubit 0 rightmost)")
+            if c.is_dynamic():
+                raise ValueError("an expectation value needs a circuit without mid-circuit measurement")
+            chosen = "statevector"
+        else:
+            chosen = choose_method(c, noise, method)
+        out = Compiled(
+            num_qubits=c.num_qubits,
+            num_clbits=c.num_clbits,
+            cregs=c.cregs,
+            digest=c.digest() if digest else "",
+            method=chosen,
+            terms=terms,
+        )
+        if chosen != "statevector":
+            out.circuit = c
+            return out
+        out.pairs = tuple(c.final_measurements())
+        if 1 <= c.num_qubits <= lds_qubits:
+            try:
+                out.table, out.pool = static_table(c)
+                return out
+            except Unsupported:
+                pass
+        if widths is not None:
+            w, low_bits, low_width = widths.get(c.num_qubits, (4, 0, 0))
+            out.passes = plan_statevector(c.operations, c.num_qubits, w, low_bits, low_width)
+        else:
+            out.circuit = c
+        return out
+    except Exception as exc:  # noqa: BLE001 - handed back to the caller, raised there in order
+        return Compiled(error=exc)
+
+
+def _job(args: tuple) -> Compiled:
+    return compile_one(*args)
+
+
+def compile_texts(sources: list[str], params: dict | None, noise: Any, method: str, lds_qubits: int) -> list[Compiled]:
+    """Read many programs in the workers (results in program order)."""
+    if _POOL is None:
+        raise RuntimeError("the front-end pool isn't running")
+    jobs = [(s, params, noise, method, lds_qubits) for s in sources]
+    chunk = max(1, len(jobs) // (4 * _WORKERS))
+    return _POOL.map(_job, jobs, chunksize=chunk)
+
+
+def submit(*args: Any) -> Any:
+    """:func:`compile_one` of ``args`` in a worker, asynchronously: an ``AsyncResult`` whose
+    ``get()`` is the :class:`Compiled`."""
+    if _POOL is None:
+        raise RuntimeError("the front-end pool isn't running")
+    return _POOL.apply_async(_job, (args,))

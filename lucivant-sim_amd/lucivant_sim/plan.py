@@ -18,6 +18,7 @@ rounding only.
 from __future__ import annotations
 
 from dataclasses import dataclass
+from functools import lru_cache
 
 import numpy as np
 
@@ -29,9 +30,10 @@ from lucivant_sim.noise import Channel, NoiseModel
 KERNEL_MAX_BITS = 5
 
 
-def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
-    """``sub`` acting on local positions ``positions`` (first = its most significant bit) of a
-    k-position operator whose position j is index bit k-1-j, as a 2^k x 2^k matrix."""
+@lru_cache(maxsize=4096)
+def _embedding(positions: tuple[int, ...], k: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Index arrays of :func:`embed`: the sub-operator row/column of every (row, column) of the
+    k-position operator, and where the untouched positions agree."""
     m = len(positions)
     d = 1 << k
     idx = np.arange(d)
@@ -41,7 +43,19 @@ def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
         sub_idx |= ((idx >> (k - 1 - p)) & 1) << (m - 1 - i)
         mask |= 1 << (k - 1 - p)
     rest = idx & ~mask
-    return np.where(rest[:, None] == rest[None, :], sub[sub_idx[:, None], sub_idx[None, :]], 0).astype(np.complex128)
+    same = rest[:, None] == rest[None, :]
+    rows = np.broadcast_to(sub_idx[:, None], (d, d))
+    cols = np.broadcast_to(sub_idx[None, :], (d, d))
+    for a in (same, rows, cols):
+        a.flags.writeable = False
+    return same, rows, cols
+
+
+def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
+    """``sub`` acting on local positions ``positions`` (first = its most significant bit) of a
+    k-position operator whose position j is index bit k-1-j, as a 2^k x 2^k matrix."""
+    same, rows, cols = _embedding(tuple(positions), k)
+    return np.where(same, np.asarray(sub, dtype=np.complex128)[rows, cols], 0)
 
 
 def is_diagonal(m: Matrix) -> bool:
@@ -72,13 +86,26 @@ class _Block:
     def extend(self, qubits: tuple[int, ...]) -> None:
         new = [q for q in qubits if q not in self.qubits]
         if new:
-            self.matrix = np.kron(self.matrix, np.eye(1 << len(new), dtype=np.complex128))
+            # kron(matrix, I): the new qubits become the least significant positions
+            d, e = self.matrix.shape[0], 1 << len(new)
+            out = np.zeros((d * e, d * e), dtype=np.complex128)
+            diag = np.arange(e)
+            out.reshape(d, e, d, e)[:, diag, :, diag] = self.matrix
+            self.matrix = out
             self.qubits.extend(new)
 
     def absorb(self, matrix: Matrix, qubits: tuple[int, ...]) -> None:
         self.extend(qubits)
-        pos = [self.qubits.index(q) for q in qubits]
-        self.matrix = embed(matrix, pos, len(self.qubits)) @ self.matrix
+        k = len(self.qubits)
+        if len(qubits) == 1:
+            # a one-qubit gate on position p (index bit k-1-p) mixes row pairs p apart
+            p = self.qubits.index(qubits[0])
+            d = 1 << k
+            rows = self.matrix.reshape(1 << p, 2, (d >> (p + 1)) * d)
+            self.matrix = np.matmul(np.asarray(matrix, dtype=np.complex128), rows).reshape(d, d)
+        else:
+            pos = [self.qubits.index(q) for q in qubits]
+            self.matrix = embed(matrix, pos, k) @ self.matrix
         self.gates += 1
 
 

@@ -229,9 +229,6 @@ def run_programs(
     n = programs[0].n
     if any(p.n != n for p in programs):
         raise ValueError("programs of one launch must have the same width")
-    if n > max_qubits(dtype):
-        raise Unsupported(f"{n} qubits don't fit one workgroup's LDS")
-    device = require_device()
     table = np.zeros(sum(len(p.table) if p.table is not None else len(p.ops) for p in programs), dtype=OP_DTYPE)
     starts, counts = [], []
     mats, reals = [], []
@@ -266,18 +263,67 @@ def run_programs(
         reals.append(rp)
         mat_base += p.mat_size
         real_base += len(p.reals)
-    instances = len(programs) * repeat
+    mat_host = np.concatenate(mats) if mats else np.zeros(0, dtype=np.complex128)
+    real_host = np.concatenate(reals) if reals else np.zeros(0)
+    return run_table(
+        n,
+        table,
+        mat_host,
+        np.asarray(starts, dtype=np.int64),
+        np.asarray(counts, dtype=np.int64),
+        reals=real_host,
+        repeat=repeat,
+        dtype=dtype,
+        draws=draws,
+        want_clbits=want_clbits,
+        want_probs=want_probs,
+        want_state=want_state,
+        pauli=pauli,
+    )
+
+
+def run_table(
+    n: int,
+    table: np.ndarray,
+    pool: np.ndarray,
+    starts: np.ndarray,
+    counts: np.ndarray,
+    *,
+    reals: np.ndarray | None = None,
+    repeat: int = 1,
+    dtype: int = nat.C128,
+    draws: np.ndarray | None = None,
+    want_clbits: bool = False,
+    want_probs: bool = False,
+    want_state: bool = False,
+    pauli: list[list[tuple[int, int]]] | None = None,
+) -> BatchOutputs:
+    """:func:`run_programs` over programs already laid end to end: program j is rows
+    ``starts[j] : starts[j] + counts[j]`` of ``table``, whose matrix offsets index ``pool``
+    (``optable.static_tables`` builds this form directly)."""
+    programs = len(starts)
+    if programs == 0:
+        return BatchOutputs()
+    if n > max_qubits(dtype):
+        raise Unsupported(f"{n} qubits don't fit one workgroup's LDS")
+    device = require_device()
+    mat_base = int(pool.size)
+    instances = programs * repeat
     if draws is not None and draws.size and draws.shape[0] != instances:
         raise ValueError(f"draws has {draws.shape[0]} rows for {instances} instances")
     if pauli is not None:
-        if len(pauli) != len(programs) or len({len(t) for t in pauli}) > 1:
+        if len(pauli) != programs or len({len(t) for t in pauli}) > 1:
             raise ValueError("give every program the same number of Pauli strings")
         if any(x >> n or z >> n for terms in pauli for x, z in terms):
             raise ValueError("a Pauli mask names a qubit beyond the state")
-    op_start = np.repeat(np.asarray(starts, dtype=np.int32), repeat)
-    op_count = np.repeat(np.asarray(counts, dtype=np.int32), repeat)
-    mat_host = np.concatenate(mats) if mats else np.zeros(1, dtype=np.complex128)
-    real_host = np.concatenate(reals) if reals else np.zeros(1)
+    if table.dtype != OP_DTYPE or pool.dtype != np.complex128:
+        raise ValueError("the op table must have OP_DTYPE rows and the pool complex128 entries")
+    if np.any(starts < 0) or np.any(counts < 0) or np.any(starts + counts > len(table)):
+        raise ValueError("a program's rows lie outside the op table")
+    op_start = np.repeat(starts.astype(np.int32), repeat)
+    op_count = np.repeat(counts.astype(np.int32), repeat)
+    mat_host = pool if pool.size else np.zeros(1, dtype=np.complex128)
+    real_host = reals if reals is not None and reals.size else np.zeros(0)
     ndraws = draws.shape[1] if draws is not None and draws.size else 0
     _validate(table, n, mat_base, real_host, ndraws)
     if real_host.size == 0:
@@ -302,7 +348,7 @@ def run_programs(
     nterms = len(pauli[0]) if pauli else 0
     px = pz = ex = None
     if nterms:
-        masks = np.array(pauli, dtype=np.uint64).reshape(len(programs), nterms, 2)
+        masks = np.array(pauli, dtype=np.uint64).reshape(programs, nterms, 2)
         masks = np.repeat(masks, repeat, axis=0)
         px = up(np.ascontiguousarray(masks[:, :, 0]))
         pz = up(np.ascontiguousarray(masks[:, :, 1]))
@@ -337,6 +383,17 @@ def run_programs(
     if ex is not None:
         out.expect = ex.cpu().numpy().reshape(instances, nterms)
     return out
+
+
+def run_tables(n: int, tables: list[np.ndarray], pools: list[np.ndarray], **kwargs) -> BatchOutputs:  # type: ignore[no-untyped-def]
+    """:func:`run_table` over per-program op tables, each with its own matrix pool (as
+    ``optable.static_table`` returns them)."""
+    rows = np.fromiter(map(len, tables), dtype=np.int64, count=len(tables))
+    sizes = np.fromiter((p.size for p in pools), dtype=np.int64, count=len(pools))
+    table = np.concatenate(tables) if tables else np.zeros(0, dtype=OP_DTYPE)
+    table["mat"] += np.repeat(np.cumsum(sizes) - sizes, rows).astype(np.int32)
+    pool = np.concatenate(pools) if pools else np.zeros(0, dtype=np.complex128)
+    return run_table(n, table, pool, np.cumsum(rows) - rows, rows, **kwargs)
 
 
 def _validate(table: np.ndarray, n: int, mat_size: int, reals: np.ndarray, ndraws: int) -> None:

@@ -60,7 +60,12 @@ class QasmError(ValueError):
     def __init__(self, message: str, line: int = 0, column: int = 0) -> None:
         self.line = line
         self.column = column
+        self.message = message
         super().__init__(f"line {line}, column {column}: {message}" if line else message)
+
+    def __reduce__(self) -> tuple:
+        # keeps line and column when the error crosses a process boundary
+        return (type(self), (self.message, self.line, self.column))
 
 
 # ----------------------------------------------------------------------------------- lexing

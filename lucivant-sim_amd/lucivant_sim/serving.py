@@ -25,6 +25,7 @@ import itertools
 import queue
 import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Any, Mapping
 
@@ -32,12 +33,12 @@ import numpy as np
 import torch
 
 from lucivant_sim import _native as nat
+from lucivant_sim import frontend
 from lucivant_sim import program as pg
-from lucivant_sim.circuit import Circuit
-from lucivant_sim.engines import Statevector, pauli_masks, ptr, require_device, upload
+from lucivant_sim.engines import Statevector, launch_expectations, pauli_masks, ptr, require_device, upload
+from lucivant_sim.frontend import Compiled, compile_one
 from lucivant_sim.noise import NoiseModel
-from lucivant_sim.plan import plan_statevector
-from lucivant_sim.simulator import FUSION_WIDTH, _sample, choose_method, to_circuit
+from lucivant_sim.simulator import FUSION_WIDTH, _sample, _sample_many, _tuned
 
 _ids = itertools.count()
 
@@ -59,6 +60,9 @@ class Request:
     result: Any = None
     error: BaseException | None = None
     event: threading.Event = field(default_factory=threading.Event, repr=False)
+    # the program as read (in a front-end worker when the pool runs, else in ``submit``)
+    _reading: Any = field(default=None, repr=False)
+    _compiled: Compiled | None = field(default=None, repr=False)
 
     def wait(self, timeout: float | None = None) -> Any:
         self.event.wait(timeout)
@@ -74,7 +78,7 @@ class Request:
 @dataclass
 class _Job:
     req: Request
-    circuit: Circuit
+    c: Compiled
     method: str
     footprint: int
     terms: dict[str, float] | None = None
@@ -85,6 +89,8 @@ class _Job:
     stream: torch.cuda.Stream | None = None
     mats: Any = None
     addrs: list[int] | None = None
+    # outputs queued behind the last pass: (event, device tensor), read once the event completes
+    finishing: tuple[torch.cuda.Event, torch.Tensor] | None = None
 
 
 class Server:
@@ -108,27 +114,36 @@ class Server:
         self.streams = [torch.cuda.Stream() for _ in range(max(1, streams))]
         self._rr = 0
         self.inbox: queue.Queue[Request] = queue.Queue()
+        self.reading: list[Request] = []
         self.waiting: list[_Job] = []
         self.resident: list[_Job] = []
         self.completed: list[Request] = []
         self.max_lds = pg.max_qubits(dtype)
+        # fusion choice per width for the passes of wide requests (planned where they are read)
+        self.widths = {n: ((self.width, 0, 0) if fusion_width else _tuned(n)) for n in range(self.max_lds + 1, 34)}
         self._stop = threading.Event()
         self._thread: threading.Thread | None = None
+        # host sampling of wide count requests (a cumulative sum over 2^n probabilities) runs
+        # beside the scheduler, so launches continue meanwhile
+        self._sampler = ThreadPoolExecutor(max_workers=2, thread_name_prefix="lsim-sample")
+        self._lock = threading.Lock()
+        self._sampling: list[Any] = []
         self.ticks = 0
         self.peak_used = 0
 
     # -- submission ---------------------------------------------------------------------
 
     def submit(self, req: Request) -> Request:
-        """Queue a request. Its program is parsed here, in the caller's thread, so the host work
-        of arriving requests overlaps what the scheduler has on the GPU."""
+        """Queue a request. Its program is read (parsed, checked, tabulated or planned) in a
+        front-end worker process when the pool runs, else here in the caller's thread; either way
+        that host work overlaps what the scheduler has on the GPU."""
         if not req.arrival:
             req.arrival = time.perf_counter()
-        try:
-            req.program = to_circuit(req.program)
-        except BaseException as exc:  # noqa: BLE001 - reported to the requester
-            self._finish_error(req, exc)
-            return req
+        args = (req.program, None, req.noise, req.method, self.max_lds, req.observable, self.widths, False)
+        if frontend.active() and isinstance(req.program, str):
+            req._reading = frontend.submit(*args)
+        else:
+            req._compiled = compile_one(*args)
         self.inbox.put(req)
         return req
 
@@ -140,6 +155,9 @@ class Server:
         self._stop.set()
         if self._thread is not None:
             self._thread.join()
+        for f in self._sampling:
+            f.result()
+        self._sampling = []
 
     def _loop(self) -> None:
         while not self._stop.is_set():
@@ -147,42 +165,45 @@ class Server:
                 time.sleep(0.0005)
 
     def run_until_idle(self) -> None:
-        while self.tick() or self.waiting or self.resident or not self.inbox.empty():
+        while self.tick() or self.reading or self.waiting or self.resident or not self.inbox.empty():
             pass
+        for f in self._sampling:
+            f.result()
+        self._sampling = []
 
     # -- scheduling ---------------------------------------------------------------------
 
-    def _prepare(self, req: Request) -> _Job:
-        circuit = to_circuit(req.program)
-        if req.observable is not None:
-            terms = {req.observable: 1.0} if isinstance(req.observable, str) else dict(req.observable)
-            for label in terms:
-                if len(label) != circuit.num_qubits or set(label.upper()) - set("IXYZ"):
-                    raise ValueError(f"{label!r} must be a Pauli string of length {circuit.num_qubits}")
-            if circuit.is_dynamic():
-                raise ValueError("an expectation value needs a circuit without mid-circuit measurement")
-            method = "statevector"
-        else:
-            terms = None
-            method = choose_method(circuit, req.noise, req.method)
-            if method not in ("statevector", "trajectories", "density_matrix"):
-                raise ValueError(f"the server runs statevector, density-matrix and trajectory requests, not {method!r}")
-        size = 16 << circuit.num_qubits
-        if method == "density_matrix":
-            size = 16 << (2 * circuit.num_qubits)
-        footprint = size * (req.shots if method == "trajectories" and circuit.num_qubits > self.max_lds else 1)
-        return _Job(req, circuit, method, footprint, terms)
+    def _prepare(self, req: Request, c: Compiled) -> _Job:
+        if c.method not in ("statevector", "trajectories", "density_matrix"):
+            raise ValueError(f"the server runs statevector, density-matrix and trajectory requests, not {c.method!r}")
+        size = 16 << c.num_qubits
+        if c.method == "density_matrix":
+            size = 16 << (2 * c.num_qubits)
+        footprint = size * (req.shots if c.method == "trajectories" and c.num_qubits > self.max_lds else 1)
+        return _Job(req, c, c.method, footprint, c.terms)
 
     def _admit(self) -> None:
         while True:
             try:
-                req = self.inbox.get_nowait()
+                self.reading.append(self.inbox.get_nowait())
             except queue.Empty:
                 break
+        unread = []
+        for req in self.reading:
+            if req._compiled is None:
+                if not req._reading.ready():
+                    unread.append(req)
+                    continue
+                req._compiled = req._reading.get()
+                req._reading = None
+            c = req._compiled
             try:
-                self.waiting.append(self._prepare(req))
+                if c.error is not None:
+                    raise c.error
+                self.waiting.append(self._prepare(req, c))
             except BaseException as exc:  # noqa: BLE001 - reported to the requester
                 self._finish_error(req, exc)
+        self.reading = unread
         still = []
         for job in self.waiting:
             if self.used + job.footprint <= self.budget or not self.resident:
@@ -202,45 +223,47 @@ class Server:
         small = [
             j
             for j in self.resident
-            if j.state is None and j.method != "density_matrix" and 1 <= j.circuit.num_qubits <= self.max_lds
+            if j.state is None
+            and (j.c.table is not None or (j.method == "trajectories" and 1 <= j.c.num_qubits <= self.max_lds))
         ]
         self._run_small(small)
         wide = [j for j in self.resident if j not in small]
         for job in wide:
-            self._advance_wide(job)
+            if job.finishing is None:
+                self._advance_wide(job)
         for job in wide:
-            if job.passes is not None and job.next_pass >= len(job.passes):
+            if job.finishing is None and job.passes is not None and job.next_pass >= len(job.passes) and job.state is not None:
+                self._queue_outputs(job)
+            if job.finishing is not None and job.finishing[0].query():
                 self._complete_wide(job)
+        self._sampling = [f for f in self._sampling if not f.done()]
         return True
 
     def _run_small(self, jobs: list[_Job]) -> None:
-        by_key: dict[tuple[int, str, int], list[_Job]] = {}
+        by_key: dict[tuple, list[_Job]] = {}
         for j in jobs:
-            key = (j.circuit.num_qubits, j.method if j.terms is None else "expect", len(j.terms or ()))
+            kind = j.method if j.terms is None else "expect"
+            key = (j.c.num_qubits, kind, len(j.terms or ()), j.req.shots, id(j.req.noise) if kind != "expect" else 0)
             by_key.setdefault(key, []).append(j)
-        for (n, kind, nterms), group in by_key.items():
+        for (n, kind, _nterms, shots, _noise), group in by_key.items():
             try:
                 if kind == "expect":
-                    progs = [pg.compile_static(j.circuit) for j in group]
                     masks = [[pauli_masks(lbl) for lbl in j.terms] for j in group]  # type: ignore[union-attr]
-                    out = pg.run_programs(progs, dtype=self.dtype, pauli=masks)
+                    out = pg.run_tables(n, [j.c.table for j in group], [j.c.pool for j in group], dtype=self.dtype, pauli=masks)
                     for idx, j in enumerate(group):
                         total = 0.0
                         for w, v in zip(j.terms.values(), out.expect[idx]):  # type: ignore[union-attr,index]
                             total += w * float(v)
                         self._finish(j, total)
                 elif kind == "statevector":
-                    progs = [pg.compile_static(j.circuit) for j in group]
-                    out = pg.run_programs(progs, dtype=self.dtype, want_probs=True)
-                    for idx, j in enumerate(group):
-                        rng = np.random.default_rng(j.req.seed)
-                        self._finish(j, _sample(j.circuit, out.probs[idx], j.req.shots, j.req.noise, rng))  # type: ignore[index]
+                    out = pg.run_tables(n, [j.c.table for j in group], [j.c.pool for j in group], dtype=self.dtype, want_probs=True)
+                    counts = _sample_many([j.c for j in group], out.probs, shots, group[0].req.noise, [j.req.seed for j in group])
+                    for j, cnt in zip(group, counts):
+                        self._finish(j, cnt)
                 else:
                     from lucivant_sim.batch import run_many
 
-                    res = run_many([j.circuit for j in group], group[0].req.shots, seeds=[j.req.seed for j in group], noise=group[0].req.noise, method="trajectories", dtype=self.dtype) if _same_shots_noise(group) else [
-                        run_many([j.circuit], j.req.shots, seeds=[j.req.seed], noise=j.req.noise, method="trajectories", dtype=self.dtype)[0] for j in group
-                    ]
+                    res = run_many([j.c.circuit for j in group], shots, seeds=[j.req.seed for j in group], noise=group[0].req.noise, method="trajectories", dtype=self.dtype)
                     for j, r in zip(group, res):
                         self._finish(j, r.counts)
             except BaseException as exc:  # noqa: BLE001
@@ -253,7 +276,7 @@ class Server:
             from lucivant_sim.simulator import run
 
             try:
-                res = run(job.circuit, job.req.shots, seed=job.req.seed, noise=job.req.noise, method="density_matrix")
+                res = run(job.c.circuit, job.req.shots, seed=job.req.seed, noise=job.req.noise, method="density_matrix")
                 job.passes, job.next_pass = [], 0
                 self._finish(job, res.counts)
             except BaseException as exc:  # noqa: BLE001
@@ -264,8 +287,9 @@ class Server:
             from lucivant_sim.trajectories import run_trajectories
 
             try:
-                tally = run_trajectories(job.circuit, job.req.noise, job.req.shots, np.random.default_rng(job.req.seed), dtype=self.dtype)
-                c = job.circuit
+                c = job.c.circuit
+                assert c is not None
+                tally = run_trajectories(c, job.req.noise, job.req.shots, np.random.default_rng(job.req.seed), dtype=self.dtype)
                 counts = {c.format_bits(b): k for b, k in sorted(tally.items())} if c.num_clbits else {}
                 job.passes, job.next_pass = [], 0
                 self._finish(job, counts)
@@ -277,8 +301,9 @@ class Server:
             job.stream = self.streams[self._rr % len(self.streams)]
             self._rr += 1
             with torch.cuda.stream(job.stream):
-                job.state = Statevector(job.circuit.num_qubits, dtype=self.dtype)
-                job.passes = plan_statevector(job.circuit.operations, job.circuit.num_qubits, self.width)
+                job.state = Statevector(job.c.num_qubits, dtype=self.dtype)
+                job.passes = job.c.passes
+                assert job.passes is not None
                 job.mats, job.addrs = upload([p.matrix for p in job.passes], self.dtype)
         assert job.passes is not None and job.addrs is not None and job.stream is not None
         lib = nat.load()
@@ -288,46 +313,72 @@ class Server:
         for p, addr in zip(job.passes[job.next_pass : stop], job.addrs[job.next_pass : stop]):
             if p.kind == "scale":
                 z = complex(p.matrix[0])
-                lib.lsim_sv_scale(state, self.dtype, job.circuit.num_qubits, 1, z.real, z.imag, hs)
+                lib.lsim_sv_scale(state, self.dtype, job.c.num_qubits, 1, z.real, z.imag, hs)
                 continue
             fn = lib.lsim_sv_apply_dense if p.kind == "dense" else lib.lsim_sv_apply_diag
-            fn(state, self.dtype, job.circuit.num_qubits, 1, p.k, nat.int_array(p.bits), addr, None, hs)
+            fn(state, self.dtype, job.c.num_qubits, 1, p.k, nat.int_array(p.bits), addr, None, hs)
         job.next_pass = stop
+
+    def _queue_outputs(self, job: _Job) -> None:
+        """Queue a wide request's outputs behind its last pass, on its stream: Pauli
+        expectations, or |psi|^2 for sampling; an event marks when they are ready."""
+        assert job.state is not None and job.stream is not None
+        with torch.cuda.stream(job.stream):
+            sv = job.state
+            if job.terms is not None:
+                out = launch_expectations("lsim_sv_pauli_expect", sv.buffer, sv.dtype, sv.n, sv.n, list(job.terms))
+            else:
+                out = torch.empty(1 << sv.n, dtype=torch.float64, device=sv.buffer.device)
+                nat.call("lsim_sv_probabilities", ptr(sv.buffer), sv.dtype, sv.n, 1, ptr(out), job.stream.cuda_stream)
+            done = torch.cuda.Event()
+            done.record(job.stream)
+        job.finishing = (done, out)
 
     def _complete_wide(self, job: _Job) -> None:
         if job.req.done:
             return
-        assert job.state is not None and job.stream is not None
-        with torch.cuda.stream(job.stream):
-            if job.terms is not None:
-                vals = job.state.expectations(list(job.terms))
-                total = 0.0
-                for w, v in zip(job.terms.values(), vals):
-                    total += w * float(v)
-                self._finish(job, total)
-            else:
-                probs = job.state.probabilities()
-                self._finish(job, _sample(job.circuit, probs, job.req.shots, job.req.noise, np.random.default_rng(job.req.seed)))
+        assert job.finishing is not None
+        _, out = job.finishing
+        values = out.cpu().numpy()  # the event has completed: no wait
+        job.finishing = None
+        if job.terms is not None:
+            total = 0.0
+            for w, v in zip(job.terms.values(), values):
+                total += w * float(v)
+            self._finish(job, total)
+            return
+        # the state's HBM is free for admission now; the host sampling runs beside the scheduler
+        self._release(job)
+        self._sampling.append(self._sampler.submit(self._sample_wide, job, values))
+
+    def _sample_wide(self, job: _Job, probs: np.ndarray) -> None:
+        try:
+            self._finish(job, _sample(job.c, probs, job.req.shots, job.req.noise, np.random.default_rng(job.req.seed)))
+        except BaseException as exc:  # noqa: BLE001
+            self._finish_error(job.req, exc)
 
     # -- bookkeeping ----------------------------------------------------------------------
 
     def _finish(self, job: _Job, result: Any) -> None:
         job.req.result = result
         job.req.done = time.perf_counter()
-        self.completed.append(job.req)
-        job.req.event.set()
         self._release(job)
+        with self._lock:
+            self.completed.append(job.req)
+        job.req.event.set()
 
     def _finish_error(self, req: Request, exc: BaseException) -> None:
         req.error = exc
         req.done = time.perf_counter()
-        self.completed.append(req)
+        with self._lock:
+            self.completed.append(req)
         req.event.set()
 
     def _release(self, job: _Job) -> None:
-        if job in self.resident:
-            self.resident.remove(job)
-            self.used -= job.footprint
+        with self._lock:
+            if job in self.resident:
+                self.resident.remove(job)
+                self.used -= job.footprint
         job.state = None
         job.mats = None
 
@@ -344,10 +395,6 @@ class Server:
             "ticks": self.ticks,
             "peak_hbm_bytes": self.peak_used,
         }
-
-
-def _same_shots_noise(group: list[_Job]) -> bool:
-    return len({j.req.shots for j in group}) == 1 and len({id(j.req.noise) for j in group}) == 1
 
 
 def serve_stream(server: Server, requests: list[Request], arrivals: list[float]) -> dict[str, float]:

@@ -17,7 +17,7 @@ import json
 import os
 import secrets
 import time
-from collections.abc import Mapping
+from collections.abc import Mapping, Sequence
 from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Any, Literal
@@ -237,15 +237,18 @@ def _sample(
     dist = marginal(probs, circuit.num_qubits, measured)
     cumulative = np.cumsum(dist)
     cumulative /= cumulative[-1]
-    outcomes = np.searchsorted(cumulative, rng.random(shots), side="right").astype(np.int64)
-    outcomes = np.minimum(outcomes, len(dist) - 1)
+    draws = rng.random(shots)
     wide = circuit.num_clbits > 62
     flips = noise is not None and any((e := noise.readout_error(q)) is not None and (e.p01 or e.p10) for q, _ in pairs)
     if not wide and not flips and len(dist) <= 1 << 16:
-        # no readout draws: the bit pattern is a fixed function of the outcome, tabulated once
+        # no readout draws: the bit pattern is a fixed function of the outcome, tabulated once,
+        # and only the tally matters, so the draws are searched in sorted order (much faster)
+        outcomes = np.minimum(np.searchsorted(cumulative, np.sort(draws), side="right"), len(dist) - 1)
         bits = _outcome_table(tuple(pairs), tuple(measured), len(dist))[outcomes]
         values, counts = np.unique(bits, return_counts=True)
         return dict(zip(circuit.format_many(values), counts.tolist()))
+    outcomes = np.searchsorted(cumulative, draws, side="right").astype(np.int64)
+    outcomes = np.minimum(outcomes, len(dist) - 1)
     bits = np.zeros(shots, dtype=object if wide else np.int64)
     for q, c in pairs:
         value = (outcomes >> position[q]) & 1
@@ -261,6 +264,37 @@ def _sample(
     if wide:
         return {circuit.format_bits(int(v)): int(n) for v, n in zip(values, counts)}
     return dict(zip(circuit.format_many(values), counts.tolist()))
+
+
+def _sample_many(circuits: Sequence[Any], probs: NDArray[np.float64], shots: int, noise: NoiseModel | None, seeds: Sequence[int]) -> list[dict[str, int]]:
+    """``[_sample(c, probs[j], shots, noise, default_rng(seeds[j])) ...]`` for circuits of one
+    width, with the array work done once for the batch where that gives the same numbers: the
+    cumulative sums of all rows in one call (each row summed in order, as for one vector), and
+    counts tallied by ``bincount`` instead of ``unique`` (the same sorted outcomes). Circuits whose
+    clbits depend on readout draws or whose measured set isn't every qubit take ``_sample``."""
+    out: list[dict[str, int] | None] = [None] * len(circuits)
+    fast: list[tuple[int, tuple]] = []
+    for j, c in enumerate(circuits):
+        pairs = tuple(c.final_measurements())
+        flips = noise is not None and any((e := noise.readout_error(q)) is not None and (e.p01 or e.p10) for q, _ in pairs)
+        if shots and pairs and not flips and 0 < c.num_clbits <= 16 and len({q for q, _ in pairs}) == c.num_qubits:
+            fast.append((j, pairs))
+        else:
+            out[j] = _sample(c, probs[j], shots, noise, np.random.default_rng(seeds[j]))
+    if fast:
+        size = probs.shape[1]
+        cumulative = np.cumsum(probs[[j for j, _ in fast]], axis=1)
+        cumulative /= cumulative[:, -1:]
+        everything = tuple(range(size.bit_length() - 1))
+        for row, (j, pairs) in enumerate(fast):
+            c = circuits[j]
+            u = np.sort(np.random.default_rng(seeds[j]).random(shots))
+            outcomes = np.minimum(np.searchsorted(cumulative[row], u, side="right"), size - 1)
+            bits = _outcome_table(pairs, everything, size)[outcomes]
+            tally = np.bincount(bits, minlength=1 << c.num_clbits)
+            values = np.flatnonzero(tally)
+            out[j] = dict(zip(c.format_many(values), tally[values].tolist()))
+    return out  # type: ignore[return-value]
 
 
 _OUTCOME_TABLES: dict[tuple, np.ndarray] = {}

@@ -234,3 +234,31 @@ def test_dense_tiled_placements_equal_the_oracle_and_the_direct_kernel(k: int, d
             lib.lsim_set_dense_tiling(old)
         assert np.abs(tiled - want).max() < TOL[dtype] * 4, targets
         assert np.abs(tiled - direct).max() < TOL[dtype] * 4, targets
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+def test_dense_k5_operator_splits_agree(dtype: int) -> None:
+    """Every split of a 5-qubit operator's reads between the scalar cache and LDS gives the same
+    state bit for bit, equal to the oracle, with and without a selector; other values are refused."""
+    n = 11
+    lib = nat.load()
+    rng = np.random.default_rng(77)
+    placements = [(10, 9, 8, 7, 6), (4, 3, 2, 1, 0), (0, 10, 5, 2, 7)] + [tuple(int(x) for x in rng.choice(n, size=5, replace=False)) for _ in range(3)]
+    old = lib.lsim_set_dense_k5_split(-1)
+    try:
+        for i, targets in enumerate(placements):
+            psi = states(n, 2, 500 + i)
+            u = unitary(5, 90 + i)
+            want = orc.apply_sv(psi[1], u, targets, n)
+            outs = []
+            for split in (0, 8, 12, 16, 20, 24, 32):
+                assert lib.lsim_set_dense_k5_split(split) == 0
+                assert lib.lsim_set_dense_k5_split(-1) == split
+                outs.append(apply_dense(psi, n, np.stack([u, u]), targets, dtype, [-1, 1]))
+            for o in outs:
+                assert np.array_equal(o, outs[0]), targets
+                assert np.array_equal(o[0], psi[0].astype(NPDT[dtype])), targets
+                assert np.abs(o[1] - want).max() < TOL[dtype] * 4, targets
+        assert lib.lsim_set_dense_k5_split(7) == nat.EINVAL
+    finally:
+        lib.lsim_set_dense_k5_split(old)
