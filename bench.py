@@ -311,7 +311,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=0, help="fusion width (0: run the autotuner and use its choice)")
     ap.add_argument("--tiling", type=int, choices=(0, 1), default=None, help="force the LDS-tiled dense kernel off/on")
-    ap.add_argument("--k5-split", type=int, default=None, help="rows of 5-qubit operators read through the scalar cache")
+    ap.add_argument("--mfma", type=int, default=None, help="mask of operator widths on the matrix cores (bit k: k qubits)")
+    ap.add_argument("--streaming", type=int, choices=(0, 1), default=None, help="non-temporal accesses for high placements off/on")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and trajectory legs")
     ap.add_argument(
@@ -349,8 +350,10 @@ def main() -> int:
     tuned = None
     if args.tiling is not None:
         nat.load().lsim_set_dense_tiling(args.tiling)
-    if args.k5_split is not None:
-        nat.call("lsim_set_dense_k5_split", args.k5_split)
+    if args.mfma is not None:
+        nat.load().lsim_set_dense_mfma(args.mfma)
+    if args.streaming is not None:
+        nat.load().lsim_set_dense_streaming(args.streaming)
     if args.width:
         width = args.width
     elif args.profile:
@@ -361,7 +364,8 @@ def main() -> int:
         tuned = autotune.tune(circuit, store=False)
         width = tuned["width"]
         nat.load().lsim_set_dense_tiling(tuned["tiled"])
-        nat.call("lsim_set_dense_k5_split", tuned["k5_split"])
+        nat.load().lsim_set_dense_mfma(tuned["mfma"])
+        nat.load().lsim_set_dense_streaming(tuned["streaming"])
     gates = sum(1 for op in circuit.operations if isinstance(op, ls.circuit.Gate))
     low_bits = tuned["low_bits"] if tuned else 0
     low_width = tuned["low_width"] if tuned else 0
@@ -453,7 +457,8 @@ def main() -> int:
             "gates": gates,
             "fusion_width": width,
             "low_qubit_width": low_width or None,
-            "k5_scalar_rows": int(nat.load().lsim_set_dense_k5_split(-1)),
+            "matrix_core_widths": [k for k in (3, 4, 5) if (nat.load().lsim_set_dense_mfma(-1) >> k) & 1],
+            "streaming": bool(nat.load().lsim_set_dense_streaming(-1)),
             "passes": len(passes),
             "dense_passes": dense,
             "bytes_per_pass": BYTES_PER_PASS,

@@ -66,12 +66,17 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
  * Returns the previous setting. No reference counterpart (a tuning switch). */
 int lsim_set_dense_tiling(int enabled);
 
-/* Select how the dense kernel reads a 5-qubit operator: its first `scalar_rows` rows through the
- * scalar cache, the others from LDS (0, 8, 12, 16, 20, 24 or 32; the default is 0, all from
- * LDS). Returns LSIM_OK, or LSIM_EINVAL for another value; a negative argument changes nothing
- * and returns the current setting. A tuning choice only: every split gives the same results.
- * No reference counterpart (engines.py:38-119 has one NumPy path); the autotuner sets it. */
-int lsim_set_dense_k5_split(int scalar_rows);
+/* Select which complex128 operator widths run on the FP64 matrix cores (v_mfma_f64_16x16x4f64,
+ * 16 groups per wavefront step) instead of the vector kernel: bit k of `widths` for k = 3, 4, 5
+ * (states of at least k + 4 bits; other bits are ignored). The default is 1 << 5. Returns the
+ * previous mask; a negative argument only reads it. Results agree with the vector kernel to
+ * rounding. No reference counterpart (a tuning switch the autotuner sets). */
+int lsim_set_dense_mfma(int widths);
+
+/* Streaming (non-temporal) loads and stores in the dense kernels for placements whose targets are
+ * all at bit 3 or above: 1 on (the default), 0 off; returns the previous setting, a negative
+ * argument only reads it. Results are identical either way. No reference counterpart. */
+int lsim_set_dense_streaming(int enabled);
 
 /* psi <- diag(d) psi on k <= 5 target bits: the diagonal case of _apply_slices
  * (engines.py:91-104; rz, s, t, cz, cp). `diags`: M diagonals of 2^k entries; `sel` as above. */

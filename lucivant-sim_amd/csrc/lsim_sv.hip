@@ -14,20 +14,37 @@
 
 namespace lsim {
 
-// rows of a 5-qubit operator the dense kernel reads through the scalar cache (the rest from LDS)
-static int g_k5_scalar_rows = 0;
+// bit k set: complex128 k-qubit operators (k = 3..5) run on the matrix cores (k_dense_mfma)
+static int g_mfma_widths = 1 << 5;
+// streaming (non-temporal) loads and stores for placements with every target at bit 3 or above
+static int g_dense_streaming = 1;
 
-// SR: for K = 5, how many leading rows of U are read through the scalar cache; the other rows
-// are staged in LDS (see below). Ignored for K < 5 (every row through the scalar cache).
-template <int K, typename R, int SR = 1 << K>
+// loads and stores with the streaming (non-temporal) hint, or plain ones
+template <bool NT, typename T>
+__device__ __forceinline__ T ld(const T* p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_(T* p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
+// NT: streaming (non-temporal) loads and stores, chosen when every target is at bit 3 or above:
+// each load instruction then covers whole 128-byte lines, and keeping them in cache gains
+// nothing (lower targets rely on the cache for the partner loads of the same line).
+template <int K, typename R, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_dense(typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t groups,
             const typename Cx<R>::V* __restrict__ mats, const int* __restrict__ sel,
             uint32_t batch_base, Targets t) {
   using V = typename Cx<R>::V;
   constexpr int D = 1 << K;
-  // rows of a 5-qubit operator read from LDS
-  constexpr int LR = K == 5 ? D - SR : 0;
   const uint32_t b = batch_base + blockIdx.y;
   const V* __restrict__ U = mats;
   if (sel != nullptr) {
@@ -35,38 +52,116 @@ __global__ void __launch_bounds__(kBlock)
     if (m < 0) return;
     U += (size_t)m * (D * D);
   }
-  // A 5-qubit operator is 16 KiB, as large as the scalar cache. Rows SR.. are staged once per
-  // workgroup in LDS and read as wavefront-uniform broadcasts; rows ..SR-1 stream through the
-  // scalar cache. The split divides the operator reads between the two paths: an LDS broadcast
-  // of 16 B still returns 1 KiB per wavefront, so all rows from LDS make LDS the bottleneck,
-  // and all rows through the scalar cache thrash it.
-  __shared__ V ulds[LR > 0 ? LR * D : 1];
-  if constexpr (LR > 0) {
-    for (int e = threadIdx.x; e < LR * D; e += kBlock) ulds[e] = U[SR * D + e];
+  // A 5-qubit operator is 16 KiB, as large as the scalar cache: it is staged once per workgroup
+  // in LDS and read as wavefront-uniform broadcasts. (complex128 5-qubit operators normally run
+  // on the matrix cores instead, see k_dense_mfma.)
+  __shared__ V ulds[K == 5 ? D * D : 1];
+  if constexpr (K == 5) {
+    for (int e = threadIdx.x; e < D * D; e += kBlock) ulds[e] = U[e];
     __syncthreads();
   }
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   if (g >= groups) return;
-  V* __restrict__ st = states + ((size_t)b << nbits);
+  R* __restrict__ st = reinterpret_cast<R*>(states + ((size_t)b << nbits));
   const uint32_t base = spread(g, t.sorted, K);
   V in[D];
 #pragma unroll
-  for (int c = 0; c < D; ++c) in[c] = st[base + t.offs[c]];
+  for (int c = 0; c < D; ++c) {
+    const R* p = st + 2 * (size_t)(base + t.offs[c]);
+    in[c].x = ld<NT>(p);
+    in[c].y = ld<NT>(p + 1);
+  }
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     R re = 0, im = 0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const V u = (LR > 0 && r >= SR) ? ulds[(r - SR) * D + c] : U[r * D + c];
+      const V u = K == 5 ? ulds[r * D + c] : U[r * D + c];
       re = fma(u.x, in[c].x, re);
       re = fma(-u.y, in[c].y, re);
       im = fma(u.x, in[c].y, im);
       im = fma(u.y, in[c].x, im);
     }
-    V o;
-    o.x = re;
-    o.y = im;
-    st[base + t.offs[r]] = o;
+    R* p = st + 2 * (size_t)(base + t.offs[r]);
+    st_<NT>(p, re);
+    st_<NT>(p + 1, im);
+  }
+}
+
+// K-qubit operator (K = 3..5) on the FP64 matrix cores. The complex 2^K x 2^K operator U acts as
+// the real 2^(K+1) x 2^(K+1) M = [[Re U, -Im U], [Im U, Re U]] on x = [Re psi_g; Im psi_g], and
+// one wavefront applies it to 16 groups at a time: Y = M X with X, Y of 2^(K+1) x 16, i.e.
+// 2^(K+1)/16 row tiles x 2^(K+1)/4 k-steps of v_mfma_f64_16x16x4f64. The lane layout of that
+// instruction on gfx950 (A: lane l holds A[l%16][l/16]; B: B[l/16][l%16]; C/D: register v of
+// tile tt holds D[16 tt + 4v + l/16][l%16], checked with a one-instruction probe) puts, on lane
+// (i = l%16, k = l/16):
+//   - operator fragment (tile tt, step kk): M[16 tt + i][4 kk + k], loaded once per wavefront and
+//     kept in registers, so no operand is broadcast from LDS or the scalar cache;
+//   - input steps j and j + 2^(K-2): Re and Im of amplitude 4 j + k of group i (one 16-byte load);
+//   - output register R = 4 tt + v holds row 4R + k: Re of amplitude 4R + k for R < 2^(K-2), and
+//     its Im in register R + 2^(K-2).
+// For one amplitude index the 16 lanes of a row read 16 consecutive groups. The flop count equals
+// the vector kernel's (4 real products per complex one); what changes is operand delivery, which
+// limits the vector kernel at K = 5 (an LDS broadcast per four FMAs). NT: streaming (non-temporal)
+// loads and stores, chosen when every target is at bit 3 or above, so that each load instruction
+// covers whole 128-byte lines and nothing is gained by keeping them in cache.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <int K, bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_dense_mfma(double2* __restrict__ states, uint32_t nbits, uint32_t batches,
+                 const double2* __restrict__ mats, const int* __restrict__ sel,
+                 uint32_t batch_base, Targets t) {
+  constexpr int D = 1 << K;            // complex dimension
+  constexpr int TILES = 2 * D / 16;    // row tiles of M
+  constexpr int STEPS = 2 * D / 4;     // k-steps
+  constexpr int LOADS = D / 4;         // amplitudes per lane per 16 groups
+  const uint32_t b = batch_base + blockIdx.y;
+  const double2* __restrict__ U = mats;
+  if (sel != nullptr) {
+    const int m = sel[b];
+    if (m < 0) return;
+    U += (size_t)m * (D * D);
+  }
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t i = lane & 15u, k = lane >> 4;
+  double a[TILES][STEPS];
+#pragma unroll
+  for (int tt = 0; tt < TILES; ++tt)
+#pragma unroll
+    for (int kk = 0; kk < STEPS; ++kk) {
+      const uint32_t row = 16u * tt + i, col = 4u * kk + k;
+      const double2 u = U[(row % D) * D + (col % D)];
+      a[tt][kk] = ((row < (uint32_t)D) == (col < (uint32_t)D)) ? u.x : (row < (uint32_t)D ? -u.y : u.y);
+    }
+  // amplitude offsets split into a wavefront-uniform part and this lane's part (index bit
+  // (K-1-j) of the matrix index selects target listed[j]; the lane's k is bits 1..0 of both its
+  // input and its output amplitude indices)
+  const uint32_t in_lane = (((k >> 1) & 1u) << t.listed[K - 2]) | ((k & 1u) << t.listed[K - 1]);
+  double* __restrict__ st = reinterpret_cast<double*>(states + ((size_t)b << nbits));
+  const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint32_t nwaves = gridDim.x * (kBlock / 64);
+  for (uint32_t batch = wave; batch < batches; batch += nwaves) {
+    const uint32_t base = spread(batch * 16u + i, t.sorted, K);
+    f64x2 x[LOADS];
+#pragma unroll
+    for (int j = 0; j < LOADS; ++j)
+      x[j] = ld<NT>(reinterpret_cast<const f64x2*>(st + 2 * (size_t)(base + t.offs[4 * j] + in_lane)));
+    f64x4 acc[TILES];
+#pragma unroll
+    for (int tt = 0; tt < TILES; ++tt) {
+      acc[tt] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < LOADS; ++j) acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tt][j], x[j].x, acc[tt], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < LOADS; ++j) acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tt][j + LOADS], x[j].y, acc[tt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < LOADS; ++r) {
+      const f64x2 o = {acc[r / 4][r % 4], acc[(r + LOADS) / 4][(r + LOADS) % 4]};
+      st_<NT>(reinterpret_cast<f64x2*>(st + 2 * (size_t)(base + t.offs[4 * r] + in_lane)), o);
+    }
   }
 }
 
@@ -281,6 +376,33 @@ static int launch_dense_tiled(void* states, int nbits, long long batch, int k, c
   return check_launch("tiled dense operator");
 }
 
+static int launch_dense_mfma(void* states, int nbits, long long batch, int k, const Targets& t,
+                             const void* mats, const int* sel, hipStream_t s) {
+  const uint32_t batches = 1u << (nbits - k - 4);  // 16 groups of 2^k amplitudes per wavefront step
+  // enough wavefronts to fill 256 CUs several times over, then grid-stride
+  const uint32_t waves = batches < 8192u ? batches : 8192u;
+  const uint32_t gx = (waves + 3) / 4;
+  // streaming accesses when every target is at bit 3 or above (whole lines per instruction)
+  const bool nt = g_dense_streaming && t.sorted[0] >= 3;
+  for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
+    const dim3 grid(gx, gy), block(kBlock);
+    double2* st = (double2*)states;
+    const double2* m = (const double2*)mats;
+    const uint32_t bb = (uint32_t)b0;
+    switch (k * 2 + (nt ? 1 : 0)) {
+      case 6: k_dense_mfma<3, false><<<grid, block, 0, s>>>(st, nbits, batches, m, sel, bb, t); break;
+      case 7: k_dense_mfma<3, true><<<grid, block, 0, s>>>(st, nbits, batches, m, sel, bb, t); break;
+      case 8: k_dense_mfma<4, false><<<grid, block, 0, s>>>(st, nbits, batches, m, sel, bb, t); break;
+      case 9: k_dense_mfma<4, true><<<grid, block, 0, s>>>(st, nbits, batches, m, sel, bb, t); break;
+      case 10: k_dense_mfma<5, false><<<grid, block, 0, s>>>(st, nbits, batches, m, sel, bb, t); break;
+      case 11: k_dense_mfma<5, true><<<grid, block, 0, s>>>(st, nbits, batches, m, sel, bb, t); break;
+      default: return fail(LSIM_EINVAL, "the matrix-core dense operator needs k between 3 and 5");
+    }
+  }
+  return check_launch("dense operator (matrix cores)");
+}
+
 template <typename R>
 static int launch_dense(void* states, int nbits, long long batch, int k, const Targets& t,
                         const void* mats, const int* sel, hipStream_t s) {
@@ -292,23 +414,22 @@ static int launch_dense(void* states, int nbits, long long batch, int k, const T
     dim3 grid(gx, gy), block(kBlock);
     V* st = (V*)states;
     const V* m = (const V*)mats;
+    const bool nt = g_dense_streaming && t.sorted[0] >= 3;
+#define LSIM_DENSE(KK)                                                                     \
+  do {                                                                                     \
+    if (nt)                                                                                \
+      k_dense<KK, R, true><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t);  \
+    else                                                                                   \
+      k_dense<KK, R, false><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); \
+  } while (0)
     switch (k) {
-      case 1: k_dense<1, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-      case 2: k_dense<2, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-      case 3: k_dense<3, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-      case 4: k_dense<4, R><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-      case 5:
-        switch (g_k5_scalar_rows) {
-          case 0: k_dense<5, R, 0><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-          case 8: k_dense<5, R, 8><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-          case 12: k_dense<5, R, 12><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-          case 16: k_dense<5, R, 16><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-          case 20: k_dense<5, R, 20><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-          case 24: k_dense<5, R, 24><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-          default: k_dense<5, R, 32><<<grid, block, 0, s>>>(st, nbits, groups, m, sel, (uint32_t)b0, t); break;
-        }
-        break;
+      case 1: LSIM_DENSE(1); break;
+      case 2: LSIM_DENSE(2); break;
+      case 3: LSIM_DENSE(3); break;
+      case 4: LSIM_DENSE(4); break;
+      case 5: LSIM_DENSE(5); break;
     }
+#undef LSIM_DENSE
   }
   return check_launch("dense operator");
 }
@@ -340,14 +461,19 @@ int lsim_set_dense_tiling(int enabled) {
   return old;
 }
 
-int lsim_set_dense_k5_split(int scalar_rows) {
-  if (scalar_rows < 0) return g_k5_scalar_rows;
-  LSIM_REQUIRE(scalar_rows == 0 || scalar_rows == 8 || scalar_rows == 12 || scalar_rows == 16 ||
-                   scalar_rows == 20 || scalar_rows == 24 || scalar_rows == 32,
-               "the 5-qubit split must be 0, 8, 12, 16, 20, 24 or 32 rows, not %d", scalar_rows);
-  g_k5_scalar_rows = scalar_rows;
-  return LSIM_OK;
+int lsim_set_dense_mfma(int widths) {
+  const int old = g_mfma_widths;
+  if (widths >= 0) g_mfma_widths = widths & ((1 << 3) | (1 << 4) | (1 << 5));
+  return old;
 }
+
+int lsim_set_dense_streaming(int enabled) {
+  const int old = g_dense_streaming;
+  if (enabled >= 0) g_dense_streaming = enabled != 0;
+  return old;
+}
+
+
 
 int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,
                       void* stream) {
@@ -376,8 +502,11 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
       return launch_dense_tiled<double>(states, nbits, batch, k, tl, mats, sel, as_stream(stream));
     return launch_dense_tiled<float>(states, nbits, batch, k, tl, mats, sel, as_stream(stream));
   }
-  if (dtype == LSIM_C128)
+  if (dtype == LSIM_C128) {
+    if (k >= 3 && ((g_mfma_widths >> k) & 1) && nbits >= k + 4)
+      return launch_dense_mfma(states, nbits, batch, k, t, mats, sel, as_stream(stream));
     return launch_dense<double>(states, nbits, batch, k, t, mats, sel, as_stream(stream));
+  }
   return launch_dense<float>(states, nbits, batch, k, t, mats, sel, as_stream(stream));
 }
 

@@ -38,7 +38,8 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "lsim_sv_set_basis": ([_c_void, _c_int, _c_int, _c_ll, _c_ll, _c_void], _c_int),
     "lsim_sv_apply_dense": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_void, _c_void, _c_void], _c_int),
     "lsim_set_dense_tiling": ([_c_int], _c_int),
-    "lsim_set_dense_k5_split": ([_c_int], _c_int),
+    "lsim_set_dense_mfma": ([_c_int], _c_int),
+    "lsim_set_dense_streaming": ([_c_int], _c_int),
     "lsim_sv_apply_diag": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_void, _c_void, _c_void], _c_int),
     "lsim_sv_scale": ([_c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),
     "lsim_sv_axpy": ([_c_void, _c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),

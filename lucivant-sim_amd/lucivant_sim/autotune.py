@@ -1,16 +1,17 @@
 """Autotuning of fusion width and kernel variant, measured on the device, cached on disk.
 
-Three choices set the cost of a statevector simulation on the MI355X:
+Four choices set the cost of a statevector simulation on the MI355X:
 
 - the fusion width (qubits per dense pass): wider passes mean fewer HBM sweeps but more FP64
   work per amplitude (8 * 2^k flop per 32 B; k = 5 sits at the FP64 ridge of ~9.8 flop/B);
 - the dense-kernel variant for placements with low target bits: the LDS-tiled kernel or the
   direct one;
-- for 5-qubit passes, how the operator's reads are split between the scalar cache and LDS.
+- which operator widths (3, 4, 5 qubits) run on the FP64 matrix cores, and whether the dense
+  kernels stream (non-temporal accesses) when every target is at bit 3 or above.
 
 :func:`tune` plans a circuit at each candidate width, runs every candidate (with the tiled kernel
 on and off) on a scratch state of the same size, and times it with HIP events, then times the
-winner's 5-qubit split candidates; the winner is stored
+winner's plan for each matrix-core width mask with streaming on and off; the winner is stored
 per (gfx architecture, qubit count) in ``~/.cache/lucivant_sim/autotune.json`` (or
 ``$LSIM_AUTOTUNE``) and applied by :func:`apply`, which ``simulator`` calls on first use. The LDS
 program path has its own knob, the fusion width of compiled programs (0 = one op per gate), tuned
@@ -68,8 +69,8 @@ def _time_passes(sv: Statevector, passes: list, reps: int) -> float:
 #: (width, low_width) candidates: blocks touching qubits 0..2 may be held narrower (0: no cap)
 CANDIDATES = ((3, 0), (4, 0), (5, 0), (4, 3), (5, 3), (5, 4))
 LOW_BITS = 3
-#: rows of a 5-qubit operator read through the scalar cache (the rest from LDS)
-K5_SPLITS = (0, 8, 16, 24, 32)
+#: matrix-core width masks (bit k: k-qubit operators on the matrix cores)
+MFMA_MASKS = (0, 1 << 5, (1 << 5) | (1 << 4), (1 << 5) | (1 << 4) | (1 << 3))
 
 
 def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES, reps: int = 2, store: bool = True) -> dict[str, Any]:
@@ -90,29 +91,35 @@ def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES,
     best = min(timings, key=timings.__getitem__)
     w_s, lw_s, t_s = best.split("_")
     width, low_width, tiled = int(w_s[1:]), int(lw_s[3:]), int(t_s[5:])
-    # the winner's 5-qubit passes: how their operator reads are split between scalar cache and LDS
-    split = int(lib.lsim_set_dense_k5_split(-1))
-    split_timings: dict[str, float] = {}
+    # the winner's plan: which operator widths run on the matrix cores, streaming accesses on/off
+    mfma = int(lib.lsim_set_dense_mfma(-1))
+    streaming = int(lib.lsim_set_dense_streaming(-1))
+    variant_timings: dict[str, float] = {}
     winner = plans[(width, low_width)]
-    if any(p.k == 5 for p in winner):
-        old_split = split
-        old_tiling = lib.lsim_set_dense_tiling(tiled)
-        try:
-            for rows in K5_SPLITS:
-                lib.lsim_set_dense_k5_split(rows)
-                split_timings[f"k5split{rows}"] = _time_passes(sv, winner, reps)
-        finally:
-            lib.lsim_set_dense_k5_split(old_split)
-            lib.lsim_set_dense_tiling(old_tiling)
-        split = int(min(split_timings, key=split_timings.__getitem__)[len("k5split") :])
-        timings.update(split_timings)
+    old_mfma, old_streaming = mfma, streaming
+    old_tiling = lib.lsim_set_dense_tiling(tiled)
+    try:
+        for mask in MFMA_MASKS:
+            for stream_on in (1, 0):
+                lib.lsim_set_dense_mfma(mask)
+                lib.lsim_set_dense_streaming(stream_on)
+                variant_timings[f"mfma{mask}_nt{stream_on}"] = _time_passes(sv, winner, reps)
+    finally:
+        lib.lsim_set_dense_mfma(old_mfma)
+        lib.lsim_set_dense_streaming(old_streaming)
+        lib.lsim_set_dense_tiling(old_tiling)
+    choice = min(variant_timings, key=variant_timings.__getitem__)
+    m_s, nt_s = choice.split("_")
+    mfma, streaming = int(m_s[4:]), int(nt_s[2:])
+    timings.update(variant_timings)
     rec = {
         "width": width,
         "low_width": low_width,
         "low_bits": LOW_BITS,
         "tiled": tiled,
-        "k5_split": split,
-        "seconds": min([timings[best], *split_timings.values()]),
+        "mfma": mfma,
+        "streaming": streaming,
+        "seconds": min(timings[best], variant_timings[choice]),
         "timings": timings,
         "passes": {f"w{w}_low{lw}": len(p) for (w, lw), p in plans.items()},
     }
@@ -144,7 +151,8 @@ def apply(num_qubits: int) -> tuple[int, int, int]:
         return simulator.FUSION_WIDTH, 0, 0
     lib = nat.load()
     lib.lsim_set_dense_tiling(int(rec.get("tiled", 1)))
-    lib.lsim_set_dense_k5_split(int(rec.get("k5_split", 0)))
+    lib.lsim_set_dense_mfma(int(rec.get("mfma", 1 << 5)))
+    lib.lsim_set_dense_streaming(int(rec.get("streaming", 1)))
     return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0))
 
 

@@ -146,9 +146,28 @@ def _run_many_frontend(programs, shots, seeds, noise, method, params, dtype, kee
     return [r for r in results if r is not None]
 
 
-def _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0) -> None:  # type: ignore[no-untyped-def]
-    from lucivant_sim.trajectories import run_trajectories
+def trajectory_counts(circuits: Sequence[Any], programs: Sequence[pg.Program], shots: int, seeds: Sequence[int], dtype: int = nat.C128) -> list[dict[str, int]]:
+    """Counts of every circuit's ``shots`` trajectories from one LDS-program launch, one
+    workgroup per shot. ``programs`` come from ``program.compile_trajectory``; each circuit's
+    draws are its own seeded stream in the reference's order (``rng.random(shots * D)``)."""
+    if not programs or shots == 0:
+        return [{} for _ in programs]
+    width = max(p.draws for p in programs)
+    draws = np.zeros((len(programs) * shots, max(width, 1)))
+    for j, (p, seed) in enumerate(zip(programs, seeds)):
+        if p.draws:
+            draws[j * shots : (j + 1) * shots, : p.draws] = np.random.default_rng(seed).random(shots * p.draws).reshape(shots, p.draws)
+    out = pg.run_programs(list(programs), repeat=shots, dtype=dtype, draws=draws, want_clbits=True)
+    assert out.clbits is not None
+    counts = []
+    for j, c in enumerate(circuits):
+        bits = out.clbits[j * shots : (j + 1) * shots]
+        uniq, cnt = np.unique(bits, return_counts=True)
+        counts.append(dict(zip(c.format_many(uniq), cnt.tolist())) if c.num_clbits else {})
+    return counts
 
+
+def _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0) -> None:  # type: ignore[no-untyped-def]
     progs, ok = [], []
     for i in idx:
         try:
@@ -156,24 +175,9 @@ def _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0) ->
             ok.append(i)
         except pg.Unsupported:
             results[i] = run(circuits[i], shots, seed=seeds[i], noise=noise, method="trajectories")
-    if not ok or shots == 0:
-        for i in ok:
-            results[i] = _result(circuits[i], {}, shots, seeds[i], "trajectories", noise, t0)
-        return
-    width = max(p.draws for p in progs)
-    draws = np.zeros((len(ok) * shots, max(width, 1)))
-    for j, (i, p) in enumerate(zip(ok, progs)):
-        if p.draws:
-            draws[j * shots : (j + 1) * shots, : p.draws] = np.random.default_rng(seeds[i]).random(shots * p.draws).reshape(shots, p.draws)
-    out = pg.run_programs(progs, repeat=shots, dtype=dtype, draws=draws, want_clbits=True)
-    assert out.clbits is not None
-    for j, i in enumerate(ok):
-        c = circuits[i]
-        bits = out.clbits[j * shots : (j + 1) * shots]
-        uniq, cnt = np.unique(bits, return_counts=True)
-        counts = dict(zip(c.format_many(uniq), cnt.tolist())) if c.num_clbits else {}
-        results[i] = _result(c, counts, shots, seeds[i], "trajectories", noise, t0)
-    del run_trajectories
+    counts = trajectory_counts([circuits[i] for i in ok], progs, shots, [seeds[i] for i in ok], dtype)
+    for i, cnt in zip(ok, counts):
+        results[i] = _result(circuits[i], cnt, shots, seeds[i], "trajectories", noise, t0)
 
 
 def expectation_many(

@@ -32,7 +32,8 @@ _WORKERS = 0
 @dataclass
 class Compiled:
     """One program as read by a worker. ``table``/``pool`` are set for ideal static circuits the
-    LDS program kernel can run; ``passes`` for wider statevectors when fusion widths were given;
+    LDS program kernel can run; ``passes`` for wider statevectors and ``program`` (an LDS
+    trajectory program) for small noisy or dynamic circuits, when fusion widths were given;
     ``circuit`` when the launching process needs the whole circuit (dynamic circuits, noisy runs,
     wide states otherwise); ``terms`` are an expectation request's checked Pauli terms; ``error``
     holds the exception reading the program raised, re-raised by the caller."""
@@ -48,6 +49,7 @@ class Compiled:
     circuit: Circuit | None = None
     terms: dict[str, float] | None = None
     passes: list | None = None
+    program: Any = None
     error: BaseException | None = None
     _shell: Circuit | None = field(default=None, repr=False)
 
@@ -156,6 +158,14 @@ def compile_one(
         )
         if chosen != "statevector":
             out.circuit = c
+            if chosen == "trajectories" and widths is not None and 1 <= c.num_qubits <= lds_qubits:
+                # the server runs small trajectory requests as LDS programs
+                from lucivant_sim import program as pg
+
+                try:
+                    out.program = pg.compile_trajectory(c, noise)
+                except pg.Unsupported:
+                    pass
             return out
         out.pairs = tuple(c.final_measurements())
         if 1 <= c.num_qubits <= lds_qubits:

@@ -130,6 +130,8 @@ class Server:
         self._sampling: list[Any] = []
         self.ticks = 0
         self.peak_used = 0
+        #: scheduler-thread time per phase (admission, small batches, wide launches, completion)
+        self.host_seconds = {"admit": 0.0, "small": 0.0, "wide_launch": 0.0, "wide_finish": 0.0}
 
     # -- submission ---------------------------------------------------------------------
 
@@ -216,27 +218,30 @@ class Server:
 
     def tick(self) -> bool:
         """One scheduling round; True if it did any work."""
+        t0 = time.perf_counter()
         self._admit()
+        t1 = time.perf_counter()
+        self.host_seconds["admit"] += t1 - t0
         if not self.resident:
             return False
         self.ticks += 1
-        small = [
-            j
-            for j in self.resident
-            if j.state is None
-            and (j.c.table is not None or (j.method == "trajectories" and 1 <= j.c.num_qubits <= self.max_lds))
-        ]
+        small = [j for j in self.resident if j.state is None and (j.c.table is not None or j.c.program is not None)]
         self._run_small(small)
+        t2 = time.perf_counter()
+        self.host_seconds["small"] += t2 - t1
         wide = [j for j in self.resident if j not in small]
         for job in wide:
             if job.finishing is None:
                 self._advance_wide(job)
+        t3 = time.perf_counter()
+        self.host_seconds["wide_launch"] += t3 - t2
         for job in wide:
             if job.finishing is None and job.passes is not None and job.next_pass >= len(job.passes) and job.state is not None:
                 self._queue_outputs(job)
             if job.finishing is not None and job.finishing[0].query():
                 self._complete_wide(job)
         self._sampling = [f for f in self._sampling if not f.done()]
+        self.host_seconds["wide_finish"] += time.perf_counter() - t3
         return True
 
     def _run_small(self, jobs: list[_Job]) -> None:
@@ -261,11 +266,11 @@ class Server:
                     for j, cnt in zip(group, counts):
                         self._finish(j, cnt)
                 else:
-                    from lucivant_sim.batch import run_many
+                    from lucivant_sim.batch import trajectory_counts
 
-                    res = run_many([j.c.circuit for j in group], shots, seeds=[j.req.seed for j in group], noise=group[0].req.noise, method="trajectories", dtype=self.dtype)
-                    for j, r in zip(group, res):
-                        self._finish(j, r.counts)
+                    counts = trajectory_counts([j.c for j in group], [j.c.program for j in group], shots, [j.req.seed for j in group], self.dtype)
+                    for j, cnt in zip(group, counts):
+                        self._finish(j, cnt)
             except BaseException as exc:  # noqa: BLE001
                 for j in group:
                     self._finish_error(j.req, exc)
@@ -394,6 +399,7 @@ class Server:
             "circuits_per_s": float(lat.size / span) if span > 0 else float("inf"),
             "ticks": self.ticks,
             "peak_hbm_bytes": self.peak_used,
+            "scheduler_seconds": {k: round(v, 3) for k, v in self.host_seconds.items()},
         }
 
 

@@ -237,28 +237,39 @@ def test_dense_tiled_placements_equal_the_oracle_and_the_direct_kernel(k: int, d
 
 
 @pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
-def test_dense_k5_operator_splits_agree(dtype: int) -> None:
-    """Every split of a 5-qubit operator's reads between the scalar cache and LDS gives the same
-    state bit for bit, equal to the oracle, with and without a selector; other values are refused."""
-    n = 11
+@pytest.mark.parametrize("k", [3, 4, 5])
+@pytest.mark.parametrize("extra", [4, 7])
+def test_dense_streaming_and_matrix_core_variants_agree(k: int, dtype: int, extra: int) -> None:
+    """Every kernel variant (vector or matrix cores, streaming or cached accesses) gives the oracle's
+    state on high, low, mixed and scattered placements, leaves unselected states untouched, and
+    the switches read back. complex64 always takes the vector kernel. ``extra`` = 4 is the
+    smallest state the matrix-core kernel takes (one step of 16 groups)."""
+    n = k + extra
     lib = nat.load()
-    rng = np.random.default_rng(77)
-    placements = [(10, 9, 8, 7, 6), (4, 3, 2, 1, 0), (0, 10, 5, 2, 7)] + [tuple(int(x) for x in rng.choice(n, size=5, replace=False)) for _ in range(3)]
-    old = lib.lsim_set_dense_k5_split(-1)
+    rng = np.random.default_rng(10 * k)
+    placements = [tuple(range(n - 1, n - 1 - k, -1)), tuple(range(k - 1, -1, -1)), tuple(range(k)), (n - 1, 0) + tuple(range(2, 2 + k - 2))]
+    placements += [tuple(int(x) for x in rng.choice(n, size=k, replace=False)) for _ in range(3)]
+    placements += [tuple(int(x) for x in rng.choice(range(3, n), size=k, replace=False)) for _ in range(3)]
+    old_m, old_s = lib.lsim_set_dense_mfma(-1), lib.lsim_set_dense_streaming(-1)
     try:
         for i, targets in enumerate(placements):
-            psi = states(n, 2, 500 + i)
-            u = unitary(5, 90 + i)
-            want = orc.apply_sv(psi[1], u, targets, n)
+            psi = states(n, 3, 700 + 10 * k + i)
+            u = unitary(k, 120 + i)
+            want0 = orc.apply_sv(psi[0], u, targets, n)
+            want2 = orc.apply_sv(psi[2], u, targets, n)
             outs = []
-            for split in (0, 8, 12, 16, 20, 24, 32):
-                assert lib.lsim_set_dense_k5_split(split) == 0
-                assert lib.lsim_set_dense_k5_split(-1) == split
-                outs.append(apply_dense(psi, n, np.stack([u, u]), targets, dtype, [-1, 1]))
-            for o in outs:
-                assert np.array_equal(o, outs[0]), targets
-                assert np.array_equal(o[0], psi[0].astype(NPDT[dtype])), targets
-                assert np.abs(o[1] - want).max() < TOL[dtype] * 4, targets
-        assert lib.lsim_set_dense_k5_split(7) == nat.EINVAL
+            for mask in (0, 1 << k):
+                for streaming in (0, 1):
+                    lib.lsim_set_dense_mfma(mask)
+                    lib.lsim_set_dense_streaming(streaming)
+                    assert lib.lsim_set_dense_mfma(-1) == mask and lib.lsim_set_dense_streaming(-1) == streaming
+                    got = apply_dense(psi, n, np.stack([u, u]), targets, dtype, [1, -1, 0])
+                    assert np.abs(got[0] - want0).max() < TOL[dtype] * 4, (targets, mask, streaming)
+                    assert np.abs(got[2] - want2).max() < TOL[dtype] * 4, (targets, mask, streaming)
+                    assert np.array_equal(got[1], psi[1].astype(NPDT[dtype])), (targets, mask, streaming)
+                    outs.append(got)
+            # streaming changes nothing but the cache policy: bit-identical results
+            assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[2], outs[3]), targets
     finally:
-        lib.lsim_set_dense_k5_split(old)
+        lib.lsim_set_dense_mfma(old_m)
+        lib.lsim_set_dense_streaming(old_s)

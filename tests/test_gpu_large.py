@@ -4,6 +4,9 @@ probability sums and expectations of diagonal observables against the probabilit
 
 from __future__ import annotations
 
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 import torch
@@ -41,6 +44,33 @@ def test_26_qubit_circuit_round_trip_and_observables() -> None:
     amp0 = complex(sv.buffer[0].item())
     rest = float(torch.linalg.vector_norm(sv.buffer[1:]).item())
     assert abs(amp0 - 1) < 1e-10 and rest < 1e-10
+
+
+@pytest.mark.parametrize("width", [4, 5])
+def test_26_qubit_circuit_matches_the_oracle_fixture(width: int) -> None:
+    """The 26-qubit configuration against the oracle's own full run (tests/golden/
+    make_full_size.py oracle-c2): 4096 amplitudes at seeded indices, probability checksums and the
+    8 Pauli expectations, within 1e-10, for two fusion plans (widths 4 and 5)."""
+    rec = json.loads((Path(__file__).parent / "golden" / "full_c2_oracle.json").read_text())
+    src, labels = wl.config_c2()
+    assert labels == rec["labels"]
+    c = ls.parse(src)
+    sv = Statevector(26)
+    sv.run(plan_statevector(c.operations, 26, width))
+    idx = torch.tensor(rec["indices"], dtype=torch.int64, device=sv.buffer.device)
+    got = sv.buffer[idx].cpu().numpy()
+    want = np.array([complex(re, im) for re, im in rec["amplitudes"]])
+    assert np.abs(got - want).max() < 1e-10
+    probs = sv.probabilities()
+    assert abs(probs.sum() - rec["prob_sum"]) < 1e-10
+    index_mean = float(np.dot(probs, np.arange(2**26, dtype=np.float64)) / 2**26)
+    assert abs(index_mean - rec["prob_index_mean"]) < 1e-10
+    assert int(probs.argmax()) == rec["prob_argmax"] and abs(float(probs.max()) - rec["prob_max"]) < 1e-12
+    e = sv.expectations(labels)
+    assert np.abs(e - np.array(rec["expectations"])).max() < 1e-10
+    if width == 5:
+        # the public API re-simulates per call (simulator.py:517-519)
+        assert abs(ls.expectation(src, labels[0]) - rec["expectations"][0]) < 1e-10
 
 
 def test_13_qubit_density_matrix_trace_and_round_trip() -> None:
