@@ -14,8 +14,9 @@
 
 namespace lsim {
 
-// bit k set: complex128 k-qubit operators (k = 3..5) run on the matrix cores (k_dense_mfma)
-static int g_mfma_widths = 1 << 5;
+// bit k set: complex128 k-qubit operators (k = 3..5) run on the matrix cores (k_dense_mfma);
+// for k = 3, 4 only placements with a target below bit 3, where the vector kernel's lanes stride
+static int g_mfma_widths = (1 << 3) | (1 << 4) | (1 << 5);
 // streaming (non-temporal) loads and stores for placements with every target at bit 3 or above
 static int g_dense_streaming = 1;
 
@@ -503,7 +504,7 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
     return launch_dense_tiled<float>(states, nbits, batch, k, tl, mats, sel, as_stream(stream));
   }
   if (dtype == LSIM_C128) {
-    if (k >= 3 && ((g_mfma_widths >> k) & 1) && nbits >= k + 4)
+    if (k >= 3 && ((g_mfma_widths >> k) & 1) && nbits >= k + 4 && (k == 5 || t.sorted[0] < 3))
       return launch_dense_mfma(states, nbits, batch, k, t, mats, sel, as_stream(stream));
     return launch_dense<double>(states, nbits, batch, k, t, mats, sel, as_stream(stream));
   }

@@ -151,7 +151,7 @@ def apply(num_qubits: int) -> tuple[int, int, int]:
         return simulator.FUSION_WIDTH, 0, 0
     lib = nat.load()
     lib.lsim_set_dense_tiling(int(rec.get("tiled", 1)))
-    lib.lsim_set_dense_mfma(int(rec.get("mfma", 1 << 5)))
+    lib.lsim_set_dense_mfma(int(rec.get("mfma", (1 << 3) | (1 << 4) | (1 << 5))))
     lib.lsim_set_dense_streaming(int(rec.get("streaming", 1)))
     return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0))
 

@@ -131,7 +131,7 @@ class Server:
         self.ticks = 0
         self.peak_used = 0
         #: scheduler-thread time per phase (admission, small batches, wide launches, completion)
-        self.host_seconds = {"admit": 0.0, "small": 0.0, "wide_launch": 0.0, "wide_finish": 0.0}
+        self.host_seconds = {"admit": 0.0, "small": 0.0, "wide_launch": 0.0, "wide_outputs": 0.0, "wide_finish": 0.0, "copy_out": 0.0}
 
     # -- submission ---------------------------------------------------------------------
 
@@ -238,10 +238,13 @@ class Server:
         for job in wide:
             if job.finishing is None and job.passes is not None and job.next_pass >= len(job.passes) and job.state is not None:
                 self._queue_outputs(job)
+        t4 = time.perf_counter()
+        self.host_seconds["wide_outputs"] += t4 - t3
+        for job in wide:
             if job.finishing is not None and job.finishing[0].query():
                 self._complete_wide(job)
         self._sampling = [f for f in self._sampling if not f.done()]
-        self.host_seconds["wide_finish"] += time.perf_counter() - t3
+        self.host_seconds["wide_finish"] += time.perf_counter() - t4
         return True
 
     def _run_small(self, jobs: list[_Job]) -> None:
@@ -344,7 +347,9 @@ class Server:
             return
         assert job.finishing is not None
         _, out = job.finishing
+        t0 = time.perf_counter()
         values = out.cpu().numpy()  # the event has completed: no wait
+        self.host_seconds["copy_out"] += time.perf_counter() - t0
         job.finishing = None
         if job.terms is not None:
             total = 0.0

@@ -1,7 +1,7 @@
 """Full-size fixtures: the 26-qubit statevector and 13-qubit density-matrix configurations.
 
     python tests/golden/make_full_size.py oracle-c2      # ~15 min, NumPy oracle
-    python tests/golden/make_full_size.py reference-c3  # ~1 h, the reference itself
+    python tests/golden/make_full_size.py reference-c3  # ~50 min, the reference itself
 
 ``oracle-c2`` runs the oracle (oracle/engine.py, pinned by tests/test_oracle.py) on the 26-qubit
 configuration and records what a device run must reproduce within 1e-10: 4096 amplitudes at

@@ -73,6 +73,19 @@ def test_26_qubit_circuit_matches_the_oracle_fixture(width: int) -> None:
         assert abs(ls.expectation(src, labels[0]) - rec["expectations"][0]) < 1e-10
 
 
+def test_13_qubit_density_matrix_counts_equal_the_reference() -> None:
+    """The 13-qubit density-matrix configuration (noise on every gate, readout errors), 4000 shots
+    at seed 7, against lucivant-sim 0.3.0's own run of it (tests/golden/make_full_size.py
+    reference-c3): identical counts, digests and fingerprint."""
+    rec = json.loads((Path(__file__).parent / "golden" / "full_c3_reference.json").read_text())
+    res = ls.run(wl.config_c3(), 4000, seed=7, noise=wl.noise_c3(), method="density_matrix")
+    assert res.circuit_digest == rec["circuit_digest"]
+    assert res.noise_digest == rec["noise_digest"]
+    assert res.fingerprint == rec["fingerprint"]
+    assert res.counts == rec["counts"]
+    assert res.counts_digest == rec["counts_digest"]
+
+
 def test_13_qubit_density_matrix_trace_and_round_trip() -> None:
     src = wl.config_c3(13, 6)
     c = ls.parse(src)
