@@ -104,8 +104,8 @@ __global__ void __launch_bounds__(kBlock)
 // For one amplitude index the 16 lanes of a row read 16 consecutive groups. The flop count equals
 // the vector kernel's (4 real products per complex one); what changes is operand delivery, which
 // limits the vector kernel at K = 5 (an LDS broadcast per four FMAs). NT: streaming (non-temporal)
-// loads and stores, chosen when every target is at bit 3 or above, so that each load instruction
-// covers whole 128-byte lines and nothing is gained by keeping them in cache.
+// loads and stores, chosen when each load instruction covers whole 128-byte lines (see
+// launch_dense_mfma), so nothing is gained by keeping them in cache.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
@@ -383,8 +383,15 @@ static int launch_dense_mfma(void* states, int nbits, long long batch, int k, co
   // enough wavefronts to fill 256 CUs several times over, then grid-stride
   const uint32_t waves = batches < 8192u ? batches : 8192u;
   const uint32_t gx = (waves + 3) / 4;
-  // streaming accesses when every target is at bit 3 or above (whole lines per instruction)
-  const bool nt = g_dense_streaming && t.sorted[0] >= 3;
+  // Streaming accesses when each load instruction covers whole 128-byte lines (index bits 0-2):
+  // an instruction spans the 4 lowest non-target bits (16 consecutive groups) and the two targets
+  // the lanes' k selects (listed[k-2], listed[k-1]), so every target below bit 3 must be one of
+  // those two.
+  uint32_t low_targets = 0;
+  for (int j = 0; j < k; ++j)
+    if (t.listed[j] < 3) low_targets |= 1u << t.listed[j];
+  const uint32_t lane_targets = (1u << t.listed[k - 2]) | (1u << t.listed[k - 1]);
+  const bool nt = g_dense_streaming && (low_targets & ~lane_targets) == 0;
   for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
     const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
     const dim3 grid(gx, gy), block(kBlock);

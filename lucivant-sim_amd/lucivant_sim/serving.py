@@ -63,6 +63,8 @@ class Request:
     # the program as read (in a front-end worker when the pool runs, else in ``submit``)
     _reading: Any = field(default=None, repr=False)
     _compiled: Compiled | None = field(default=None, repr=False)
+    #: when the request was read, admitted and first launched, and its outputs queued
+    stamps: dict[str, float] = field(default_factory=dict, repr=False)
 
     def wait(self, timeout: float | None = None) -> Any:
         self.event.wait(timeout)
@@ -88,9 +90,9 @@ class _Job:
     next_pass: int = 0
     stream: torch.cuda.Stream | None = None
     mats: Any = None
-    addrs: list[int] | None = None
+    calls: list[tuple[Any, tuple]] | None = None
     # outputs queued behind the last pass: (event, device tensor), read once the event completes
-    finishing: tuple[torch.cuda.Event, torch.Tensor] | None = None
+    finishing: tuple[torch.cuda.Event, torch.Tensor, Any] | None = None
 
 
 class Server:
@@ -199,6 +201,7 @@ class Server:
                 req._compiled = req._reading.get()
                 req._reading = None
             c = req._compiled
+            req.stamps.setdefault("read", time.perf_counter())
             try:
                 if c.error is not None:
                     raise c.error
@@ -212,6 +215,7 @@ class Server:
                 self.used += job.footprint
                 self.peak_used = max(self.peak_used, self.used)
                 self.resident.append(job)
+                job.req.stamps["admitted"] = time.perf_counter()
             else:
                 still.append(job)
         self.waiting = still
@@ -305,6 +309,7 @@ class Server:
                 self._finish_error(job.req, exc)
                 self._release(job)
             return
+        lib = nat.load()
         if job.state is None:
             job.stream = self.streams[self._rr % len(self.streams)]
             self._rr += 1
@@ -312,19 +317,26 @@ class Server:
                 job.state = Statevector(job.c.num_qubits, dtype=self.dtype)
                 job.passes = job.c.passes
                 assert job.passes is not None
-                job.mats, job.addrs = upload([p.matrix for p in job.passes], self.dtype)
-        assert job.passes is not None and job.addrs is not None and job.stream is not None
-        lib = nat.load()
-        hs = job.stream.cuda_stream
-        state = ptr(job.state.buffer)
-        stop = min(len(job.passes), job.next_pass + self.chunk)
-        for p, addr in zip(job.passes[job.next_pass : stop], job.addrs[job.next_pass : stop]):
-            if p.kind == "scale":
-                z = complex(p.matrix[0])
-                lib.lsim_sv_scale(state, self.dtype, job.c.num_qubits, 1, z.real, z.imag, hs)
-                continue
-            fn = lib.lsim_sv_apply_dense if p.kind == "dense" else lib.lsim_sv_apply_diag
-            fn(state, self.dtype, job.c.num_qubits, 1, p.k, nat.int_array(p.bits), addr, None, hs)
+                job.mats, addrs = upload([p.matrix for p in job.passes], self.dtype)
+            # every launch's function and arguments, built once
+            state, n, hs = ptr(job.state.buffer), job.c.num_qubits, job.stream.cuda_stream
+            job.calls = []
+            for p, addr in zip(job.passes, addrs):
+                if p.kind == "scale":
+                    z = complex(p.matrix[0])
+                    job.calls.append((lib.lsim_sv_scale, (state, self.dtype, n, 1, z.real, z.imag, hs)))
+                else:
+                    fn = lib.lsim_sv_apply_dense if p.kind == "dense" else lib.lsim_sv_apply_diag
+                    job.calls.append((fn, (state, self.dtype, n, 1, p.k, nat.int_array(p.bits), addr, None, hs)))
+        assert job.calls is not None
+        job.req.stamps.setdefault("launched", time.perf_counter())
+        stop = min(len(job.calls), job.next_pass + self.chunk)
+        for fn, args in job.calls[job.next_pass : stop]:
+            if fn(*args):
+                exc = nat.NativeError(lib.lsim_last_error().decode(errors="replace"))
+                self._finish_error(job.req, exc)
+                self._release(job)
+                return
         job.next_pass = stop
 
     def _queue_outputs(self, job: _Job) -> None:
@@ -333,22 +345,28 @@ class Server:
         assert job.state is not None and job.stream is not None
         with torch.cuda.stream(job.stream):
             sv = job.state
+            dev = None
             if job.terms is not None:
                 out = launch_expectations("lsim_sv_pauli_expect", sv.buffer, sv.dtype, sv.n, sv.n, list(job.terms))
             else:
-                out = torch.empty(1 << sv.n, dtype=torch.float64, device=sv.buffer.device)
-                nat.call("lsim_sv_probabilities", ptr(sv.buffer), sv.dtype, sv.n, 1, ptr(out), job.stream.cuda_stream)
+                dev = torch.empty(1 << sv.n, dtype=torch.float64, device=sv.buffer.device)
+                nat.call("lsim_sv_probabilities", ptr(sv.buffer), sv.dtype, sv.n, 1, ptr(dev), job.stream.cuda_stream)
+                # straight into pinned host memory by DMA, behind the kernel on the same stream
+                out = torch.empty(1 << sv.n, dtype=torch.float64, pin_memory=True)
+                out.copy_(dev, non_blocking=True)
             done = torch.cuda.Event()
             done.record(job.stream)
-        job.finishing = (done, out)
+        job.finishing = (done, out, dev)  # dev stays alive until the copy has completed
+        job.req.stamps["outputs_queued"] = time.perf_counter()
 
     def _complete_wide(self, job: _Job) -> None:
         if job.req.done:
             return
         assert job.finishing is not None
-        _, out = job.finishing
+        _, out, _dev = job.finishing
         t0 = time.perf_counter()
-        values = out.cpu().numpy()  # the event has completed: no wait
+        # the event has completed: the values are on the host already (pinned) or copied now
+        values = out.numpy() if out.device.type == "cpu" else out.cpu().numpy()
         self.host_seconds["copy_out"] += time.perf_counter() - t0
         job.finishing = None
         if job.terms is not None:
@@ -391,12 +409,24 @@ class Server:
                 self.used -= job.footprint
         job.state = None
         job.mats = None
+        job.calls = None
 
     def stats(self) -> dict[str, float]:
         lat = np.array([r.latency for r in self.completed if r.error is None])
         if lat.size == 0:
             return {"completed": 0}
         span = max(r.done for r in self.completed) - min(r.arrival for r in self.completed)
+        phases: dict[str, list[float]] = {"read": [], "wait": [], "device": [], "finish": []}
+        for r in self.completed:
+            st = r.stamps
+            if r.error is not None or "read" not in st:
+                continue
+            phases["read"].append(st["read"] - r.arrival)
+            if "admitted" in st:
+                phases["wait"].append(st["admitted"] - st["read"])
+            if "launched" in st and "outputs_queued" in st:
+                phases["device"].append(st["outputs_queued"] - st["launched"])
+                phases["finish"].append(r.done - st["outputs_queued"])
         return {
             "completed": int(lat.size),
             "p50_ms": float(np.percentile(lat, 50) * 1e3),
@@ -405,6 +435,9 @@ class Server:
             "ticks": self.ticks,
             "peak_hbm_bytes": self.peak_used,
             "scheduler_seconds": {k: round(v, 3) for k, v in self.host_seconds.items()},
+            # median seconds per phase: arrival -> read, read -> admitted, launches of wide
+            # requests, outputs queued -> result
+            "phase_median_s": {k: round(float(np.median(v)), 4) for k, v in phases.items() if v},
         }
 
 

@@ -26,6 +26,7 @@ import numpy as np
 from numpy.typing import NDArray
 
 from lucivant_sim import __version__
+from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Circuit, IfElse
 from lucivant_sim.clifford import clifford_problem
 from lucivant_sim.engines import MAX_DENSITY_QUBITS, DensityMatrix, SimulationError, Statevector, marginal
@@ -53,8 +54,10 @@ MAX_SHOTS = 10_000_000
 #: the fingerprint, so results stay verifiable against the reference implementation.
 ENGINE = "lucivant-sim-mi355x"
 
-#: Statevector fusion width (qubits per dense pass); set by the autotuner when it has run.
-FUSION_WIDTH = 4
+#: Statevector fusion width (qubits per dense pass) when the autotuner has stored no choice for
+#: this device. 5 measured best on gfx950 (26 qubits: 62.4 ms per simulation at width 5, 77.7 ms
+#: at 4, with 5-qubit operators on the matrix cores).
+FUSION_WIDTH = 5
 
 Program = str | os.PathLike[str] | Circuit | Any
 
@@ -100,6 +103,9 @@ class Result:
     #: Final state of an ideal static circuit before its measurements (statevector method).
     statevector: NDArray[np.complex128] | None = field(default=None, repr=False, compare=False)
     engine: str = field(default=ENGINE, compare=False)
+    #: For ``precision="complex64"`` runs: the precision used and how the run compares with
+    #: complex128 (first shots rerun with the same draws). Not part of the fingerprint.
+    precision: dict[str, Any] | None = field(default=None, repr=False, compare=False)
 
     @property
     def fingerprint(self) -> str:
@@ -323,12 +329,21 @@ def run(
     noise: NoiseModel | None = None,
     method: Method | str = "automatic",
     params: Mapping[str, float] | None = None,
+    precision: str = "complex128",
 ) -> Result:
-    """Simulate ``program`` ``shots`` times and return the counts (simulator.py:320-393)."""
+    """Simulate ``program`` ``shots`` times and return the counts (simulator.py:320-393).
+
+    ``precision="complex64"`` (beyond the reference) runs the trajectory method's states in
+    complex64 with float32 arithmetic and float64 reductions and renormalisation; the Result's
+    ``precision`` field then states how its shots compare with complex128."""
     if not 0 <= shots <= MAX_SHOTS:
         raise ValueError(f"shots must be between 0 and {MAX_SHOTS:,}")
+    if precision not in ("complex128", "complex64"):
+        raise ValueError("precision must be 'complex128' or 'complex64'")
     circuit = to_circuit(program, params)
     chosen = choose_method(circuit, noise, method)
+    if precision == "complex64" and chosen != "trajectories":
+        raise ValueError("precision='complex64' applies to the trajectory method; exact outputs stay complex128")
     seed = secrets.randbelow(2**63) if seed is None else int(seed)
     if seed < 0:
         raise ValueError("seed must not be negative")
@@ -340,12 +355,19 @@ def run(
             f"the {chosen!r} method isn't part of the MI355X build; pass method='statevector' "
             "(or 'trajectories' for noisy or dynamic circuits)"
         )
+    checked = None
     if chosen == "trajectories":
-        from lucivant_sim.trajectories import run_trajectories
+        from lucivant_sim.trajectories import complex64_check, run_trajectories, shot_values
 
         if circuit.num_qubits > 0:
             Statevector(circuit.num_qubits)  # the size check before any shot
-        tally = run_trajectories(circuit, noise, shots, rng)
+        if precision == "complex64":
+            values, used = shot_values(circuit, noise, shots, rng, dtype=nat.C64)
+            uniq, cnt = np.unique(np.asarray(values, dtype=np.int64), return_counts=True)
+            tally = {int(v): int(c) for v, c in zip(uniq, cnt)}
+            checked = complex64_check(circuit, noise, seed, values, used)
+        else:
+            tally = run_trajectories(circuit, noise, shots, rng)
         counts = {circuit.format_bits(b): n for b, n in sorted(tally.items())} if circuit.num_clbits else {}
     else:
         state: Statevector | DensityMatrix = (
@@ -368,6 +390,7 @@ def run(
         version=__version__,
         elapsed_seconds=elapsed,
         statevector=vector,
+        precision=checked,
     )
 
 

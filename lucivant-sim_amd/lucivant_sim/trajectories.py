@@ -391,23 +391,49 @@ def run_sequential(circuit: Circuit, noise: NoiseModel | None, shots: int, rng: 
     return results
 
 
+def shot_values(
+    circuit: Circuit, noise: NoiseModel | None, shots: int, rng: np.random.Generator, *, dtype: int = nat.C128
+) -> tuple[list[int] | np.ndarray, int]:
+    """The classical bit pattern of every shot, in shot order, and the precision it was computed
+    in (circuits whose if/else arms draw unequally run shot after shot in complex128)."""
+    if shots == 0:
+        return [], dtype
+    if draw_count(circuit.operations, noise) is None:
+        return run_sequential(circuit, noise, shots, rng), nat.C128
+    values = run_lds(circuit, noise, shots, rng, dtype=dtype)
+    if values is None:
+        values = run_batched(circuit, noise, shots, rng, dtype=dtype)
+    return values, dtype
+
+
 def run_trajectories(
     circuit: Circuit, noise: NoiseModel | None, shots: int, rng: np.random.Generator, *, dtype: int = nat.C128
 ) -> dict[int, int]:
     """Tally of classical bit patterns over ``shots`` trajectories."""
-    if shots == 0:
-        return {}
-    if draw_count(circuit.operations, noise) is None:
-        bits = run_sequential(circuit, noise, shots, rng)
+    values, _ = shot_values(circuit, noise, shots, rng, dtype=dtype)
+    if isinstance(values, list):
         tally: dict[int, int] = {}
-        for v in bits:
+        for v in values:
             tally[v] = tally.get(v, 0) + 1
         return tally
-    values = run_lds(circuit, noise, shots, rng, dtype=dtype)
-    if values is None:
-        values = run_batched(circuit, noise, shots, rng, dtype=dtype)
     uniq, counts = np.unique(values, return_counts=True)
     return {int(v): int(c) for v, c in zip(uniq, counts)}
 
 
-__all__ = ["compile_program", "draw_count", "run_batched", "run_sequential", "run_trajectories"]
+#: shots of a complex64 run that are rerun in complex128 to state its error
+PRECISION_CHECK_SHOTS = 256
+
+
+def complex64_check(circuit: Circuit, noise: NoiseModel | None, seed: int, values: list[int] | np.ndarray, used: int) -> dict[str, object]:
+    """How a complex64 trajectory run compares with complex128: its first shots are rerun in
+    complex128 with the same draws (the same seeded stream) and compared shot by shot. The shots
+    are deterministic given their draws, so any difference is the precision's doing."""
+    if used != nat.C64:
+        return {"dtype": "complex128", "note": "this circuit runs shot after shot in complex128"}
+    m = min(len(values), PRECISION_CHECK_SHOTS)
+    ref, _ = shot_values(circuit, noise, m, np.random.default_rng(seed), dtype=nat.C128)
+    differing = int(np.sum(np.asarray(values[:m], dtype=np.int64) != np.asarray(ref, dtype=np.int64)))
+    return {"dtype": "complex64", "checked_shots": m, "shots_differing_from_complex128": differing}
+
+
+__all__ = ["complex64_check", "compile_program", "draw_count", "run_batched", "run_sequential", "run_trajectories", "shot_values"]

@@ -257,3 +257,21 @@ def test_complex64_trajectories_stay_close_to_complex128() -> None:
     b = run_batched(circuit, noise, 2000, np.random.default_rng(3), dtype=nat.C64)
     # same draws: shots can only differ where a draw lies within float32 rounding of a boundary
     assert np.mean(a != b) < 0.01
+
+
+def test_complex64_trajectories_state_their_error() -> None:
+    """precision='complex64' runs trajectories in complex64 and the Result states how its shots
+    compare with complex128 (the first shots rerun with the same draws). The draws are identical,
+    so outcomes can differ only where a probability sits within float32 rounding of a draw."""
+    src, noise = wl.teleport_chain(9, 3), wl.noise_c4()
+    r64 = ls.run(src, 2000, seed=3, noise=noise, method="trajectories", precision="complex64")
+    r128 = ls.run(src, 2000, seed=3, noise=noise, method="trajectories")
+    assert r64.precision is not None and r64.precision["dtype"] == "complex64"
+    assert r64.precision["checked_shots"] == 256
+    assert r64.precision["shots_differing_from_complex128"] <= 2
+    assert r128.precision is None
+    tvd = 0.5 * sum(abs(r64.counts.get(k, 0) - r128.counts.get(k, 0)) for k in set(r64.counts) | set(r128.counts)) / 2000
+    assert tvd <= 2 / 2000
+    assert r64.fingerprint == r128.fingerprint  # precision is not part of what was run
+    with pytest.raises(ValueError):
+        ls.run(wl.random_layers(5, 2, 1), 10, seed=3, method="statevector", precision="complex64")
