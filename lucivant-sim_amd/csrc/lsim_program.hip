@@ -78,7 +78,9 @@ __device__ __forceinline__ void lds_dense_k(typename Cx<R>::V* s, int n, const u
     V in[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) in[c] = s[base + offs[c]];
-#pragma unroll
+    // one output row at a time for wide operators: fully unrolled, the 2^K x 2^K operator reads
+    // would be scheduled ahead and spill (this kernel holds every op kind at once)
+#pragma unroll(K >= 4 ? 1 : D)
     for (int r = 0; r < D; ++r) {
       R re = 0, im = 0;
 #pragma unroll
@@ -116,7 +118,9 @@ __device__ __forceinline__ void lds_diag(typename Cx<R>::V* s, int n, int k, con
   const uint32_t size = 1u << n;
   for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
     uint32_t m = 0;
-    for (int q = 0; q < k; ++q) m |= ((i >> t[q]) & 1u) << (k - 1 - q);
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (q < k) m |= ((i >> t[q]) & 1u) << (k - 1 - q);
     const double2 f = d[m];
     const V a = s[i];
     V o;
@@ -147,51 +151,72 @@ __device__ __forceinline__ void wg_sum(double (&v)[NV], double* red) {
   __syncthreads();
 }
 
-// Reduced density matrix on k <= 2 target qubits (row-major D x D complex, Hermitian).
-template <typename R>
-__device__ void lds_rdm(const typename Cx<R>::V* s, int n, int k, const uint8_t* t, double* red,
-                        double2* rho) {
+// Reduced density matrix on K <= 2 target qubits (row-major D x D complex, Hermitian). K is a
+// template parameter so that the accumulator indices are constants and stay in registers.
+template <int K, typename R>
+__device__ void lds_rdm_k(const typename Cx<R>::V* s, int n, const uint8_t* t, double* red,
+                          double2* rho) {
   using V = typename Cx<R>::V;
-  const int D = 1 << k;
-  // (re, im) of sum a_r conj(a_c) for r <= c: at most 4 * 5 / 2 pairs
-  double acc[20];
+  constexpr int D = 1 << K;
+  constexpr int NV = D * (D + 1);  // (re, im) of sum a_r conj(a_c) for r <= c
+  double acc[NV];
 #pragma unroll
-  for (int i = 0; i < 20; ++i) acc[i] = 0;
-  const uint32_t groups = 1u << (n - k);
+  for (int i = 0; i < NV; ++i) acc[i] = 0;
+  const uint32_t groups = 1u << (n - K);
   uint32_t lo = t[0], hi = t[0];
-  if (k == 2) {
+  if constexpr (K == 2) {
     lo = t[0] < t[1] ? t[0] : t[1];
     hi = t[0] < t[1] ? t[1] : t[0];
   }
+  uint32_t offs[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+      if ((c >> (K - 1 - i)) & 1) o |= 1u << t[i];
+    offs[c] = o;
+  }
   for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
     uint32_t base = ((g >> lo) << (lo + 1)) | (g & ((1u << lo) - 1u));
-    if (k == 2) base = ((base >> hi) << (hi + 1)) | (base & ((1u << hi) - 1u));
-    double x[4], y[4];
+    if constexpr (K == 2) base = ((base >> hi) << (hi + 1)) | (base & ((1u << hi) - 1u));
+    double x[D], y[D];
+#pragma unroll
     for (int c = 0; c < D; ++c) {
-      uint32_t o = base;
-      for (int i = 0; i < k; ++i)
-        if ((c >> (k - 1 - i)) & 1) o |= 1u << t[i];
-      const V a = s[o];
+      const V a = s[base | offs[c]];
       x[c] = (double)a.x;
       y[c] = (double)a.y;
     }
     int v = 0;
+#pragma unroll
     for (int r = 0; r < D; ++r)
+#pragma unroll
       for (int c = r; c < D; ++c) {
         acc[v++] += x[r] * x[c] + y[r] * y[c];
         acc[v++] += y[r] * x[c] - x[r] * y[c];
       }
   }
-  wg_sum<20>(acc, red);
+  wg_sum<NV>(acc, red);
   if (threadIdx.x == 0) {
     int v = 0;
+#pragma unroll
     for (int r = 0; r < D; ++r)
+#pragma unroll
       for (int c = r; c < D; ++c) {
         rho[r * D + c] = make_double2(acc[v], r == c ? 0.0 : acc[v + 1]);
         rho[c * D + r] = make_double2(acc[v], r == c ? 0.0 : -acc[v + 1]);
         v += 2;
       }
   }
+}
+
+template <typename R>
+__device__ void lds_rdm(const typename Cx<R>::V* s, int n, int k, const uint8_t* t, double* red,
+                        double2* rho) {
+  if (k == 1)
+    lds_rdm_k<1, R>(s, n, t, red, rho);
+  else
+    lds_rdm_k<2, R>(s, n, t, red, rho);
 }
 
 struct ProgShared {
@@ -231,7 +256,16 @@ __global__ void __launch_bounds__(kProgThreads)
   unsigned long long clbits = 0;
   uint32_t flags = 0;
   for (int p = 0; p < count; ++p) {
-    const lsim_prog_op op = prog[p];
+    // the op is the same for the whole workgroup: read it into scalar registers, so that every
+    // field, and every address built from one (the operator's matrix above all), is uniform
+    lsim_prog_op op;
+    {
+      static_assert(sizeof(lsim_prog_op) % 4 == 0, "ops are whole 32-bit words");
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(prog + p);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(&op);
+#pragma unroll
+      for (int w = 0; w < (int)(sizeof(lsim_prog_op) / 4); ++w) dst[w] = __builtin_amdgcn_readfirstlane(src[w]);
+    }
     if (!guard_ok(op, flags)) continue;
     switch (op.kind) {
       case OP_DENSE:

@@ -114,6 +114,8 @@ class Server:
         self.width = fusion_width or FUSION_WIDTH
         self.chunk = chunk_passes
         self.streams = [torch.cuda.Stream() for _ in range(max(1, streams))]
+        # small batches (LDS programs) run on a stream of their own, beside the wide requests
+        self.small_stream = torch.cuda.Stream()
         self._rr = 0
         self.inbox: queue.Queue[Request] = queue.Queue()
         self.reading: list[Request] = []
@@ -230,7 +232,9 @@ class Server:
             return False
         self.ticks += 1
         small = [j for j in self.resident if j.state is None and (j.c.table is not None or j.c.program is not None)]
-        self._run_small(small)
+        if small:
+            with torch.cuda.stream(self.small_stream):
+                self._run_small(small)
         t2 = time.perf_counter()
         self.host_seconds["small"] += t2 - t1
         wide = [j for j in self.resident if j not in small]
@@ -345,15 +349,15 @@ class Server:
         assert job.state is not None and job.stream is not None
         with torch.cuda.stream(job.stream):
             sv = job.state
-            dev = None
             if job.terms is not None:
-                out = launch_expectations("lsim_sv_pauli_expect", sv.buffer, sv.dtype, sv.n, sv.n, list(job.terms))
+                dev = launch_expectations("lsim_sv_pauli_expect", sv.buffer, sv.dtype, sv.n, sv.n, list(job.terms))
             else:
                 dev = torch.empty(1 << sv.n, dtype=torch.float64, device=sv.buffer.device)
                 nat.call("lsim_sv_probabilities", ptr(sv.buffer), sv.dtype, sv.n, 1, ptr(dev), job.stream.cuda_stream)
-                # straight into pinned host memory by DMA, behind the kernel on the same stream
-                out = torch.empty(1 << sv.n, dtype=torch.float64, pin_memory=True)
-                out.copy_(dev, non_blocking=True)
+            # straight into pinned host memory by DMA, behind the kernels on the same stream, so that
+            # completing the request issues no device work (nothing that could wait on other streams)
+            out = torch.empty(dev.numel(), dtype=torch.float64, pin_memory=True)
+            out.copy_(dev, non_blocking=True)
             done = torch.cuda.Event()
             done.record(job.stream)
         job.finishing = (done, out, dev)  # dev stays alive until the copy has completed
@@ -365,8 +369,7 @@ class Server:
         assert job.finishing is not None
         _, out, _dev = job.finishing
         t0 = time.perf_counter()
-        # the event has completed: the values are on the host already (pinned) or copied now
-        values = out.numpy() if out.device.type == "cpu" else out.cpu().numpy()
+        values = out.numpy()  # the event has completed: the values are in pinned host memory
         self.host_seconds["copy_out"] += time.perf_counter() - t0
         job.finishing = None
         if job.terms is not None:

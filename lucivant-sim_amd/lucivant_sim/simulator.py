@@ -242,19 +242,18 @@ def _sample(
     position = {q: i for i, q in enumerate(measured)}
     dist = marginal(probs, circuit.num_qubits, measured)
     cumulative = np.cumsum(dist)
-    cumulative /= cumulative[-1]
     draws = rng.random(shots)
     wide = circuit.num_clbits > 62
     flips = noise is not None and any((e := noise.readout_error(q)) is not None and (e.p01 or e.p10) for q, _ in pairs)
     if not wide and not flips and len(dist) <= 1 << 16:
         # no readout draws: the bit pattern is a fixed function of the outcome, tabulated once,
         # and only the tally matters, so the draws are searched in sorted order (much faster)
+        cumulative /= cumulative[-1]
         outcomes = np.minimum(np.searchsorted(cumulative, np.sort(draws), side="right"), len(dist) - 1)
         bits = _outcome_table(tuple(pairs), tuple(measured), len(dist))[outcomes]
         values, counts = np.unique(bits, return_counts=True)
         return dict(zip(circuit.format_many(values), counts.tolist()))
-    outcomes = np.searchsorted(cumulative, draws, side="right").astype(np.int64)
-    outcomes = np.minimum(outcomes, len(dist) - 1)
+    outcomes = np.minimum(_search_normalised(cumulative, draws), len(dist) - 1).astype(np.int64)
     bits = np.zeros(shots, dtype=object if wide else np.int64)
     for q, c in pairs:
         value = (outcomes >> position[q]) & 1
@@ -270,6 +269,25 @@ def _sample(
     if wide:
         return {circuit.format_bits(int(v)): int(n) for v, n in zip(values, counts)}
     return dict(zip(circuit.format_many(values), counts.tolist()))
+
+
+def _search_normalised(cumulative: NDArray[np.float64], draws: NDArray[np.float64]) -> NDArray[np.int64]:
+    """``searchsorted(cumulative / cumulative[-1], draws, side="right")`` without dividing the whole
+    array: x / total is non-decreasing in x, so a binary search that divides only the entries it
+    probes (about log2(len) per draw) lands on the same indices, division for division."""
+    total = cumulative[-1]
+    if len(cumulative) <= 1 << 16 or not total > 0:
+        return np.searchsorted(cumulative / total, draws, side="right")
+    lo = np.zeros(len(draws), dtype=np.int64)
+    hi = np.full(len(draws), len(cumulative), dtype=np.int64)
+    while True:
+        active = lo < hi
+        if not active.any():
+            return lo
+        mid = (lo + hi) // 2
+        below = active & (cumulative[np.minimum(mid, len(cumulative) - 1)] / total <= draws)
+        lo = np.where(below, mid + 1, lo)
+        hi = np.where(active & ~below, mid, hi)
 
 
 def _sample_many(circuits: Sequence[Any], probs: NDArray[np.float64], shots: int, noise: NoiseModel | None, seeds: Sequence[int]) -> list[dict[str, int]]:

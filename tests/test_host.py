@@ -328,3 +328,22 @@ def test_program_tables_are_bounds_checked_before_launch() -> None:
             pg._validate(bad, 9, prog.mat_size, reals, prog.draws)
     with pytest.raises(ValueError, match="draw"):
         pg._validate(table, 9, prog.mat_size, reals, 0)
+
+
+def test_search_over_normalised_cumulative_sums_without_dividing_all() -> None:
+    """The sampler's binary search divides only the entries it probes, and lands where
+    searchsorted over the divided array does: with zero-probability plateaus, exact ties and draws
+    equal to normalised entries."""
+    from lucivant_sim.simulator import _search_normalised
+
+    rng = np.random.default_rng(0)
+    for n in (10, 17, 20):
+        p = rng.random(1 << n) ** 4
+        p[5:50] = 0.0
+        p /= p.sum()
+        cum = np.cumsum(p)
+        normalised = cum / cum[-1]
+        u = rng.random(3000)
+        u[:100] = normalised[rng.integers(0, len(cum), 100)]
+        u[100] = 0.0
+        assert np.array_equal(_search_normalised(cum, u), np.searchsorted(normalised, u, side="right")), n
