@@ -138,10 +138,10 @@ int lsim_dm_pauli_expect(const void* rho, int dtype, int n, long long batch, int
  * ((clbits & cmask) == cpattern) != negate. Matrices are complex128 in `mats` (also for
  * LSIM_C64 states); `reals` holds float64 side data (see kinds below).
  */
-#define LSIM_OP_DENSE 0   /* mats[mat]: 2^k x 2^k matrix */
-#define LSIM_OP_DIAG 1    /* mats[mat]: 2^k diagonal */
-#define LSIM_OP_MIX 2     /* reals[aux..aux+nb): cumulative probabilities; reals[aux+nb+j]: the mats
-                             offset of branch j or -1 for the identity; one draw */
+#define LSIM_OP_DENSE 0   /* k <= 4; mats[mat]: 2^k x 2^k matrix */
+#define LSIM_OP_DIAG 1    /* k <= 5; mats[mat]: 2^k diagonal */
+#define LSIM_OP_MIX 2     /* k <= 4; reals[aux..aux+nb): cumulative probabilities; reals[aux+nb+j]:
+                             the mats offset of branch j or -1 for the identity; one draw */
 #define LSIM_OP_KRAUS 3   /* k <= 2; mats[mat + j*4^k]: the nb Kraus operators; one draw */
 #define LSIM_OP_MEASURE 4 /* qubit t[0] into clbit (or none if clbit < 0); one draw, plus one more
                              for a readout flip when aux >= 0 (reals[aux] = p01, reals[aux+1] = p10) */

@@ -80,7 +80,8 @@ __device__ __forceinline__ void lds_dense_k(typename Cx<R>::V* s, int n, const u
     for (int c = 0; c < D; ++c) in[c] = s[base + offs[c]];
     // one output row at a time for wide operators: fully unrolled, the 2^K x 2^K operator reads
     // would be scheduled ahead and spill (this kernel holds every op kind at once)
-#pragma unroll(K >= 4 ? 1 : D)
+    constexpr int kRowUnroll = K >= 3 ? 1 : D;
+#pragma unroll kRowUnroll
     for (int r = 0; r < D; ++r) {
       R re = 0, im = 0;
 #pragma unroll
@@ -99,6 +100,9 @@ __device__ __forceinline__ void lds_dense_k(typename Cx<R>::V* s, int n, const u
   }
 }
 
+// Dense operators in programs have at most 4 qubits (the host refuses wider ones): a 5-qubit
+// operator's 32 inputs per group would hold 128 VGPRs and halve the whole interpreter's
+// occupancy (4 -> 2 waves per SIMD in complex128), which every program pays.
 template <typename R>
 __device__ __forceinline__ void lds_dense(typename Cx<R>::V* s, int n, int k, const uint8_t* t,
                                           const double2* __restrict__ U) {
@@ -106,8 +110,7 @@ __device__ __forceinline__ void lds_dense(typename Cx<R>::V* s, int n, int k, co
     case 1: lds_dense_k<1, R>(s, n, t, U); break;
     case 2: lds_dense_k<2, R>(s, n, t, U); break;
     case 3: lds_dense_k<3, R>(s, n, t, U); break;
-    case 4: lds_dense_k<4, R>(s, n, t, U); break;
-    default: lds_dense_k<5, R>(s, n, t, U); break;
+    default: lds_dense_k<4, R>(s, n, t, U); break;
   }
 }
 

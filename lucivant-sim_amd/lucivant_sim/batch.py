@@ -24,7 +24,7 @@ from lucivant_sim.circuit import Circuit
 from lucivant_sim.engines import SimulationError, pauli_masks
 from lucivant_sim.frontend import Compiled
 from lucivant_sim.noise import NoiseModel
-from lucivant_sim.optable import static_tables
+from lucivant_sim.optable import static_table, static_tables
 from lucivant_sim.simulator import METHODS, MAX_SHOTS, Result, _sample_many, _static, choose_method, expectation, run, to_circuit
 
 
@@ -96,13 +96,22 @@ def run_many(
             results[i] = run(c, shots, seed=seeds[i], noise=noise, method=m)
     for (m, n), idx in groups.items():
         if m == "statevector":
-            group = [circuits[i] for i in idx]
             try:
-                table, pool, starts, rows = static_tables(group)
+                table, pool, starts, rows = static_tables([circuits[i] for i in idx])
             except pg.Unsupported:
+                # the circuits the program kernel can't take run one by one; the rest together
+                keep = []
                 for i in idx:
-                    results[i] = run(circuits[i], shots, seed=seeds[i], noise=noise, method=m)
-                continue
+                    try:
+                        static_table(circuits[i])
+                        keep.append(i)
+                    except pg.Unsupported:
+                        results[i] = run(circuits[i], shots, seed=seeds[i], noise=noise, method=m)
+                idx = keep
+                if not idx:
+                    continue
+                table, pool, starts, rows = static_tables([circuits[i] for i in idx])
+            group = [circuits[i] for i in idx]
             out = pg.run_table(n, table, pool, starts, rows, dtype=dtype, want_probs=True, want_state=keep_statevector)
             assert out.probs is not None
             counts = _sample_many(group, out.probs, shots, noise, [seeds[i] for i in idx])

@@ -20,6 +20,8 @@ OP_DENSE, OP_DIAG, OP_MIX, OP_KRAUS, OP_MEASURE, OP_RESET, OP_COND, OP_PHASE = r
 
 MAX_FLAGS = 32
 MAX_CLBITS = 64
+#: widest dense (and mixture) operator of a program; diagonal ones may have 5 qubits
+MAX_DENSE_QUBITS = 4
 
 
 class ProgOp(ctypes.Structure):
@@ -122,6 +124,8 @@ def static_tables(circuits: Sequence[Circuit]) -> tuple[np.ndarray, np.ndarray, 
         mats = np.concatenate([matrices[i] for i in idx]).astype(np.complex128, copy=False).reshape(len(idx), d, d)
         off = ~np.eye(d, dtype=bool)
         diag = ~np.any(mats[:, off], axis=1)
+        if k > MAX_DENSE_QUBITS and not np.all(diag):
+            raise Unsupported(f"dense operators on more than {MAX_DENSE_QUBITS} qubits")
         sizes[idx] = np.where(diag, d, d * d)
         kinds[idx] = np.where(diag, OP_DIAG, OP_DENSE)
         groups.append((idx, mats, diag, d))

@@ -27,6 +27,7 @@ from lucivant_sim.plan import is_diagonal
 
 from lucivant_sim.optable import (  # noqa: F401  (re-exported)
     MAX_CLBITS,
+    MAX_DENSE_QUBITS,
     MAX_FLAGS,
     OP_COND,
     OP_DENSE,
@@ -77,8 +78,8 @@ class Program:
         return off
 
     def op(self, kind: int, qubits=(), *, mat=0, nb=0, aux=-1, draw=0, clbit=-1, guard=(0, 0), flag=0, negate=0, cmask=0, cpattern=0) -> None:  # type: ignore[no-untyped-def]
-        if len(qubits) > 5:
-            raise Unsupported("operators on more than 5 qubits")
+        if len(qubits) > 5 or (kind in (OP_DENSE, OP_MIX) and len(qubits) > MAX_DENSE_QUBITS):
+            raise Unsupported(f"dense operators on more than {MAX_DENSE_QUBITS} qubits")
         t = tuple(qubits) + (0,) * (5 - len(qubits))
         self.ops.append((kind, len(qubits), flag, negate, t, (0, 0, 0), mat, nb, aux, draw, clbit, guard[0], guard[1], cmask, cpattern))
 
@@ -406,6 +407,8 @@ def _validate(table: np.ndarray, n: int, mat_size: int, reals: np.ndarray, ndraw
         raise ValueError("unknown op kind")
     if np.any(k > np.minimum(5, n)):
         raise ValueError("an operator is wider than 5 qubits or than the state")
+    if np.any(np.isin(kind, [OP_DENSE, OP_MIX]) & (k > MAX_DENSE_QUBITS)):
+        raise ValueError(f"a dense operator is wider than {MAX_DENSE_QUBITS} qubits")
     targeted = np.isin(kind, [OP_DENSE, OP_DIAG, OP_MIX, OP_KRAUS, OP_MEASURE, OP_RESET])
     if np.any(targeted & (k < 1)) or np.any(np.isin(kind, [OP_MEASURE, OP_RESET]) & (k != 1)):
         raise ValueError("an operator has no target qubit (or a measurement more than one)")

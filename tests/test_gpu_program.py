@@ -36,10 +36,23 @@ def test_static_programs_equal_the_oracle(fuse: int) -> None:
 
 
 def test_thirteen_qubit_program_and_wide_gates() -> None:
-    src = wl.random_layers(13, 3, 9) + "ccx q[0], q[5], q[12];\nc4x q[1], q[2], q[3], q[4], q[11];\ngphase(0.4);\n"
+    src = wl.random_layers(13, 3, 9) + "ccx q[0], q[5], q[12];\nc3x q[1], q[2], q[3], q[11];\ngphase(0.4);\n"
     c = ls.parse(src)
     out = pg.run_programs([pg.compile_static(c)], want_state=True)
     assert np.abs(out.state[0] - orc.evolve_sv(c)).max() < 1e-10  # type: ignore[index]
+
+
+def test_five_qubit_dense_gates_leave_the_program_path() -> None:
+    """Programs take dense operators of at most 4 qubits; a circuit with a 5-qubit gate is refused
+    by the program compiler and the batch API runs it on the single-circuit path instead."""
+    from lucivant_sim.batch import run_many
+
+    wide = wl.random_layers(8, 2, 1) + "c4x q[1], q[2], q[3], q[4], q[7];\n"
+    with pytest.raises(pg.Unsupported):
+        pg.compile_static(ls.parse(wide))
+    narrow = wl.random_layers(8, 2, 2)
+    res = run_many([wide, narrow], 500, seeds=[4, 5])
+    assert [r.counts for r in res] == [ls.run(wide, 500, seed=4).counts, ls.run(narrow, 500, seed=5).counts]
 
 
 def test_complex64_program_within_1e5() -> None:
