@@ -472,7 +472,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "lsim::k_dense<K,double>",
+            "kernel": "lsim::k_dense<K,double,NT> and lsim::k_dense_mfma<K,NT>: every dense pass of the step",
             "avg_launch_ms": round(avg * 1e3, 4),
             "per_k_avg_ms": {str(k): round(float(np.mean(v)) * 1e3, 4) for k, v in durations.items() if v},
         },

@@ -238,12 +238,13 @@ def test_dense_tiled_placements_equal_the_oracle_and_the_direct_kernel(k: int, d
 
 @pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
 @pytest.mark.parametrize("k", [3, 4, 5])
-@pytest.mark.parametrize("extra", [4, 7])
+@pytest.mark.parametrize("extra", [4, 5, 7])
 def test_dense_streaming_and_matrix_core_variants_agree(k: int, dtype: int, extra: int) -> None:
     """Every kernel variant (vector or matrix cores, streaming or cached accesses) gives the oracle's
     state on high, low, mixed and scattered placements, leaves unselected states untouched, and
-    the switches read back. complex64 always takes the vector kernel. ``extra`` = 4 is the
-    smallest state the matrix-core kernel takes (one step of 16 groups)."""
+    the switches read back. complex64 always takes the vector kernel. ``extra`` = 5 is the
+    smallest state the matrix-core kernel takes (two steps of 16 groups); 4 stays on the vector
+    kernel."""
     n = k + extra
     lib = nat.load()
     rng = np.random.default_rng(10 * k)
