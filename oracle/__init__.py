@@ -1,9 +1,9 @@
 """The oracle: a CPU restatement in NumPy of lucivant-sim 0.3.0's engine arithmetic.
 
-TEST INFRASTRUCTURE ONLY. Nothing in the product imports this package; only ``tests/``,
-``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it, as the checker (or, for
-the baseline, as the CPU implementation that is timed). Each function cites the reference
-file:line whose behaviour it restates.
+TEST INFRASTRUCTURE ONLY. Nothing in the product imports this package; only ``tests/``, the
+smoke test in ``__graft_entry__.py`` and ``bench.py``'s CPU-baseline leg use it: to compare the
+device results with (or, for the baseline, as the CPU implementation that is timed). Each
+function cites the reference file:line whose behaviour it restates.
 
 How each part is pinned (see ``tests/test_oracle.py``):
 - gate application: brute force against the full 2^n x 2^n Kronecker operator on every target
