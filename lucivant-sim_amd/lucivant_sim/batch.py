@@ -129,29 +129,40 @@ def _run_many_frontend(programs, shots, seeds, noise, method, params, dtype, kee
         raise ValueError("give one non-negative seed per program")
     if method not in METHODS:
         raise ValueError(f"method must be one of {', '.join(METHODS)}")
-    compiled = frontend.compile_texts(programs, dict(params) if params else None, noise, method, pg.max_qubits(dtype))
-    for c in compiled:
-        if c.error is not None:
-            raise c.error
-    results: list[Result | None] = [None] * len(compiled)
-    tabled: dict[int, list[int]] = {}
-    traj: dict[int, list[int]] = {}
-    for i, c in enumerate(compiled):
-        if c.table is not None:
-            tabled.setdefault(c.num_qubits, []).append(i)
-        elif c.method == "trajectories" and 1 <= c.num_qubits <= pg.max_qubits(dtype):
-            traj.setdefault(c.num_qubits, []).append(i)
-        else:
-            results[i] = run(c.circuit, shots, seed=seeds[i], noise=noise, method=c.method)
-    for n, idx in tabled.items():
-        out = pg.run_tables(n, [compiled[i].table for i in idx], [compiled[i].pool for i in idx], dtype=dtype, want_probs=True, want_state=keep_statevector)
-        counts = _sample_many([compiled[i] for i in idx], out.probs, shots, noise, [seeds[i] for i in idx])
-        for j, i in enumerate(idx):
-            vec = out.state[j].astype(np.complex128) if out.state is not None else None
-            results[i] = _result(compiled[i], counts[j], shots, seeds[i], "statevector", noise, t0, vec)
-    circuits = [c.circuit for c in compiled]
-    for _n, idx in traj.items():
-        _trajectory_group(circuits, idx, seeds, shots, noise, dtype, results, t0)
+    lds = pg.max_qubits(dtype)
+    results: list[Result | None] = [None] * len(programs)
+    # chunks small enough that reading (in the workers) and launching, sampling and building
+    # Results (here) overlap, large enough that each launch still fills a good part of the GPU
+    chunk = max(32, -(-len(programs) // (4 * max(frontend.workers(), 1))))
+    for first, compiled in frontend.compile_chunks(programs, dict(params) if params else None, noise, method, lds, chunk):
+        for c in compiled:
+            if c.error is not None:
+                raise c.error
+        tabled: dict[int, list[int]] = {}
+        traj: dict[int, list[int]] = {}
+        for j, c in enumerate(compiled):
+            i = first + j
+            if c.table is not None:
+                tabled.setdefault(c.num_qubits, []).append(j)
+            elif c.method == "trajectories" and 1 <= c.num_qubits <= lds:
+                traj.setdefault(c.num_qubits, []).append(j)
+            else:
+                results[i] = run(c.circuit, shots, seed=seeds[i], noise=noise, method=c.method)
+        for n, idx in tabled.items():
+            out = pg.run_tables(n, [compiled[j].table for j in idx], [compiled[j].pool for j in idx], dtype=dtype, want_probs=True, want_state=keep_statevector)
+            counts = _sample_many([compiled[j] for j in idx], out.probs, shots, noise, [seeds[first + j] for j in idx])
+            for r, j in enumerate(idx):
+                vec = out.state[r].astype(np.complex128) if out.state is not None else None
+                results[first + j] = _result(compiled[j], counts[r], shots, seeds[first + j], "statevector", noise, t0, vec)
+        if traj:
+            circuits = [c.circuit for c in compiled]
+            chunk_seeds = seeds[first : first + len(compiled)]
+            chunk_results: list[Result | None] = [None] * len(compiled)
+            for _n, idx in traj.items():
+                _trajectory_group(circuits, idx, chunk_seeds, shots, noise, dtype, chunk_results, t0)
+            for j, r in enumerate(chunk_results):
+                if r is not None:
+                    results[first + j] = r
     return [r for r in results if r is not None]
 
 

@@ -197,6 +197,23 @@ def compile_texts(sources: list[str], params: dict | None, noise: Any, method: s
     return _POOL.map(_job, jobs, chunksize=chunk)
 
 
+def _chunk_job(args: tuple) -> list[Compiled]:
+    sources, rest = args
+    return [compile_one(src, *rest) for src in sources]
+
+
+def compile_chunks(sources: list[str], params: dict | None, noise: Any, method: str, lds_qubits: int, chunk: int):  # type: ignore[no-untyped-def]
+    """Read many programs in the workers, ``chunk`` per task; yields ``(first index, [Compiled])``
+    in program order as chunks complete, so the caller can launch early chunks while the workers
+    read later ones."""
+    if _POOL is None:
+        raise RuntimeError("the front-end pool isn't running")
+    rest = (params, noise, method, lds_qubits)
+    tasks = [(sources[i : i + chunk], rest) for i in range(0, len(sources), chunk)]
+    for j, part in enumerate(_POOL.imap(_chunk_job, tasks)):
+        yield j * chunk, part
+
+
 def submit(*args: Any) -> Any:
     """:func:`compile_one` of ``args`` in a worker, asynchronously: an ``AsyncResult`` whose
     ``get()`` is the :class:`Compiled`."""
