@@ -104,8 +104,8 @@ def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
     """1024 distinct 12-qubit circuits of the 12-qubit configuration's shape (20 layers of u3 on
     every qubit + cx on a random pairing, 1000 shots each, distinct seeds), from OpenQASM text to
     Results. plain = ``plain_run`` per circuit; batched = ``run_many`` over all texts: read in the
-    front-end worker processes, one LDS-program launch, sampling for the whole batch in this
-    process. ``in_process`` = the same launch and sampling from already-read circuits, with no
+    front-end worker processes in chunks, one LDS-program launch and one sampling pass per chunk in
+    this process, overlapped with the reading of later chunks. ``in_process`` = the same launch and sampling from already-read circuits, with no
     worker processes (host work in one thread)."""
     import numpy as np
 
