@@ -424,9 +424,20 @@ def main() -> int:
             fn(state, nat.C128, N, 1, k, bits, addr, None, hs)
             b.record(stream)
             all_dense.append((k, a, b))
+    # the attainable stream rate on this device: lsim_sv_scale reads and writes the same 1 GiB
+    # once (k_scale), timed the same way right after the dense launches
+    copies = []
+    for _ in range(16):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        lib.lsim_sv_scale(state, nat.C128, N, 1, 1.0, 0.0, hs)
+        b.record(stream)
+        copies.append((a, b))
     torch.cuda.synchronize()
     for k, a, b in all_dense:
         durations[k].append(a.elapsed_time(b) * 1e-3)
+    copy_avg = float(np.mean([a.elapsed_time(b) * 1e-3 for a, b in copies]))
     dense_times = [t for ts in durations.values() for t in ts]
     avg = float(np.mean(dense_times))
     achieved = BYTES_PER_PASS / avg / 1e9
@@ -475,6 +486,8 @@ def main() -> int:
             "kernel": "lsim::k_dense<K,double,NT> and lsim::k_dense_mfma<K,NT>: every dense pass of the step",
             "avg_launch_ms": round(avg * 1e3, 4),
             "per_k_avg_ms": {str(k): round(float(np.mean(v)) * 1e3, 4) for k, v in durations.items() if v},
+            "copy_kernel_GBps": round(BYTES_PER_PASS / copy_avg / 1e9, 1),
+            "frac_of_copy": round(copy_avg / avg, 4),
         },
     }
     del sv

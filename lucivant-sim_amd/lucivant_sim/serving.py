@@ -129,7 +129,7 @@ class Server:
         self._thread: threading.Thread | None = None
         # host sampling of wide count requests (a cumulative sum over 2^n probabilities) runs
         # beside the scheduler, so launches continue meanwhile
-        self._sampler = ThreadPoolExecutor(max_workers=2, thread_name_prefix="lsim-sample")
+        self._sampler = ThreadPoolExecutor(max_workers=4, thread_name_prefix="lsim-sample")
         self._lock = threading.Lock()
         self._sampling: list[Any] = []
         self.ticks = 0

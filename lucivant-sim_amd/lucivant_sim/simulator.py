@@ -16,6 +16,7 @@ import hashlib
 import json
 import os
 import secrets
+import threading
 import time
 from collections.abc import Mapping, Sequence
 from dataclasses import dataclass, field
@@ -241,7 +242,7 @@ def _sample(
     measured = sorted({q for q, _ in pairs})
     position = {q: i for i, q in enumerate(measured)}
     dist = marginal(probs, circuit.num_qubits, measured)
-    cumulative = np.cumsum(dist)
+    cumulative = _cumsum(dist)
     draws = rng.random(shots)
     wide = circuit.num_clbits > 62
     flips = noise is not None and any((e := noise.readout_error(q)) is not None and (e.p01 or e.p10) for q, _ in pairs)
@@ -269,6 +270,23 @@ def _sample(
     if wide:
         return {circuit.format_bits(int(v)): int(n) for v, n in zip(values, counts)}
     return dict(zip(circuit.format_many(values), counts.tolist()))
+
+
+_CUMSUM = threading.local()
+
+
+def _cumsum(dist: NDArray[np.float64]) -> NDArray[np.float64]:
+    """``np.cumsum(dist)`` (the same left-to-right sums). Large distributions accumulate into a
+    per-thread buffer that is kept for the next call: a fresh 2^26-entry result costs more in
+    page faults than in additions (0.33 s against 0.21 s on one host core)."""
+    if len(dist) < 1 << 20:
+        return np.cumsum(dist)
+    buf = getattr(_CUMSUM, "buf", None)
+    if buf is None or len(buf) < len(dist):
+        buf = _CUMSUM.buf = np.empty(len(dist), dtype=np.float64)
+    out = buf[: len(dist)]
+    np.cumsum(dist, out=out)
+    return out
 
 
 def _search_normalised(cumulative: NDArray[np.float64], draws: NDArray[np.float64]) -> NDArray[np.int64]:

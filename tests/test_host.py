@@ -330,6 +330,24 @@ def test_program_tables_are_bounds_checked_before_launch() -> None:
         pg._validate(table, 9, prog.mat_size, reals, 0)
 
 
+def test_large_distributions_sample_as_the_oracle_with_a_reused_buffer() -> None:
+    """Wide distributions accumulate into a kept per-thread buffer; counts still equal the
+    oracle's sampler, also when a smaller distribution follows a larger one in the same buffer
+    (a stale tail must not leak into the search)."""
+    from lucivant_sim.simulator import _cumsum, _sample
+
+    rng = np.random.default_rng(5)
+    for n in (21, 20):
+        p = rng.random(1 << n) ** 3
+        p /= p.sum()
+        assert np.array_equal(_cumsum(p), np.cumsum(p))
+        src = f"OPENQASM 3.0;\ninclude \"stdgates.inc\";\nqubit[{n}] q;\nbit[{n}] c;\nc = measure q;\n"
+        c = ls.parse(src)
+        got = _sample(c, p, 3000, None, np.random.default_rng(11))
+        want = orc.sample(c, p, 3000, None, np.random.default_rng(11))
+        assert got == want, n
+
+
 def test_search_over_normalised_cumulative_sums_without_dividing_all() -> None:
     """The sampler's binary search divides only the entries it probes, and lands where
     searchsorted over the divided array does: with zero-probability plateaus, exact ties and draws
