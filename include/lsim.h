@@ -74,8 +74,10 @@ int lsim_set_dense_tiling(int enabled);
  * rounding. No reference counterpart (a tuning switch the autotuner sets). */
 int lsim_set_dense_mfma(int widths);
 
-/* Streaming (non-temporal) loads and stores in the dense kernels for placements whose targets are
- * all at bit 3 or above: 1 on (the default), 0 off; returns the previous setting, a negative
+/* Streaming (non-temporal) loads and stores in the dense, diagonal and scale kernels: 0 never;
+ * 1 (the default) unless the launch's states together fit the 256 MiB last-level cache but not
+ * the 32 MiB of L2 (cached accesses win there); 2 at every size. The dense kernels use them only for placements whose load instructions cover whole
+ * 128-byte lines (every target at bit 3 or above). Returns the previous setting; a negative
  * argument only reads it. Results are identical either way. No reference counterpart. */
 int lsim_set_dense_streaming(int enabled);
 

@@ -17,8 +17,25 @@ namespace lsim {
 // bit k set: complex128 k-qubit operators (k = 3..5) run on the matrix cores (k_dense_mfma);
 // for k = 3, 4 only placements with a target below bit 3, where the vector kernel's lanes stride
 static int g_mfma_widths = (1 << 3) | (1 << 4) | (1 << 5);
-// streaming (non-temporal) loads and stores for placements with every target at bit 3 or above
+// streaming (non-temporal) loads and stores: 0 never; 1 (the default) unless the launch's states
+// fit the last-level cache but not the L2s; 2 at every size. The dense kernels also need a
+// placement whose load instructions cover whole lines.
 static int g_dense_streaming = 1;
+// MI355X: 8 XCDs with 4 MiB of L2 each, and a 256 MiB Infinity Cache (MALL) in front of HBM.
+// Complex128 passes with 40 random 1-5 qubit operators, per pass:
+//   - 20 qubits (16 MiB): 10.6 us streaming, 14.4 us cached. The state sits in the L2s, and a
+//     cached kernel ends by writing its dirty lines back, which streaming stores avoid;
+//   - 22-24 qubits (64-256 MiB): cached wins, 24.2 / 41.3 / 79.1 us against 25.4 / 48.0 / 93.7,
+//     because the MALL keeps the state from one pass to the next;
+//   - 25-26 qubits (512 MiB, 1 GiB): every pass streams from HBM and the hint wins, 185 / 380 us
+//     against 197 / 408.
+constexpr uint64_t kL2Bytes = 32ull << 20;
+constexpr uint64_t kLastLevelCacheBytes = 256ull << 20;
+
+static bool streams(uint64_t bytes) {
+  if (g_dense_streaming == 2) return true;
+  return g_dense_streaming == 1 && (bytes <= kL2Bytes || bytes > kLastLevelCacheBytes);
+}
 
 // loads and stores with the streaming (non-temporal) hint, or plain ones
 template <bool NT, typename T>
@@ -36,9 +53,26 @@ __device__ __forceinline__ void st_(T* p, T v) {
     *p = v;
 }
 
-// NT: streaming (non-temporal) loads and stores, chosen when every target is at bit 3 or above:
-// each load instruction then covers whole 128-byte lines, and keeping them in cache gains
-// nothing (lower targets rely on the cache for the partner loads of the same line).
+// a complex value read or written as its two components, with or without the streaming hint
+template <bool NT, typename V>
+__device__ __forceinline__ V ldc(const V* p) {
+  const auto* q = reinterpret_cast<const decltype(p->x)*>(p);
+  V v;
+  v.x = ld<NT>(q);
+  v.y = ld<NT>(q + 1);
+  return v;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void stc(V* p, V v) {
+  auto* q = reinterpret_cast<decltype(p->x)*>(p);
+  st_<NT>(q, v.x);
+  st_<NT>(q + 1, v.y);
+}
+
+// NT: streaming (non-temporal) loads and stores, chosen (for the sizes streams() picks) when
+// every target is at bit 3 or above: each load instruction then covers whole 128-byte lines, and
+// keeping them in cache gains nothing (lower targets rely on the cache for the partner loads of
+// the same line).
 template <int K, typename R, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_dense(typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t groups,
@@ -260,7 +294,7 @@ __global__ void __launch_bounds__(64)
 
 // Diagonal operator: every amplitude is scaled by the diagonal entry its target bits select.
 // The (at most 32-entry) diagonal is staged in LDS; four amplitudes per thread.
-template <typename R>
+template <typename R, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_diag(typename Cx<R>::V* __restrict__ states, uint32_t nbits, int k,
            const typename Cx<R>::V* __restrict__ diags, const int* __restrict__ sel,
@@ -286,12 +320,12 @@ __global__ void __launch_bounds__(kBlock)
     if (i < size) {
       uint32_t m = 0;
       for (int q = 0; q < k; ++q) m |= (uint32_t)((i >> t.listed[q]) & 1u) << (k - 1 - q);
-      st[i] = cmul(d[m], st[i]);
+      stc<NT>(st + i, cmul(d[m], ldc<NT>(st + i)));
     }
   }
 }
 
-template <typename R>
+template <typename R, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_scale(typename Cx<R>::V* __restrict__ states, uint64_t total, double re, double im) {
   using V = typename Cx<R>::V;
@@ -302,7 +336,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const uint64_t i = first + (uint64_t)j * kBlock;
-    if (i < total) states[i] = cmul(c, states[i]);
+    if (i < total) stc<NT>(states + i, cmul(c, ldc<NT>(states + i)));
   }
 }
 
@@ -393,7 +427,8 @@ static int launch_dense_mfma(void* states, int nbits, long long batch, int k, co
   // two steps per iteration; enough wavefronts to fill 256 CUs several times over, then stride
   const uint32_t waves = batches / 2 < 8192u ? batches / 2 : 8192u;
   const uint32_t gx = (waves + 3) / 4;
-  // Streaming accesses when each load instruction covers whole 128-byte lines (index bits 0-2):
+  // Streaming accesses (sizes as for streams()) when each load instruction covers whole 128-byte
+  // lines (index bits 0-2):
   // an instruction spans the 4 lowest non-target bits (16 consecutive groups) and the two targets
   // the lanes' k selects (listed[k-2], listed[k-1]), so every target below bit 3 must be one of
   // those two.
@@ -401,7 +436,7 @@ static int launch_dense_mfma(void* states, int nbits, long long batch, int k, co
   for (int j = 0; j < k; ++j)
     if (t.listed[j] < 3) low_targets |= 1u << t.listed[j];
   const uint32_t lane_targets = (1u << t.listed[k - 2]) | (1u << t.listed[k - 1]);
-  const bool nt = g_dense_streaming && (low_targets & ~lane_targets) == 0;
+  const bool nt = streams(((uint64_t)batch << nbits) * 16) && (low_targets & ~lane_targets) == 0;
   for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
     const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
     const dim3 grid(gx, gy), block(kBlock);
@@ -432,7 +467,7 @@ static int launch_dense(void* states, int nbits, long long batch, int k, const T
     dim3 grid(gx, gy), block(kBlock);
     V* st = (V*)states;
     const V* m = (const V*)mats;
-    const bool nt = g_dense_streaming && t.sorted[0] >= 3;
+    const bool nt = streams(((uint64_t)batch << nbits) * sizeof(V)) && t.sorted[0] >= 3;
 #define LSIM_DENSE(KK)                                                                     \
   do {                                                                                     \
     if (nt)                                                                                \
@@ -487,7 +522,7 @@ int lsim_set_dense_mfma(int widths) {
 
 int lsim_set_dense_streaming(int enabled) {
   const int old = g_dense_streaming;
-  if (enabled >= 0) g_dense_streaming = enabled != 0;
+  if (enabled >= 0) g_dense_streaming = enabled > 2 ? 2 : enabled;
   return old;
 }
 
@@ -535,15 +570,27 @@ int lsim_sv_apply_diag(void* states, int dtype, int nbits, long long batch, int 
   Targets t;
   if (int rc = make_targets(nbits, k, targets, &t)) return rc;
   const uint64_t size = 1ull << nbits;
+  const bool nt = streams(((uint64_t)batch << nbits) * dtype_bytes(dtype));
   for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
     const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
     dim3 grid(blocks_for(size), gy);
-    if (dtype == LSIM_C128)
-      k_diag<double><<<grid, kBlock, 0, as_stream(stream)>>>(
-          (double2*)states, nbits, k, (const double2*)diags, sel, (uint32_t)b0, t);
-    else
-      k_diag<float><<<grid, kBlock, 0, as_stream(stream)>>>(
-          (float2*)states, nbits, k, (const float2*)diags, sel, (uint32_t)b0, t);
+    hipStream_t s = as_stream(stream);
+    const uint32_t bb = (uint32_t)b0;
+    if (dtype == LSIM_C128) {
+      double2* st = (double2*)states;
+      const double2* dg = (const double2*)diags;
+      if (nt)
+        k_diag<double, true><<<grid, kBlock, 0, s>>>(st, nbits, k, dg, sel, bb, t);
+      else
+        k_diag<double, false><<<grid, kBlock, 0, s>>>(st, nbits, k, dg, sel, bb, t);
+    } else {
+      float2* st = (float2*)states;
+      const float2* dg = (const float2*)diags;
+      if (nt)
+        k_diag<float, true><<<grid, kBlock, 0, s>>>(st, nbits, k, dg, sel, bb, t);
+      else
+        k_diag<float, false><<<grid, kBlock, 0, s>>>(st, nbits, k, dg, sel, bb, t);
+    }
   }
   return check_launch("diagonal operator");
 }
@@ -552,12 +599,20 @@ int lsim_sv_scale(void* states, int dtype, int nbits, long long batch, double re
                   void* stream) {
   if (int rc = check_state(states, dtype, nbits, batch)) return rc;
   const uint64_t total = (uint64_t)batch << nbits;
-  if (dtype == LSIM_C128)
-    k_scale<double><<<blocks_for(total), kBlock, 0, as_stream(stream)>>>((double2*)states, total,
-                                                                        re, im);
-  else
-    k_scale<float><<<blocks_for(total), kBlock, 0, as_stream(stream)>>>((float2*)states, total, re,
-                                                                       im);
+  const bool nt = streams(total * dtype_bytes(dtype));
+  const dim3 grid(blocks_for(total));
+  hipStream_t s = as_stream(stream);
+  if (dtype == LSIM_C128) {
+    if (nt)
+      k_scale<double, true><<<grid, kBlock, 0, s>>>((double2*)states, total, re, im);
+    else
+      k_scale<double, false><<<grid, kBlock, 0, s>>>((double2*)states, total, re, im);
+  } else {
+    if (nt)
+      k_scale<float, true><<<grid, kBlock, 0, s>>>((float2*)states, total, re, im);
+    else
+      k_scale<float, false><<<grid, kBlock, 0, s>>>((float2*)states, total, re, im);
+  }
   return check_launch("scale");
 }
 

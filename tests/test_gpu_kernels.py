@@ -112,6 +112,56 @@ def test_diagonal(dtype: int) -> None:
             assert np.abs(got[b] - want).max() < TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+def test_streaming_modes_of_diagonal_and_scale(dtype: int) -> None:
+    """Every streaming mode (0 never, 1 by size, 2 always) gives bit-identical diagonal and scale
+    results, equal to the oracle's."""
+    lib = nat.load()
+    n, batch = 9, 3
+    psi = states(n, batch, 4)
+    d = np.exp(1j * np.random.default_rng(1).uniform(0, 6.28, size=(2, 8)))
+    targets = (6, 1, 4)
+    old = lib.lsim_set_dense_streaming(-1)
+    outs = []
+    try:
+        for mode in (0, 1, 2):
+            assert lib.lsim_set_dense_streaming(mode) in (0, 1, 2) and lib.lsim_set_dense_streaming(-1) == mode
+            s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+            dd = dev(d.astype(NPDT[dtype]))
+            sel = dev(np.array([1, -1, 0], dtype=np.int32))
+            nat.call("lsim_sv_apply_diag", ptr(s), dtype, n, batch, 3, nat.int_array(targets), ptr(dd), ptr(sel), stream_handle())
+            nat.call("lsim_sv_scale", ptr(s), dtype, n, batch, 0.6, 0.8, stream_handle())
+            outs.append(s.cpu().numpy().reshape(batch, -1))
+    finally:
+        lib.lsim_set_dense_streaming(old)
+    for b, m in enumerate([1, -1, 0]):
+        want = (psi[b] if m < 0 else orc.apply_sv(psi[b], np.diag(d[m]), targets, n)) * (0.6 + 0.8j)
+        assert np.abs(outs[0][b] - want).max() < TOL[dtype] * 2
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_scale_of_a_state_beyond_the_last_level_cache() -> None:
+    """A 25-qubit complex128 state (512 MiB) takes the streaming scale kernel under the default
+    mode; it multiplies every amplitude exactly as the cached kernel does."""
+    lib = nat.load()
+    n = 25
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(1 << n, dtype=torch.complex128, device="cuda", generator=g)
+    old = lib.lsim_set_dense_streaming(-1)
+    try:
+        got = []
+        for mode in (1, 0):
+            lib.lsim_set_dense_streaming(mode)
+            y = x.clone()
+            nat.call("lsim_sv_scale", ptr(y), nat.C128, n, 1, 0.6, -0.8, stream_handle())
+            got.append(y)
+    finally:
+        lib.lsim_set_dense_streaming(old)
+    torch.cuda.synchronize()
+    assert torch.equal(got[0], got[1])
+    assert float((got[0] - x * (0.6 - 0.8j)).abs().max()) < 1e-14
+
+
 def test_scale_basis_probabilities_axpy() -> None:
     n, batch = 10, 2
     s = torch.empty(batch << n, dtype=torch.complex128, device="cuda")
@@ -260,7 +310,7 @@ def test_dense_streaming_and_matrix_core_variants_agree(k: int, dtype: int, extr
             want2 = orc.apply_sv(psi[2], u, targets, n)
             outs = []
             for mask in (0, 1 << k):
-                for streaming in (0, 1):
+                for streaming in (0, 2):
                     lib.lsim_set_dense_mfma(mask)
                     lib.lsim_set_dense_streaming(streaming)
                     assert lib.lsim_set_dense_mfma(-1) == mask and lib.lsim_set_dense_streaming(-1) == streaming
