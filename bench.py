@@ -312,7 +312,7 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=0, help="fusion width (0: run the autotuner and use its choice)")
     ap.add_argument("--tiling", type=int, choices=(0, 1), default=None, help="force the LDS-tiled dense kernel off/on")
     ap.add_argument("--mfma", type=int, default=None, help="mask of operator widths on the matrix cores (bit k: k qubits)")
-    ap.add_argument("--streaming", type=int, choices=(0, 1), default=None, help="non-temporal accesses for high placements off/on")
+    ap.add_argument("--streaming", type=int, choices=(0, 1, 2), default=None, help="non-temporal accesses: 0 never, 1 by state size, 2 always")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and trajectory legs")
     ap.add_argument(
@@ -469,7 +469,7 @@ def main() -> int:
             "fusion_width": width,
             "low_qubit_width": low_width or None,
             "matrix_core_widths": [k for k in (3, 4, 5) if (nat.load().lsim_set_dense_mfma(-1) >> k) & 1],
-            "streaming": bool(nat.load().lsim_set_dense_streaming(-1)),
+            "streaming_mode": int(nat.load().lsim_set_dense_streaming(-1)),
             "passes": len(passes),
             "dense_passes": dense,
             "bytes_per_pass": BYTES_PER_PASS,
