@@ -232,6 +232,53 @@ def bench_serving(qps: float, count: int = 48, seed: int = 1005) -> dict:
     }
 
 
+def bench_c2_outputs() -> dict:
+    """The 26-qubit configuration's outputs through the public API: ``probabilities(src)`` (one
+    simulation, |psi|^2 on the device, 512 MiB to the host, then the reference's dict of every
+    outcome above 1e-12) and ``expectation(src, P_j)`` of its 8 Pauli strings, one call each (each
+    call simulates the circuit again, as the reference's does), plus all 8 as one weighted
+    observable. Checks: the probabilities sum to 1, and the weighted call equals the weighted sum
+    of the single calls."""
+    import numpy as np
+
+    import lucivant_sim as ls
+    from lucivant_sim import workloads
+    from lucivant_sim.simulator import _exact_state
+
+    src, labels = workloads.config_c2()
+    circuit = ls.parse(src)
+    t0 = time.perf_counter()
+    probs = _exact_state(circuit, None).probabilities()
+    device_s = time.perf_counter() - t0
+    total = float(np.sum(probs))
+    del probs
+    t0 = time.perf_counter()
+    table = ls.probabilities(src)
+    api_s = time.perf_counter() - t0
+    entries = len(table)
+    del table
+    t0 = time.perf_counter()
+    values = [ls.expectation(src, label) for label in labels]
+    each_s = time.perf_counter() - t0
+    weights = np.random.default_rng(1003).uniform(-1, 1, len(labels))
+    t0 = time.perf_counter()
+    combined = ls.expectation(src, dict(zip(labels, weights.tolist())))
+    one_s = time.perf_counter() - t0
+    linear = abs(combined - float(np.dot(weights, values))) < 1e-10
+    return {
+        "workload": "26-qubit configuration (40 layers u3 + cx/cz, seed 1002): probabilities() and expectation() of 8 Pauli strings (seed 1003)",
+        "probabilities_api_s": round(api_s, 3),
+        "probabilities_device_s": round(device_s, 3),
+        "probabilities_note": "device_s = simulation, |psi|^2 and the copy to host memory; the rest of api_s is the reference's Python dict of every outcome above 1e-12",
+        "probabilities_entries": entries,
+        "probability_sum_minus_1": float(f"{total - 1:.3g}"),
+        "expectation_8_calls_s": round(each_s, 3),
+        "expectation_one_weighted_call_s": round(one_s, 4),
+        "expectation_values": [round(v, 12) for v in values],
+        "weighted_equals_sum_of_calls": linear,
+    }
+
+
 def bench_trajectories(shots: int = 4096) -> dict:
     """The 20-qubit trajectory configuration (teleport chain, 6 mid-circuit measurements,
     uniform depolarising 1e-3/1e-2 and amplitude damping 0.01 on h) in complex128 and complex64."""
@@ -496,6 +543,7 @@ def main() -> int:
         out["batched"] = bench_batched()
         out["batched_shots"] = bench_batched_shots()
         out["serving"] = [bench_serving(q) for q in (5.0, 20.0, 1000.0)]
+        out["outputs_26q"] = bench_c2_outputs()
         out["trajectories_20q"] = bench_trajectories()
         out["density_matrix_13q"] = bench_density_matrix()
     if not args.no_cpu:

@@ -470,9 +470,22 @@ def probabilities(
     circuit = to_circuit(program, params)
     _static(circuit, "exact probabilities")
     probs = _exact_state(circuit, noise).probabilities()
-    n = circuit.num_qubits
+    return _basis_dict(probs, circuit.num_qubits, threshold)
+
+
+def _basis_dict(probs: NDArray[np.float64], n: int, threshold: float) -> dict[str, float]:
+    """``{format(i, f"0{n}b"): probs[i] for every i with probs[i] > threshold}`` in index order
+    (simulator.py:440-442). The keys are joined from two tables of half-width bit strings, which
+    builds a 2^26-entry dict in about half the time of formatting every index."""
     keep = np.flatnonzero(probs > threshold)
-    return {format(int(i), f"0{n}b"): float(probs[i]) for i in keep}
+    values = probs[keep].tolist()
+    if n <= 12:
+        return {format(int(i), f"0{n}b"): v for i, v in zip(keep.tolist(), values)}
+    low = n // 2
+    lo = [format(i, f"0{low}b") for i in range(1 << low)]
+    hi = [format(i, f"0{n - low}b") for i in range(1 << (n - low))]
+    keys = map(str.__add__, map(hi.__getitem__, (keep >> low).tolist()), map(lo.__getitem__, (keep & ((1 << low) - 1)).tolist()))
+    return dict(zip(keys, values))
 
 
 def outcome_probabilities(

@@ -348,6 +348,23 @@ def test_large_distributions_sample_as_the_oracle_with_a_reused_buffer() -> None
         assert got == want, n
 
 
+def test_basis_dict_equals_formatting_every_index() -> None:
+    """probabilities() keys: the half-width tables give the same dict, in the same order, as
+    formatting each kept index (simulator.py:440-442), with zero plateaus and exact-threshold
+    entries dropped."""
+    from lucivant_sim.simulator import _basis_dict
+
+    rng = np.random.default_rng(2)
+    for n in (1, 5, 12, 13, 17):
+        p = rng.random(1 << n)
+        p[rng.integers(0, 1 << n, size=max(1, (1 << n) // 7))] = 0.0
+        p[0] = 1e-12
+        p /= p.sum()
+        want = {format(int(i), f"0{n}b"): float(p[i]) for i in np.flatnonzero(p > 1e-12)}
+        got = _basis_dict(p, n, 1e-12)
+        assert got == want and list(got) == list(want), n
+
+
 def test_search_over_normalised_cumulative_sums_without_dividing_all() -> None:
     """The sampler's binary search divides only the entries it probes, and lands where
     searchsorted over the divided array does: with zero-probability plateaus, exact ties and draws
