@@ -30,10 +30,14 @@ from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Barrier, Circuit, Condition, Gate, GlobalPhase, IfElse, Measure, Operation, Reset
 from lucivant_sim.engines import WORKSPACE, Statevector, _NP_DTYPE, _TORCH_DTYPE, ptr, require_device, run_passes, stream_handle, upload
 from lucivant_sim.noise import Channel, NoiseModel, ReadoutError
-from lucivant_sim.plan import plan_statevector
+from lucivant_sim.plan import Pass, _as_pass, embed
 
 #: HBM the batch of trajectory states may take (bytes); shots beyond it run in further chunks.
 BATCH_BYTES = int(os.environ.get("LSIM_TRAJECTORY_BYTES", str(64 << 30)))
+
+#: Widest block of queued operators fused into one pass over the batch (the vector kernel's
+#: scalar-operand limit; a per-shot operator table then holds 2^4 x 2^4 matrices).
+FUSE_WIDTH = 4
 
 
 # ------------------------------------------------------------------------------- draw ledger
@@ -121,31 +125,130 @@ def compile_program(circuit: Circuit, noise: NoiseModel | None) -> tuple[list[_S
 # ------------------------------------------------------------------------------- execution
 
 
+@dataclass
+class _Queued:
+    """An operator waiting to be fused: ``table[idx[b]]`` for shot b (nothing where idx[b] < 0),
+    or ``table[0]`` for every shot when ``idx`` is None. Projections and Kraus operators are
+    queued unnormalised (``unitary`` False)."""
+
+    qubits: tuple[int, ...]
+    table: list[np.ndarray]
+    idx: np.ndarray | None = None
+    unitary: bool = True
+
+
+def _group(items: list[_Queued], width: int) -> list[tuple[list[int], list[_Queued]]]:
+    """Consecutive queued operators grouped into blocks of at most ``width`` qubits: each joins
+    the latest block that touches one of its qubits when the union stays within ``width`` (blocks
+    that touch none of its qubits commute with it), else it opens a new block."""
+    blocks: list[tuple[list[int], list[_Queued]]] = []
+    last: dict[int, int] = {}
+    for it in items:
+        j = max((last.get(q, -1) for q in it.qubits), default=-1)
+        if j >= 0 and len(set(blocks[j][0]) | set(it.qubits)) <= width:
+            blocks[j][0].extend(q for q in it.qubits if q not in blocks[j][0])
+            blocks[j][1].append(it)
+        else:
+            blocks.append((list(it.qubits), [it]))
+            j = len(blocks) - 1
+        for q in it.qubits:
+            last[q] = j
+    return blocks
+
+
+def _fuse(qubits: list[int], items: list[_Queued], combos: np.ndarray) -> np.ndarray:
+    """The block's operator for each combination of per-shot choices (a row of ``combos`` holds
+    one table index per per-shot item, in order; -1 is the identity): the product of the items in
+    program order, embedded on ``qubits`` (first = most significant matrix bit)."""
+    k = len(qubits)
+    d = 1 << k
+    out = np.broadcast_to(np.eye(d, dtype=np.complex128), (len(combos), d, d)).copy()
+    column = 0
+    for it in items:
+        pos = [qubits.index(q) for q in it.qubits]
+        if it.idx is None:
+            out = embed(it.table[0], pos, k) @ out
+            continue
+        choices = np.stack([np.eye(d, dtype=np.complex128)] + [embed(m, pos, k) for m in it.table])
+        out = choices[combos[:, column] + 1] @ out
+        column += 1
+    return out
+
+
+def _descending_many(mats: np.ndarray, bits: tuple[int, ...]) -> tuple[np.ndarray, tuple[int, ...]]:
+    """:func:`plan.descending` for a stack of operators on the same targets."""
+    k = len(bits)
+    order = sorted(range(k), key=lambda i: -bits[i])
+    if order == list(range(k)):
+        return np.ascontiguousarray(mats), bits
+    idx = np.arange(1 << k)
+    old = np.zeros(1 << k, dtype=np.int64)
+    for j, i in enumerate(order):
+        old |= ((idx >> (k - 1 - j)) & 1) << (k - 1 - i)
+    return np.ascontiguousarray(mats[:, old][:, :, old]), tuple(bits[i] for i in order)
+
+
+def _partial(rho: np.ndarray, qubits: tuple[int, ...], keep: tuple[int, ...]) -> np.ndarray:
+    """Per-shot reduced density matrices on ``keep`` (a subset of ``qubits``, in its own order)
+    from those on ``qubits``; first listed = most significant index bit in both."""
+    if keep == qubits:
+        return rho
+    k = len(qubits)
+    letters = "abcdefgh"
+    rows, cols = list(letters[:k]), list(letters[k : 2 * k])
+    for i, q in enumerate(qubits):
+        if q not in keep:
+            cols[i] = rows[i]  # traced out
+    out_rows = "".join(rows[qubits.index(q)] for q in keep)
+    out_cols = "".join(cols[qubits.index(q)] for q in keep)
+    t = rho.reshape((rho.shape[0],) + (2,) * (2 * k))
+    r = np.einsum(f"z{''.join(rows)}{''.join(cols)}->z{out_rows}{out_cols}", t)
+    d = 1 << len(keep)
+    return r.reshape(rho.shape[0], d, d)
+
+
+#: A shot's unnormalised state is rescaled once its squared norm falls below this (dtype: floor).
+_RENORMALISE_BELOW = {nat.C128: 1e-150, nat.C64: 1e-8}
+
+
 class _Batch:
-    """B states of n qubits in HBM plus their classical bits and if/else flags."""
+    """B states of n qubits in HBM plus their classical bits and if/else flags.
+
+    Every operator is queued and fused, up to :data:`FUSE_WIDTH` qubits at a time, into one pass
+    over the batch: each shot gets the product of its own choices, from a table holding one fused
+    operator per distinct combination of choices in the batch. A choice is a mixture branch (from
+    the draws), an if/else arm (from the flags), a measurement's projection or a Kraus operator
+    (from the draws and the reduced density matrix). Projections and Kraus operators are applied
+    without renormalising: each decision reads the reduced density matrix divided by its trace,
+    which is the squared norm of the shot's state.
+
+    A decision needs the reduced density matrix of its qubits. It is computed on the device over
+    up to two qubits -- those of the decision and of the next ones, chosen ahead so that the
+    operators in between stay inside the set or miss it -- and then kept on the host, where each
+    queued operator inside the set transforms it, so the following decisions read it without
+    another pass. An operator that straddles the set, or a projection or Kraus operator outside it,
+    drops it."""
 
     def __init__(self, n: int, batch: int, dtype: int, nflags: int, num_clbits: int) -> None:
         self.n, self.b, self.dtype = n, batch, dtype
-        self.buf = torch.empty(batch << n, dtype=_TORCH_DTYPE[dtype], device=require_device())
-        nat.call("lsim_sv_set_basis", ptr(self.buf), dtype, n, batch, 0, stream_handle())
+        self._allocate()
         self.wide = num_clbits > 62
         self.clbits = np.zeros(batch, dtype=object if self.wide else np.int64)
         self.flags = np.zeros((max(nflags, 1), batch), dtype=bool)
-        self.pending: list[Gate] = []
+        self.pending: list[_Queued] = []
+        self.known: tuple[tuple[int, ...], np.ndarray] | None = None  # (qubits, unnormalised rho)
 
-    def active(self, guard: tuple[tuple[int, bool], ...]) -> np.ndarray | None:
-        if not guard:
-            return None
-        mask = np.ones(self.b, dtype=bool)
-        for flag, polarity in guard:
-            mask &= self.flags[flag] == polarity
-        return mask
+    # -- device operations -------------------------------------------------------------------
 
-    def flush(self) -> None:
-        if self.pending:
-            passes = plan_statevector(tuple(self.pending), self.n, 4)
-            run_passes(self.buf, self.dtype, self.n, self.b, passes)
-            self.pending = []
+    def _allocate(self) -> None:
+        self.buf = torch.empty(self.b << self.n, dtype=_TORCH_DTYPE[self.dtype], device=require_device())
+        nat.call("lsim_sv_set_basis", ptr(self.buf), self.dtype, self.n, self.b, 0, stream_handle())
+
+    def run_pass(self, p: Pass) -> None:
+        run_passes(self.buf, self.dtype, self.n, self.b, [p])
+
+    def finish(self) -> None:
+        torch.cuda.current_stream().synchronize()
 
     def apply_selected(self, kind: str, qubits: tuple[int, ...], table: np.ndarray, sel: np.ndarray) -> None:
         """Per-state operator: state b gets table[sel[b]] (dense matrices or diagonals), or is
@@ -168,6 +271,93 @@ class _Batch:
             "lsim_sv_reduced_dm", ptr(self.buf), self.dtype, self.n, self.b, k, nat.int_array(qubits), ptr(out), ptr(ws), stream_handle()
         )
         return out.cpu().numpy().reshape(self.b, 1 << k, 1 << k)
+
+    # -- the queue ---------------------------------------------------------------------------
+
+    def active(self, guard: tuple[tuple[int, bool], ...]) -> np.ndarray | None:
+        if not guard:
+            return None
+        mask = np.ones(self.b, dtype=bool)
+        for flag, polarity in guard:
+            mask &= self.flags[flag] == polarity
+        return mask
+
+    def queue(self, qubits: tuple[int, ...], table: list[np.ndarray], idx: np.ndarray | None = None, unitary: bool = True) -> None:
+        if idx is not None and not np.any(idx >= 0):
+            return
+        it = _Queued(qubits, table, idx, unitary)
+        self.pending.append(it)
+        self._follow(it)
+
+    def _follow(self, it: _Queued) -> None:
+        """Carry the kept reduced density matrix through a queued operator, or drop it."""
+        if self.known is None:
+            return
+        kept, rho = self.known
+        span = set(it.qubits)
+        if not span <= set(kept):
+            if not (span.isdisjoint(kept) and it.unitary):
+                self.known = None
+            return
+        pos = [kept.index(q) for q in it.qubits]
+        k = len(kept)
+        ops = [embed(m, pos, k) for m in it.table]
+        if it.idx is None:
+            rho = ops[0] @ rho @ ops[0].conj().T
+        else:
+            rho = rho.copy()
+            for j, e in enumerate(ops):
+                pick = it.idx == j
+                if np.any(pick):
+                    rho[pick] = e @ rho[pick] @ e.conj().T
+        self.known = (kept, rho)
+
+    def flush(self) -> None:
+        pending, self.pending = self.pending, []
+        for qubits, items in _group(pending, FUSE_WIDTH):
+            per_shot = [it for it in items if it.idx is not None]
+            if not per_shot:
+                op = _fuse(qubits, items, np.zeros((1, 0), dtype=np.int64))[0]
+                self.run_pass(_as_pass(op, tuple(qubits), len(items)))
+                continue
+            keys = np.stack([it.idx for it in per_shot], axis=1)
+            combos, inverse = np.unique(keys, axis=0, return_inverse=True)
+            sel = np.asarray(inverse, dtype=np.int64).reshape(-1)
+            if len(per_shot) == len(items):
+                # shots whose choices are all the identity are left alone
+                sel = np.where(np.all(combos < 0, axis=1)[sel], -1, sel)
+            mats, bits = _descending_many(_fuse(qubits, items, combos), tuple(qubits))
+            self.apply_selected("dense", bits, mats, sel)
+
+    def density(self, qubits: tuple[int, ...], ahead: tuple[int, ...]) -> np.ndarray:
+        """Unnormalised per-shot reduced density matrices on ``qubits`` of the state with every
+        queued operator applied; ``ahead`` (containing ``qubits``) is the set to compute on the
+        device when the kept matrices do not cover ``qubits``."""
+        if self.known is not None and set(qubits) <= set(self.known[0]):
+            return _partial(self.known[1], self.known[0], qubits)
+        # operators at the end of the queue that act inside the set wait for the next pass:
+        # the host carries the reduced density matrix through them instead
+        tail: list[_Queued] = []
+        while self.pending and set(self.pending[-1].qubits) <= set(ahead):
+            tail.insert(0, self.pending.pop())
+        self.flush()
+        rho = self.reduced(ahead)
+        trace = np.trace(rho, axis1=1, axis2=2).real
+        low = trace < _RENORMALISE_BELOW[self.dtype]
+        if np.any(low & (trace > 0)) and self.n:
+            # keep unnormalised states away from underflow: rescale those shots to norm 1
+            fix = low & (trace > 0)
+            s = np.where(fix, 1.0 / np.sqrt(np.where(fix, trace, 1.0)), 1.0)
+            self.apply_selected("diag", (0,), np.stack([s, s], axis=1).astype(np.complex128), np.where(fix, np.arange(self.b), -1))
+            rho = rho * np.where(fix, 1.0 / np.where(fix, trace, 1.0), 1.0)[:, None, None]
+        self.known = (ahead, rho)
+        for it in tail:
+            self.pending.append(it)
+            self._follow(it)
+        assert self.known is not None
+        return _partial(self.known[1], self.known[0], qubits)
+
+    # -- classical bits ----------------------------------------------------------------------
 
     def set_bits(self, clbit: int, outcome: np.ndarray, mask: np.ndarray | None) -> None:
         if self.wide:
@@ -209,101 +399,109 @@ def _mixture_choice(channel: Channel, r: np.ndarray) -> tuple[np.ndarray, list[n
     return lookup[branch], table
 
 
-def _kraus_choice(channel: Channel, rho: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """Per-shot normalised Kraus operator (engines.py:171-183): the first i with r < sum of
-    weights so far and w_i > 0, else the last one if its weight is positive, else none."""
+def _kraus_index(channel: Channel, rho: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Per-shot Kraus operator index (engines.py:171-183) given normalised reduced density
+    matrices: the first i with r < sum of weights so far and w_i > 0, else the last one if its
+    weight is positive, else -1 (the state is left alone). Also returns the chosen weights."""
     nshots = r.shape[0]
-    d = channel.kraus[0].shape[0]
-    mats = np.zeros((nshots, d, d), dtype=np.complex128)
-    sel = np.full(nshots, -1, dtype=np.int64)
-    chosen = np.zeros(nshots, dtype=bool)
+    idx = np.full(nshots, -1, dtype=np.int64)
+    chosen_w = np.zeros(nshots)
     acc = np.zeros(nshots)
     weight = np.zeros(nshots)
-    for kraus in channel.kraus:
+    for i, kraus in enumerate(channel.kraus):
         weight = np.einsum("ra,sac,rc->s", kraus, rho, kraus.conj()).real
         acc = acc + weight
-        pick = (~chosen) & (r < acc) & (weight > 0)
-        if np.any(pick):
-            mats[pick] = kraus[None] / np.sqrt(weight[pick])[:, None, None]
-            sel[pick] = np.flatnonzero(pick)
-            chosen |= pick
-    rest = (~chosen) & (weight > 0)
-    if np.any(rest):
-        last = channel.kraus[-1]
-        mats[rest] = last[None] / np.sqrt(weight[rest])[:, None, None]
-        sel[rest] = np.flatnonzero(rest)
-    return mats, sel
+        pick = (idx < 0) & (r < acc) & (weight > 0)
+        idx[pick] = i
+        chosen_w[pick] = weight[pick]
+    rest = (idx < 0) & (weight > 0)
+    idx[rest] = len(channel.kraus) - 1
+    chosen_w[rest] = weight[rest]
+    return idx, chosen_w
 
 
-def _collapse_factors(p1: np.ndarray, outcome: np.ndarray) -> np.ndarray:
-    """1/sqrt(p) of the observed branch (1 where p = 0, as engines.py:195-201 skips the divide)."""
-    p = np.where(outcome == 1, p1, 1 - p1)
-    safe = np.where(p > 0, p, 1.0)
-    return np.where(p > 0, 1.0 / np.sqrt(safe), 1.0)
+def _kraus_choice(channel: Channel, rho: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Per-shot normalised Kraus operator K_i / sqrt(w_i) for :func:`_kraus_index`'s choice
+    (``sel[b] = b``), or ``sel[b] = -1`` when no operator applies."""
+    idx, w = _kraus_index(channel, rho, r)
+    d = channel.kraus[0].shape[0]
+    mats = np.zeros((r.shape[0], d, d), dtype=np.complex128)
+    for i, kraus in enumerate(channel.kraus):
+        pick = idx == i
+        mats[pick] = kraus[None] / np.sqrt(w[pick])[:, None, None]
+    return mats, np.where(idx >= 0, np.arange(r.shape[0]), -1)
 
 
-def _run_chunk(steps: list[_Step], circuit: Circuit, draws: np.ndarray, dtype: int, nflags: int) -> np.ndarray:
-    nshots = draws.shape[0]
-    st = _Batch(circuit.num_qubits, nshots, dtype, nflags, circuit.num_clbits)
-    everyone = np.arange(nshots)
-    for step in steps:
-        mask = st.active(step.guard)
-        if step.kind == "gate" and mask is None:
-            st.pending.append(Gate("fused", step.qubits, step.matrix))  # type: ignore[arg-type]
+def _reads(step: _Step) -> bool:
+    """True for the steps whose decision reads the state."""
+    return step.kind in ("measure", "reset") or (step.kind == "channel" and step.channel is not None and step.channel.mixture is None)
+
+
+def _ahead(steps: list[_Step], t: int) -> tuple[int, ...]:
+    """Qubits to reduce over at reading step t: its own, then those of the next reading steps
+    while the set holds at most two qubits and every operator in between either stays inside the
+    set or misses it (global phases and if/else conditions touch no qubit)."""
+    chosen = list(steps[t].qubits)
+    between: list[set[int]] = []
+    for s in steps[t + 1 :]:
+        if not _reads(s):
+            if s.kind in ("gate", "channel"):
+                between.append(set(s.qubits))
             continue
-        st.flush()
+        union = set(chosen) | set(s.qubits)
+        if len(union) > 2 or any(not (b <= union or b.isdisjoint(union)) for b in between):
+            break
+        chosen += [q for q in s.qubits if q not in chosen]
+    return tuple(chosen)
+
+
+_PROJECT = [np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 0], [0, 1]], dtype=np.complex128)]
+_RESET = [np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 1], [0, 0]], dtype=np.complex128)]
+
+
+def _run_chunk(steps: list[_Step], circuit: Circuit, draws: np.ndarray, dtype: int, nflags: int, batch: _Batch | None = None) -> np.ndarray:
+    nshots = draws.shape[0]
+    st = batch if batch is not None else _Batch(circuit.num_qubits, nshots, dtype, nflags, circuit.num_clbits)
+    for t, step in enumerate(steps):
+        mask = st.active(step.guard)
         if step.kind == "gate":
-            st.apply_selected("dense", step.qubits, step.matrix[None], np.where(mask, 0, -1))  # type: ignore[index]
+            assert step.matrix is not None
+            st.queue(step.qubits, [step.matrix], None if mask is None else np.where(mask, 0, -1))
         elif step.kind == "phase":
-            z = np.exp(1j * step.angle)
-            d = np.array([[z, z]], dtype=np.complex128)
-            sel = np.zeros(nshots, dtype=np.int64) if mask is None else np.where(mask, 0, -1)
-            if circuit.num_qubits:
-                st.apply_selected("diag", (0,), d, sel)
-        elif step.kind == "channel":
-            ch = step.channel
-            assert ch is not None
+            continue  # a global phase changes no probability, and only bits leave the batch
+        elif step.kind == "cond":
+            c = step.condition
+            assert c is not None
+            holds = (st.clbit_values(c.clbits) == c.value) != c.negate
+            st.flags[step.flag] = holds.astype(bool)
+        elif step.kind == "channel" and step.channel is not None and step.channel.mixture is not None:
+            sel, table = _mixture_choice(step.channel, draws[:, step.column])
+            if table:
+                st.queue(step.qubits, table, sel if mask is None else np.where(mask, sel, -1))
+        else:
+            rho = st.density(step.qubits, _ahead(steps, t))
+            trace = np.trace(rho, axis1=1, axis2=2).real
+            norm = np.where(trace > 0, trace, 1.0)
             r = draws[:, step.column]
-            if ch.mixture is not None:
-                sel, table = _mixture_choice(ch, r)
-                if mask is not None:
-                    sel = np.where(mask, sel, -1)
-                if table:
-                    st.apply_selected("dense", step.qubits, np.stack(table), sel)
-            else:
-                mats, sel = _kraus_choice(ch, st.reduced(step.qubits), r)
-                if mask is not None:
-                    sel = np.where(mask, sel, -1)
-                st.apply_selected("dense", step.qubits, mats, sel)
-        elif step.kind in ("measure", "reset"):
-            q = step.qubits[0]
-            rho = st.reduced((q,))
-            p1 = np.minimum(np.maximum(rho[:, 1, 1].real, 0.0), 1.0)
-            outcome = (draws[:, step.column] < p1).astype(np.int64)
-            scale = _collapse_factors(p1, outcome)
-            sel = everyone if mask is None else np.where(mask, everyone, -1)
+            if step.kind == "channel":
+                ch = step.channel
+                assert ch is not None
+                idx, _ = _kraus_index(ch, rho / norm[:, None, None], r)
+                st.queue(step.qubits, list(ch.kraus), idx if mask is None else np.where(mask, idx, -1), unitary=False)
+                continue
+            p1 = np.minimum(np.maximum(rho[:, 1, 1].real / norm, 0.0), 1.0)
+            outcome = (r < p1).astype(np.int64)
+            table = _PROJECT if step.kind == "measure" else _RESET
+            st.queue(step.qubits, table, outcome if mask is None else np.where(mask, outcome, -1), unitary=False)
             if step.kind == "measure":
-                diags = np.zeros((nshots, 2), dtype=np.complex128)
-                diags[everyone, outcome] = scale
-                st.apply_selected("diag", (q,), diags, sel)
                 if step.readout is not None:
                     err = step.readout
                     flip = draws[:, step.column + 1] < np.where(outcome == 1, err.p10, err.p01)
                     outcome = outcome ^ flip.astype(np.int64)
                 if step.clbit is not None:
                     st.set_bits(step.clbit, outcome, mask)
-            else:
-                mats = np.zeros((nshots, 2, 2), dtype=np.complex128)
-                mats[:, 0, 0] = np.where(outcome == 0, scale, 0)
-                mats[:, 0, 1] = np.where(outcome == 1, scale, 0)
-                st.apply_selected("dense", (q,), mats, sel)
-        elif step.kind == "cond":
-            c = step.condition
-            assert c is not None
-            holds = (st.clbit_values(c.clbits) == c.value) != c.negate
-            st.flags[step.flag] = holds.astype(bool)
-    st.flush()
-    torch.cuda.current_stream().synchronize()
+    # operators still queued change no recorded bit: they are dropped
+    st.finish()
     return st.clbits
 
 

@@ -19,6 +19,7 @@ import lucivant_sim as ls
 from lucivant_sim import _native as nat
 from lucivant_sim import workloads as wl
 from lucivant_sim.trajectories import run_batched, run_sequential
+from dynamic_circuits import DYNAMIC, dynamic_noise
 from oracle import engine as orc
 
 pytestmark = pytest.mark.gpu
@@ -247,6 +248,19 @@ def test_batched_sequential_and_oracle_trajectories_agree_exactly() -> None:
     want = orc.trajectory_counts(circuit, noise, shots, np.random.default_rng(17))
     uniq, cnt = np.unique(batched, return_counts=True)
     assert {circuit.format_bits(int(v)): int(c) for v, c in zip(uniq, cnt)} == want
+
+
+def test_batched_trajectories_with_fused_queues_equal_the_sequential_order() -> None:
+    """On the shared dynamic circuit, the batch -- which fuses chosen operators into per-shot
+    blocks and carries reduced density matrices on the host -- gives every shot the bits the
+    reference's sequential order gives it."""
+    circuit = ls.parse(DYNAMIC)
+    noise = dynamic_noise()
+    shots = 120
+    batched = run_batched(circuit, noise, shots, np.random.default_rng(23))
+    sequential = run_sequential(circuit, noise, shots, np.random.default_rng(23))
+    assert [int(v) for v in batched] == sequential
+    assert len(set(sequential)) > 4
 
 
 def test_complex64_trajectories_stay_close_to_complex128() -> None:

@@ -365,6 +365,132 @@ def test_basis_dict_equals_formatting_every_index() -> None:
         assert got == want and list(got) == list(want), n
 
 
+def test_trajectory_queue_fuses_blocks_equal_to_program_order() -> None:
+    """The trajectory batch fuses queued operators into blocks of at most 4 qubits with one
+    operator per combination of per-shot choices. Applying the blocks in order, with each shot's
+    combination, equals applying every item in program order (the oracle's tensordot), for fixed
+    gates, mixture-like tables with identity choices (-1) and guarded gates."""
+    from lucivant_sim.trajectories import _fuse, _group, _Queued
+
+    rng = np.random.default_rng(9)
+    n, shots = 6, 40
+
+    def unitary(k: int) -> np.ndarray:
+        q, _ = np.linalg.qr(rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k)))
+        return q
+
+    items = []
+    for _ in range(30):
+        k = int(rng.integers(1, 3))
+        qubits = tuple(int(x) for x in rng.choice(n, size=k, replace=False))
+        kind = rng.integers(0, 3)
+        if kind == 0:
+            items.append(_Queued(qubits, [unitary(k)]))
+        else:
+            table = [unitary(k) for _ in range(int(rng.integers(1, 4)))]
+            idx = rng.integers(-1, len(table), size=shots)
+            items.append(_Queued(qubits, table, idx))
+    blocks = _group(items, 4)
+    assert all(len(q) <= 4 for q, _ in blocks)
+    assert sum(len(b) for _, b in blocks) == len(items)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    for b in range(0, shots, 7):
+        want = psi0.copy()
+        for it in items:
+            j = 0 if it.idx is None else int(it.idx[b])
+            if j >= 0:
+                want = orc.apply_sv(want, it.table[j], it.qubits, n)
+        got = psi0.copy()
+        for qubits, members in blocks:
+            per_shot = [it for it in members if it.idx is not None]
+            combo = np.array([[int(it.idx[b]) for it in per_shot]], dtype=np.int64).reshape(1, len(per_shot))
+            op = _fuse(qubits, members, combo)[0]
+            got = orc.apply_sv(got, op, tuple(qubits), n)
+        assert np.abs(got - want).max() < 1e-12, b
+
+
+class _HostBatch:
+    """The trajectory batch's four device operations restated on the host (the oracle's
+    tensordot for operators), so its queueing, fusion and kept reduced density matrices can be
+    checked against the reference's sequential rule on the CPU."""
+
+    @staticmethod
+    def make(n: int, nshots: int, nflags: int, num_clbits: int):  # type: ignore[no-untyped-def]
+        from lucivant_sim import trajectories as tr
+
+        class Batch(tr._Batch):
+            passes = 0
+            reductions = 0
+
+            def _allocate(self) -> None:
+                self.states = np.zeros((self.b, 1 << self.n), dtype=np.complex128)
+                self.states[:, 0] = 1.0
+
+            def run_pass(self, p) -> None:  # type: ignore[no-untyped-def]
+                Batch.passes += 1
+                m = np.diag(p.matrix) if p.kind == "diag" else p.matrix
+                for b in range(self.b):
+                    self.states[b] = orc.apply_sv(self.states[b], m, p.bits, self.n)
+
+            def apply_selected(self, kind, qubits, table, sel) -> None:  # type: ignore[no-untyped-def]
+                if not np.any(sel >= 0):
+                    return
+                Batch.passes += 1
+                for b in np.flatnonzero(sel >= 0):
+                    m = table[sel[b]]
+                    m = np.diag(m) if kind == "diag" else m
+                    self.states[b] = orc.apply_sv(self.states[b], m, tuple(qubits), self.n)
+
+            def reduced(self, qubits):  # type: ignore[no-untyped-def]
+                Batch.reductions += 1
+                k = len(qubits)
+                out = np.zeros((self.b, 1 << k, 1 << k), dtype=np.complex128)
+                axes = [self.n - 1 - q for q in qubits]
+                for b in range(self.b):
+                    t = np.moveaxis(self.states[b].reshape([2] * self.n), axes, list(range(k))).reshape(1 << k, -1)
+                    out[b] = t @ t.conj().T
+                return out
+
+            def finish(self) -> None:
+                pass
+
+        return Batch(n, nshots, 0, nflags, num_clbits)
+
+
+def _host_trajectory_counts(circuit, noise, shots: int, seed: int):  # type: ignore[no-untyped-def]
+    from lucivant_sim import trajectories as tr
+
+    steps, per_shot, nflags = tr.compile_program(circuit, noise)
+    draws = np.random.default_rng(seed).random(shots * per_shot).reshape(shots, per_shot)
+    batch = _HostBatch.make(circuit.num_qubits, shots, nflags, circuit.num_clbits)
+    bits = tr._run_chunk(steps, circuit, draws, 0, nflags, batch)
+    uniq, cnt = np.unique(bits, return_counts=True)
+    return {circuit.format_bits(int(v)): int(c) for v, c in zip(uniq, cnt)}, type(batch)
+
+
+@pytest.mark.parametrize("which", ["dynamic", "teleport"])
+def test_trajectory_batch_logic_equals_the_reference_rule(which: str) -> None:
+    """The batch's host logic -- queued and fused operators, projections and Kraus operators
+    applied unnormalised, reduced density matrices computed ahead over two qubits and carried on
+    the host -- run over host states, gives the counts of the oracle's sequential per-shot loop
+    (the reference's rule, pinned by the golden counts) for the same draws, and saves passes."""
+    from dynamic_circuits import DYNAMIC, dynamic_noise
+    from lucivant_sim.trajectories import compile_program
+
+    if which == "dynamic":
+        circuit = ls.parse(DYNAMIC)
+        noise = dynamic_noise()
+    else:
+        circuit = ls.parse(wl.teleport_chain(9, 3))
+        noise = wl.noise_c4()
+    shots = 150
+    got, batch = _host_trajectory_counts(circuit, noise, shots, 31)
+    want = orc.trajectory_counts(circuit, noise, shots, np.random.default_rng(31))
+    assert got == want
+    ops = sum(1 for s in compile_program(circuit, noise)[0] if s.kind in ("gate", "measure", "reset"))
+    assert batch.passes < ops
+
+
 def test_search_over_normalised_cumulative_sums_without_dividing_all() -> None:
     """The sampler's binary search divides only the entries it probes, and lands where
     searchsorted over the divided array does: with zero-probability plateaus, exact ties and draws
