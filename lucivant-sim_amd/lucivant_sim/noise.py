@@ -58,16 +58,31 @@ class Channel:
 
     def superoperator(self) -> Matrix:
         """S = sum_i K_i (x) conj(K_i) on the vectorised density matrix (row bits, then column
-        bits); for the analytic depolariser, (1-p) I + (p/d) |I>><<I| in that basis."""
+        bits); for the analytic depolariser, (1-p) I + (p/d) |I>><<I| in that basis. Computed
+        once per channel (a noise model hands the same channel to every matching gate) and
+        returned read-only."""
+        cached = self.__dict__.get("_superop")
+        if cached is not None:
+            return cached
         d = 2**self.num_qubits
         if self.analytic_depolarizing:
             p = float(self.params[0])
             vec_id = np.eye(d, dtype=np.complex128).reshape(-1)
-            return (1 - p) * np.eye(d * d, dtype=np.complex128) + (p / d) * np.outer(vec_id, vec_id)
-        s = np.zeros((d * d, d * d), dtype=np.complex128)
-        for k in self.kraus:
-            s += np.kron(k, k.conj())
+            s = (1 - p) * np.eye(d * d, dtype=np.complex128) + (p / d) * np.outer(vec_id, vec_id)
+        else:
+            s = np.zeros((d * d, d * d), dtype=np.complex128)
+            for k in self.kraus:
+                s += kron_conj(k)
+        s.flags.writeable = False
+        self.__dict__["_superop"] = s
         return s
+
+
+def kron_conj(m: Matrix) -> Matrix:
+    """``np.kron(m, m.conj())`` for a square matrix, as one broadcast product."""
+    m = np.asarray(m, dtype=np.complex128)
+    d = m.shape[0]
+    return (m[:, None, :, None] * m.conj()[None, :, None, :]).reshape(d * d, d * d)
 
 
 def _probability(p: float, what: str, upper: float = 1.0) -> float:

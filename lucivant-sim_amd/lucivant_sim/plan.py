@@ -24,7 +24,7 @@ import numpy as np
 
 from lucivant_sim.circuit import Barrier, Gate, GlobalPhase, Measure, Operation, Reset
 from lucivant_sim.gates import Matrix
-from lucivant_sim.noise import Channel, NoiseModel
+from lucivant_sim.noise import Channel, NoiseModel, kron_conj
 
 #: Widest operator the dense kernel applies (bits).
 KERNEL_MAX_BITS = 5
@@ -54,6 +54,8 @@ def _embedding(positions: tuple[int, ...], k: int) -> tuple[np.ndarray, np.ndarr
 def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
     """``sub`` acting on local positions ``positions`` (first = its most significant bit) of a
     k-position operator whose position j is index bit k-1-j, as a 2^k x 2^k matrix."""
+    if len(positions) == k and list(positions) == list(range(k)):
+        return np.asarray(sub, dtype=np.complex128)
     same, rows, cols = _embedding(tuple(positions), k)
     return np.where(same, np.asarray(sub, dtype=np.complex128)[rows, cols], 0)
 
@@ -191,7 +193,7 @@ def _as_pass(matrix: Matrix, bits: tuple[int, ...], gates: int) -> Pass:
 
 def superop_of_gate(u: Matrix) -> Matrix:
     """U (x) conj(U): rho -> U rho U^dagger on the row-then-column vectorisation."""
-    return np.kron(u, u.conj())
+    return kron_conj(u)
 
 
 def plan_density(
