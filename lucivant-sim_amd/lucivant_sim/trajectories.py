@@ -8,11 +8,11 @@ reset; an if/else contributes the draws of the arm it takes (engines.py:159-205)
 When every if/else has arms with the same number of draws, each shot consumes a constant number D
 of draws and shot k reads stream positions [kD, (k+1)D). This module then draws
 ``rng.random(shots * D)`` once (the same stream as the scalar draws, in the same order) and runs
-all shots together in a batch of states ``[B][2^n]`` in HBM: ideal gates as fused passes shared by
-every shot; noise branches, collapses and conditioned gates as per-shot operators selected on the
-host from the draws and from per-shot reduced density matrices, then applied by one batched
-launch. Each decision uses the reference's float64 rule (cumulative sums in the same order), so
-counts equal the reference's except where a draw falls within rounding of a branch boundary.
+all shots together in a batch of states ``[B][2^n]`` in HBM (:class:`_Batch`): gates, noise
+branches, projections and conditioned gates are queued as per-shot choices and fused into shared
+passes, and decisions read per-shot reduced density matrices, computed on the device or carried
+on the host. Each decision uses the reference's float64 rule (cumulative sums in the same order),
+so counts equal the reference's except where a draw falls within rounding of a branch boundary.
 
 Circuits whose arms draw differently are run shot after shot with the same device engine
 (:func:`run_sequential`), which reproduces the reference's draw order exactly.
