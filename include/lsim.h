@@ -63,7 +63,8 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
 
 /* Select the dense kernel variant: with tiling on (the default), operators whose targets include
  * low index bits run through LDS-staged tiles; off, every placement uses the direct kernel.
- * Returns the previous setting. No reference counterpart (a tuning switch). */
+ * Returns the previous setting; a negative argument only reads it. No reference counterpart (a
+ * tuning switch). */
 int lsim_set_dense_tiling(int enabled);
 
 /* Select which complex128 operator widths run on the FP64 matrix cores (v_mfma_f64_16x16x4f64,

@@ -510,7 +510,7 @@ extern "C" {
 
 int lsim_set_dense_tiling(int enabled) {
   const int old = g_tiling_enabled;
-  g_tiling_enabled = enabled != 0;
+  if (enabled >= 0) g_tiling_enabled = enabled != 0;
   return old;
 }
 

@@ -76,6 +76,23 @@ def test_argument_errors_are_reported_without_a_gpu(lib: ctypes.CDLL) -> None:
     assert lib.lsim_reduce_workspace_bytes(26, 1, 1) > 0
 
 
+def test_variant_switches_read_set_and_clamp(lib: ctypes.CDLL) -> None:
+    """The tuning switches return their previous value, read without changing on a negative
+    argument, and the streaming mode clamps to 0..2."""
+    old_s, old_m, old_t = lib.lsim_set_dense_streaming(-1), lib.lsim_set_dense_mfma(-1), lib.lsim_set_dense_tiling(1)
+    try:
+        assert old_s == 1 and old_m == (1 << 3) | (1 << 4) | (1 << 5)
+        assert lib.lsim_set_dense_streaming(7) == old_s
+        assert lib.lsim_set_dense_streaming(-1) == 2
+        assert lib.lsim_set_dense_streaming(0) == 2 and lib.lsim_set_dense_streaming(-1) == 0
+        assert lib.lsim_set_dense_mfma(1 << 5) == old_m and lib.lsim_set_dense_mfma(-1) == 1 << 5
+        assert lib.lsim_set_dense_tiling(0) == 1 and lib.lsim_set_dense_tiling(-1) == 0
+    finally:
+        lib.lsim_set_dense_streaming(old_s)
+        lib.lsim_set_dense_mfma(old_m)
+        lib.lsim_set_dense_tiling(old_t)
+
+
 def test_engines_refuse_to_run_without_a_gpu() -> None:
     import torch
 
