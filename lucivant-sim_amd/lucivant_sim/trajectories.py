@@ -175,6 +175,49 @@ def _fuse(qubits: list[int], items: list[_Queued], combos: np.ndarray) -> np.nda
     return out
 
 
+def _combinations(columns: list[np.ndarray]) -> tuple[np.ndarray, np.ndarray]:
+    """Distinct rows of the per-shot choice columns (each in -1..T-1) and each shot's row index:
+    the rows are packed into one integer per shot (mixed radix) when that fits, which is far
+    faster than sorting rows."""
+    keys = np.stack(columns, axis=1)
+    radix = [int(c.max()) + 2 for c in columns]
+    if float(np.prod(np.array(radix, dtype=np.float64))) < 2.0**62:
+        code = np.zeros(keys.shape[0], dtype=np.int64)
+        for c, r in zip(columns, radix):
+            code = code * r + (c.astype(np.int64) + 1)
+        _, first, inverse = np.unique(code, return_index=True, return_inverse=True)
+        return keys[first], np.asarray(inverse, dtype=np.int64).reshape(-1)
+    combos, inverse = np.unique(keys, axis=0, return_inverse=True)
+    return combos, np.asarray(inverse, dtype=np.int64).reshape(-1)
+
+
+#: Host seconds per (combination x operator) of a block fused on 2^k x 2^k matrices, per d^3.
+_FUSE_SECONDS_PER_D3 = 0.7e-9
+
+
+def _split_for_host(items: list[_Queued], k: int, shots: int, pass_seconds: float) -> list[list[_Queued]]:
+    """A block's operators in consecutive parts, each one pass, so that fusing a part on the host
+    costs no more than a pass on the device: with many per-shot choices (strong noise) every shot
+    may need its own fused operator, and one more pass is then cheaper than the host products."""
+    d3 = float(1 << (3 * k))
+    budget = max(pass_seconds, 2e-4)
+    parts: list[list[_Queued]] = []
+    part: list[_Queued] = []
+    keys: list[np.ndarray] = []
+    for it in items:
+        grown = keys + ([it.idx] if it.idx is not None else [])
+        combos = len(_combinations(grown)[0]) if grown else 1
+        if part and combos * (len(part) + 1) * d3 * _FUSE_SECONDS_PER_D3 > budget:
+            parts.append(part)
+            part = []
+            grown = [it.idx] if it.idx is not None else []
+        part.append(it)
+        keys = grown
+    if part:
+        parts.append(part)
+    return parts
+
+
 def _descending_many(mats: np.ndarray, bits: tuple[int, ...]) -> tuple[np.ndarray, tuple[int, ...]]:
     """:func:`plan.descending` for a stack of operators on the same targets."""
     k = len(bits)
@@ -312,22 +355,28 @@ class _Batch:
                     rho[pick] = e @ rho[pick] @ e.conj().T
         self.known = (kept, rho)
 
+    def _pass_seconds(self) -> float:
+        """About what one pass over the batch costs on the device (read and write at 5 TB/s)."""
+        return 2.0 * (self.b << self.n) * np.dtype(_NP_DTYPE[self.dtype]).itemsize / 5e12
+
     def flush(self) -> None:
         pending, self.pending = self.pending, []
         for qubits, items in _group(pending, FUSE_WIDTH):
-            per_shot = [it for it in items if it.idx is not None]
-            if not per_shot:
-                op = _fuse(qubits, items, np.zeros((1, 0), dtype=np.int64))[0]
-                self.run_pass(_as_pass(op, tuple(qubits), len(items)))
-                continue
-            keys = np.stack([it.idx for it in per_shot], axis=1)
-            combos, inverse = np.unique(keys, axis=0, return_inverse=True)
-            sel = np.asarray(inverse, dtype=np.int64).reshape(-1)
-            if len(per_shot) == len(items):
-                # shots whose choices are all the identity are left alone
-                sel = np.where(np.all(combos < 0, axis=1)[sel], -1, sel)
-            mats, bits = _descending_many(_fuse(qubits, items, combos), tuple(qubits))
-            self.apply_selected("dense", bits, mats, sel)
+            for part in _split_for_host(items, len(qubits), self.b, self._pass_seconds()):
+                self._apply_block(qubits, part)
+
+    def _apply_block(self, qubits: list[int], items: list[_Queued]) -> None:
+        per_shot = [it for it in items if it.idx is not None]
+        if not per_shot:
+            op = _fuse(qubits, items, np.zeros((1, 0), dtype=np.int64))[0]
+            self.run_pass(_as_pass(op, tuple(qubits), len(items)))
+            return
+        combos, sel = _combinations([it.idx for it in per_shot])
+        if len(per_shot) == len(items):
+            # shots whose choices are all the identity are left alone
+            sel = np.where(np.all(combos < 0, axis=1)[sel], -1, sel)
+        mats, bits = _descending_many(_fuse(qubits, items, combos), tuple(qubits))
+        self.apply_selected("dense", bits, mats, sel)
 
     def density(self, qubits: tuple[int, ...], ahead: tuple[int, ...]) -> np.ndarray:
         """Unnormalised per-shot reduced density matrices on ``qubits`` of the state with every

@@ -409,6 +409,20 @@ def test_trajectory_queue_fuses_blocks_equal_to_program_order() -> None:
         assert np.abs(got - want).max() < 1e-12, b
 
 
+def test_blocks_split_when_host_fusion_would_outcost_a_pass() -> None:
+    """Many distinct per-shot choices make host products dear: the block is cut into consecutive
+    parts (order kept, nothing lost) for a small batch, and kept whole for a large one."""
+    from lucivant_sim.trajectories import _Queued, _split_for_host
+
+    rng = np.random.default_rng(4)
+    items = [_Queued((i % 4,), [np.eye(2)] * 4, rng.integers(-1, 4, size=4096)) for i in range(8)]
+    items.insert(3, _Queued((0, 1), [np.eye(4)]))
+    small = _split_for_host(items, 4, 4096, pass_seconds=1e-5)
+    assert len(small) > 1 and [it for part in small for it in part] == items
+    large = _split_for_host(items, 4, 4096, pass_seconds=10.0)
+    assert large == [items]
+
+
 class _HostBatch:
     """The trajectory batch's four device operations restated on the host (the oracle's
     tensordot for operators), so its queueing, fusion and kept reduced density matrices can be
