@@ -48,6 +48,32 @@ def apply_dense(psi: np.ndarray, n: int, mats: np.ndarray, targets: tuple[int, .
     return s.cpu().numpy().reshape(batch, -1)
 
 
+@pytest.mark.parametrize("k,targets", [(3, (12, 7, 1)), (4, (11, 6, 2, 0)), (5, (12, 9, 5, 3, 1)), (5, (12, 11, 10, 9, 8))])
+def test_matrix_cores_on_a_large_batch_of_states(k: int, targets: tuple[int, ...]) -> None:
+    """2048 states of 13 qubits, each with its own operator or none: the matrix-core kernel then
+    runs few wavefronts per state, each striding over several steps of its state; it agrees with
+    the vector kernel to rounding and leaves unselected states untouched."""
+    lib = nat.load()
+    n, batch = 13, 2048
+    rng = np.random.default_rng(40 + k)
+    psi = rng.normal(size=(batch, 2**n)) + 1j * rng.normal(size=(batch, 2**n))
+    mats = np.stack([unitary(k, 300 + j) for j in range(3)])
+    sel = rng.integers(-1, 3, size=batch).astype(np.int32)
+    old = lib.lsim_set_dense_mfma(-1)
+    try:
+        outs = []
+        for mask in (1 << k, 0):
+            lib.lsim_set_dense_mfma(mask)
+            outs.append(apply_dense(psi, n, mats, targets, nat.C128, sel))
+    finally:
+        lib.lsim_set_dense_mfma(old)
+    assert np.abs(outs[0] - outs[1]).max() < 1e-12
+    untouched = sel < 0
+    assert np.array_equal(outs[0][untouched], psi[untouched])
+    for b in np.flatnonzero(~untouched)[:3]:
+        assert np.abs(outs[0][b] - orc.apply_sv(psi[b], mats[sel[b]], targets, n)).max() < 1e-10
+
+
 @pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_dense_every_placement(k: int, dtype: int) -> None:
