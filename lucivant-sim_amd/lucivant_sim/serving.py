@@ -35,7 +35,7 @@ import torch
 from lucivant_sim import _native as nat
 from lucivant_sim import frontend
 from lucivant_sim import program as pg
-from lucivant_sim.engines import Statevector, launch_expectations, pauli_masks, ptr, require_device, upload
+from lucivant_sim.engines import MAX_DENSITY_QUBITS, MAX_STATEVECTOR_QUBITS, SimulationError, Statevector, launch_expectations, pauli_masks, ptr, require_device, upload
 from lucivant_sim.frontend import Compiled, compile_one
 from lucivant_sim.noise import NoiseModel
 from lucivant_sim.simulator import FUSION_WIDTH, _sample, _sample_many, _tuned
@@ -182,6 +182,9 @@ class Server:
     def _prepare(self, req: Request, c: Compiled) -> _Job:
         if c.method not in ("statevector", "trajectories", "density_matrix"):
             raise ValueError(f"the server runs statevector, density-matrix and trajectory requests, not {c.method!r}")
+        limit = MAX_DENSITY_QUBITS if c.method == "density_matrix" else MAX_STATEVECTOR_QUBITS
+        if c.num_qubits > limit:
+            raise SimulationError(f"{c.num_qubits} qubits are too many for the {c.method} method (up to {limit})")
         size = 16 << c.num_qubits
         if c.method == "density_matrix":
             size = 16 << (2 * c.num_qubits)
@@ -245,7 +248,11 @@ class Server:
         self.host_seconds["wide_launch"] += t3 - t2
         for job in wide:
             if job.finishing is None and job.passes is not None and job.next_pass >= len(job.passes) and job.state is not None:
-                self._queue_outputs(job)
+                try:
+                    self._queue_outputs(job)
+                except BaseException as exc:  # noqa: BLE001 - e.g. no memory for the outputs
+                    self._finish_error(job.req, exc)
+                    self._release(job)
         t4 = time.perf_counter()
         self.host_seconds["wide_outputs"] += t4 - t3
         for job in wide:
@@ -317,11 +324,16 @@ class Server:
         if job.state is None:
             job.stream = self.streams[self._rr % len(self.streams)]
             self._rr += 1
-            with torch.cuda.stream(job.stream):
-                job.state = Statevector(job.c.num_qubits, dtype=self.dtype)
-                job.passes = job.c.passes
-                assert job.passes is not None
-                job.mats, addrs = upload([p.matrix for p in job.passes], self.dtype)
+            try:
+                with torch.cuda.stream(job.stream):
+                    job.state = Statevector(job.c.num_qubits, dtype=self.dtype)
+                    job.passes = job.c.passes
+                    assert job.passes is not None
+                    job.mats, addrs = upload([p.matrix for p in job.passes], self.dtype)
+            except BaseException as exc:  # noqa: BLE001 - reported to the requester
+                self._finish_error(job.req, exc)
+                self._release(job)
+                return
             # every launch's function and arguments, built once
             state, n, hs = ptr(job.state.buffer), job.c.num_qubits, job.stream.cuda_stream
             job.calls = []

@@ -45,6 +45,18 @@ def main() -> None:
         run_many([texts[0], "OPENQASM 2.0;\nqreg q[2];\nh q[5];\n"], 10, seeds=[1, 2])
     except ls.QasmError as exc:
         error_kept = exc.line == 3
+    # an error in a later chunk of a long batch (chunks of 32): the first error in program
+    # order is raised, and the pool serves the next batch normally
+    many = [wl.random_layers(6, 2, 900 + i) for i in range(80)]
+    many[67] = "OPENQASM 3;\nqubit[2] q;\n\nfoo q[0];\n"
+    many[75] = "OPENQASM 2.0;\nqreg q[2];\nh q[5];\n"
+    try:
+        run_many(many, 10, seeds=list(range(80)))
+        error_kept = False
+    except ls.QasmError as exc:
+        error_kept = error_kept and exc.line == 4
+    again = run_many(texts[:3], 700, seeds=seeds[:3])
+    error_kept = error_kept and all(a.counts == b.counts for a, b in zip(again, alone[:3]))
 
     rng = np.random.default_rng(3)
     reqs = []

@@ -70,3 +70,28 @@ def test_admission_respects_the_hbm_budget() -> None:
     assert server.peak_used <= 16 << 16
     for r in reqs:
         assert r.result == ls.run(r.program, 100, seed=r.seed).counts
+
+
+def test_bad_requests_fail_alone_and_the_rest_are_served() -> None:
+    """Requests that cannot run -- a syntax error, a state too wide for the device engines, a
+    method the server does not run -- come back with their error; the good requests in the same
+    mix come back with the single-circuit API's results."""
+    good = _requests()[:3]
+    bad = [
+        Request("OPENQASM 3;\nqubit[2] q;\nfoo q[0];\n", shots=10, seed=1),
+        Request("OPENQASM 3;\ninclude \"stdgates.inc\";\nqubit[30] q;\nh q;\nt q;\n", shots=10, seed=2),
+        Request(wl.random_layers(4, 2, 3), shots=10, seed=3, method="stabilizer"),
+    ]
+    server = Server(streams=2, chunk_passes=8)
+    for r in [good[0], bad[0], good[1], bad[1], bad[2], good[2]]:
+        server.submit(r)
+    server.run_until_idle()
+    assert isinstance(bad[0].error, ls.QasmError) and bad[0].error.line == 3
+    assert isinstance(bad[1].error, (ls.SimulationError, ValueError))
+    assert isinstance(bad[2].error, (ls.SimulationError, ValueError))
+    for r in good:
+        assert r.error is None
+        if r.observable is not None:
+            assert abs(r.result - ls.expectation(r.program, r.observable)) < 1e-10
+        else:
+            assert r.result == ls.run(r.program, r.shots, seed=r.seed, noise=r.noise).counts
