@@ -136,11 +136,12 @@ def _greedy(
 
 
 def plan_statevector(
-    ops: tuple[Operation, ...], num_qubits: int, width: int = 4, low_bits: int = 0, low_width: int = 0
+    ops: tuple[Operation, ...], num_qubits: int, width: int = 4, low_bits: int = 0, low_width: int = 0, merge: bool = True
 ) -> list[Pass]:
     """Passes for an ideal static circuit on a statevector (measurements deferred, resets of
     untouched qubits and barriers are no-ops, as in simulator.py:229-251). ``low_bits`` and
-    ``low_width`` optionally narrow the blocks that touch the lowest qubits."""
+    ``low_width`` optionally narrow the blocks that touch the lowest qubits; ``merge`` joins
+    blocks that fit together after the greedy pass (:func:`_merge`)."""
     if not 1 <= width <= KERNEL_MAX_BITS:
         raise ValueError(f"fusion width must be between 1 and {KERNEL_MAX_BITS}")
     items: list[tuple[Matrix, tuple[int, ...]]] = []
@@ -157,7 +158,10 @@ def plan_statevector(
         else:
             raise ValueError(f"a static statevector can't run {type(op).__name__}")
     passes: list[Pass] = []
-    for blk in _greedy(items, width, low_bits, low_width):
+    blocks = _greedy(items, width, low_bits, low_width)
+    if merge:
+        blocks = _merge(blocks, width, low_bits, low_width)
+    for blk in blocks:
         passes.append(_as_pass(blk.matrix, tuple(blk.qubits), blk.gates))
     if phase != 1:
         if passes:
@@ -166,6 +170,41 @@ def plan_statevector(
         elif num_qubits > 0:
             passes.append(Pass("scale", (), np.array([phase], dtype=np.complex128), 0))
     return passes
+
+
+def _merge(blocks: list[_Block], width: int, low_bits: int = 0, low_width: int = 0) -> list[_Block]:
+    """Fewer blocks: block j joins a later block j2 when their union fits the width (with the
+    same low-qubit cap as :func:`_greedy`) and no block between them shares a qubit with j, so j
+    commutes past those blocks and its operator can be applied just before j2's. Repeated until
+    nothing merges; the greedy leaves small blocks behind (a first gate on fresh qubits, the
+    remainder of a layer), and each merge saves one pass over the state."""
+    while True:
+        alive = [True] * len(blocks)
+        merged = False
+        for j, blk in enumerate(blocks):
+            if not alive[j]:
+                continue
+            mine = set(blk.qubits)
+            for j2 in range(j + 1, len(blocks)):
+                if not alive[j2]:
+                    continue
+                other = set(blocks[j2].qubits)
+                union = mine | other
+                limit = low_width if low_width and low_bits and min(union) < low_bits else width
+                if len(union) <= limit:
+                    joined = _Block(list(blk.qubits), np.eye(1 << len(blk.qubits), dtype=np.complex128))
+                    joined.absorb(blk.matrix, tuple(blk.qubits))
+                    joined.absorb(blocks[j2].matrix, tuple(blocks[j2].qubits))
+                    joined.gates = blk.gates + blocks[j2].gates
+                    blocks[j2] = joined
+                    alive[j] = False
+                    merged = True
+                    break
+                if mine & other:
+                    break
+        blocks = [b for b, a in zip(blocks, alive) if a]
+        if not merged:
+            return blocks
 
 
 def descending(matrix: Matrix, bits: tuple[int, ...]) -> tuple[Matrix, tuple[int, ...]]:
@@ -216,7 +255,7 @@ def plan_density(
         return tuple(n + x for x in q) + tuple(q)
 
     def flush() -> None:
-        for blk in _greedy(items, 2 * width):
+        for blk in _merge(_greedy(items, 2 * width), 2 * width):
             out.append(_as_pass(blk.matrix, tuple(blk.qubits), blk.gates))
         items.clear()
 

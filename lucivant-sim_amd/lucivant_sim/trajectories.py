@@ -156,6 +156,32 @@ def _group(items: list[_Queued], width: int) -> list[tuple[list[int], list[_Queu
     return blocks
 
 
+def _merge_groups(blocks: list[tuple[list[int], list[_Queued]]], width: int) -> list[tuple[list[int], list[_Queued]]]:
+    """Fewer blocks, as :func:`plan._merge` does for fused passes: block j joins a later block
+    when their union fits and no block between them shares a qubit with j."""
+    while True:
+        alive = [True] * len(blocks)
+        merged = False
+        for j, (qubits, items) in enumerate(blocks):
+            if not alive[j]:
+                continue
+            mine = set(qubits)
+            for j2 in range(j + 1, len(blocks)):
+                if not alive[j2]:
+                    continue
+                other = set(blocks[j2][0])
+                if len(mine | other) <= width:
+                    blocks[j2] = (qubits + [q for q in blocks[j2][0] if q not in mine], items + blocks[j2][1])
+                    alive[j] = False
+                    merged = True
+                    break
+                if mine & other:
+                    break
+        blocks = [b for b, a in zip(blocks, alive) if a]
+        if not merged:
+            return blocks
+
+
 def _fuse(qubits: list[int], items: list[_Queued], combos: np.ndarray) -> np.ndarray:
     """The block's operator for each combination of per-shot choices (a row of ``combos`` holds
     one table index per per-shot item, in order; -1 is the identity): the product of the items in
@@ -361,7 +387,7 @@ class _Batch:
 
     def flush(self) -> None:
         pending, self.pending = self.pending, []
-        for qubits, items in _group(pending, FUSE_WIDTH):
+        for qubits, items in _merge_groups(_group(pending, FUSE_WIDTH), FUSE_WIDTH):
             for part in _split_for_host(items, len(qubits), self.b, self._pass_seconds()):
                 self._apply_block(qubits, part)
 
