@@ -172,38 +172,51 @@ def plan_statevector(
     return passes
 
 
+def _join(first: _Block, second: _Block) -> _Block:
+    """One block applying ``first`` then ``second``."""
+    joined = _Block(list(first.qubits), np.eye(1 << len(first.qubits), dtype=np.complex128))
+    joined.absorb(first.matrix, tuple(first.qubits))
+    joined.absorb(second.matrix, tuple(second.qubits))
+    joined.gates = first.gates + second.gates
+    return joined
+
+
 def _merge(blocks: list[_Block], width: int, low_bits: int = 0, low_width: int = 0) -> list[_Block]:
-    """Fewer blocks: block j joins a later block j2 when their union fits the width (with the
-    same low-qubit cap as :func:`_greedy`) and no block between them shares a qubit with j, so j
-    commutes past those blocks and its operator can be applied just before j2's. Repeated until
-    nothing merges; the greedy leaves small blocks behind (a first gate on fresh qubits, the
-    remainder of a layer), and each merge saves one pass over the state."""
+    """Fewer blocks: two blocks join when their union fits the width (with the same low-qubit cap
+    as :func:`_greedy`) and no block between them shares a qubit with the one that moves. Forward,
+    block j moves to a later block j2 (commuting past the blocks between); backward, block j2
+    moves to an earlier block j. Repeated until nothing merges. The greedy leaves small blocks
+    behind (the first gates on fresh qubits, the rest of a layer), and each merge saves one pass
+    over the state."""
+
+    def fits(a: set[int], b: set[int]) -> bool:
+        union = a | b
+        limit = low_width if low_width and low_bits and min(union) < low_bits else width
+        return len(union) <= limit
+
     while True:
-        alive = [True] * len(blocks)
-        merged = False
-        for j, blk in enumerate(blocks):
-            if not alive[j]:
-                continue
-            mine = set(blk.qubits)
-            for j2 in range(j + 1, len(blocks)):
-                if not alive[j2]:
+        before = len(blocks)
+        for backward in (False, True):
+            alive = [True] * len(blocks)
+            order = range(len(blocks) - 1, -1, -1) if backward else range(len(blocks))
+            for j in order:
+                if not alive[j]:
                     continue
-                other = set(blocks[j2].qubits)
-                union = mine | other
-                limit = low_width if low_width and low_bits and min(union) < low_bits else width
-                if len(union) <= limit:
-                    joined = _Block(list(blk.qubits), np.eye(1 << len(blk.qubits), dtype=np.complex128))
-                    joined.absorb(blk.matrix, tuple(blk.qubits))
-                    joined.absorb(blocks[j2].matrix, tuple(blocks[j2].qubits))
-                    joined.gates = blk.gates + blocks[j2].gates
-                    blocks[j2] = joined
-                    alive[j] = False
-                    merged = True
-                    break
-                if mine & other:
-                    break
-        blocks = [b for b, a in zip(blocks, alive) if a]
-        if not merged:
+                mine = set(blocks[j].qubits)
+                others = range(j - 1, -1, -1) if backward else range(j + 1, len(blocks))
+                for j2 in others:
+                    if not alive[j2]:
+                        continue
+                    other = set(blocks[j2].qubits)
+                    if fits(mine, other):
+                        first, second = (j2, j) if backward else (j, j2)
+                        blocks[j2] = _join(blocks[first], blocks[second])
+                        alive[j] = False
+                        break
+                    if mine & other:
+                        break
+            blocks = [b for b, a in zip(blocks, alive) if a]
+        if len(blocks) == before:
             return blocks
 
 

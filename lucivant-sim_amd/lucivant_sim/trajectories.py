@@ -157,28 +157,32 @@ def _group(items: list[_Queued], width: int) -> list[tuple[list[int], list[_Queu
 
 
 def _merge_groups(blocks: list[tuple[list[int], list[_Queued]]], width: int) -> list[tuple[list[int], list[_Queued]]]:
-    """Fewer blocks, as :func:`plan._merge` does for fused passes: block j joins a later block
-    when their union fits and no block between them shares a qubit with j."""
+    """Fewer blocks, as :func:`plan._merge` does for fused passes: two blocks join when their
+    union fits and no block between them shares a qubit with the one that moves (forward: an
+    earlier block moves to a later one; backward: a later one moves to an earlier one)."""
     while True:
-        alive = [True] * len(blocks)
-        merged = False
-        for j, (qubits, items) in enumerate(blocks):
-            if not alive[j]:
-                continue
-            mine = set(qubits)
-            for j2 in range(j + 1, len(blocks)):
-                if not alive[j2]:
+        before = len(blocks)
+        for backward in (False, True):
+            alive = [True] * len(blocks)
+            order = range(len(blocks) - 1, -1, -1) if backward else range(len(blocks))
+            for j in order:
+                if not alive[j]:
                     continue
-                other = set(blocks[j2][0])
-                if len(mine | other) <= width:
-                    blocks[j2] = (qubits + [q for q in blocks[j2][0] if q not in mine], items + blocks[j2][1])
-                    alive[j] = False
-                    merged = True
-                    break
-                if mine & other:
-                    break
-        blocks = [b for b, a in zip(blocks, alive) if a]
-        if not merged:
+                mine = set(blocks[j][0])
+                others = range(j - 1, -1, -1) if backward else range(j + 1, len(blocks))
+                for j2 in others:
+                    if not alive[j2]:
+                        continue
+                    other = set(blocks[j2][0])
+                    if len(mine | other) <= width:
+                        first, second = (blocks[j2], blocks[j]) if backward else (blocks[j], blocks[j2])
+                        blocks[j2] = (first[0] + [q for q in second[0] if q not in first[0]], first[1] + second[1])
+                        alive[j] = False
+                        break
+                    if mine & other:
+                        break
+            blocks = [b for b, a in zip(blocks, alive) if a]
+        if len(blocks) == before:
             return blocks
 
 

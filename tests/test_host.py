@@ -226,6 +226,9 @@ def test_fused_statevector_passes_equal_sequential_gates(width: int, low_width: 
     psi0[0] = 1
     assert np.allclose(_run_passes_on_cpu(psi0, passes, 7), orc.evolve_sv(c), atol=1e-12)
     assert len(passes) < sum(1 for op in c.operations if isinstance(op, ls.circuit.Gate))
+    unmerged = plan_statevector(c.operations, c.num_qubits, width, 3, low_width, merge=False)
+    assert len(passes) <= len(unmerged)
+    assert np.allclose(_run_passes_on_cpu(psi0, unmerged, 7), orc.evolve_sv(c), atol=1e-12)
 
 
 @pytest.mark.parametrize("width", [1, 2])
