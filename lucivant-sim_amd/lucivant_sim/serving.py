@@ -8,9 +8,12 @@ time. The scheduler runs in ticks; at every tick it
    while others are on the GPU join as memory frees up;
 2. launches every admitted request of at most ``program.max_qubits`` qubits that hasn't run yet,
    grouped by width, as one LDS-program launch (one workgroup per circuit or per shot);
-3. advances every resident wide request by up to ``chunk_passes`` fused passes on its own HIP
-   stream (states stay resident in HBM between ticks), so wide circuits overlap one another and
-   small batches slip in between their chunks;
+3. advances resident wide requests by up to ``chunk_passes`` fused passes each on their own
+   HIP streams (states stay resident in HBM between ticks), so wide circuits overlap one another
+   and small batches slip in between their chunks. At most ``max_launching`` wide requests are
+   being launched at once, started in priority order: cheap ones first, then count requests by
+   decreasing width (their host sampling of 2^n probabilities then overlaps the device work of
+   the rest), then the others in arrival order;
 4. completes finished requests: probabilities or expectations come back to the host, counts are
    sampled with the request's own seeded generator exactly as ``run`` samples them.
 
@@ -93,6 +96,7 @@ class _Job:
     calls: list[tuple[Any, tuple]] | None = None
     # outputs queued behind the last pass: (event, device tensor), read once the event completes
     finishing: tuple[torch.cuda.Event, torch.Tensor, Any] | None = None
+    seq: int = 0  # admission order
 
 
 class Server:
@@ -106,9 +110,12 @@ class Server:
         chunk_passes: int = 64,
         dtype: int = nat.C128,
         fusion_width: int | None = None,
+        max_launching: int = 2,
     ) -> None:
         require_device()
         self.budget = hbm_budget
+        self.max_launching = max(1, max_launching)
+        self._seq = itertools.count()
         self.used = 0
         self.dtype = dtype
         self.width = fusion_width or FUSION_WIDTH
@@ -189,7 +196,7 @@ class Server:
         if c.method == "density_matrix":
             size = 16 << (2 * c.num_qubits)
         footprint = size * (req.shots if c.method == "trajectories" and c.num_qubits > self.max_lds else 1)
-        return _Job(req, c, c.method, footprint, c.terms)
+        return _Job(req, c, c.method, footprint, c.terms, seq=next(self._seq))
 
     def _admit(self) -> None:
         while True:
@@ -241,9 +248,15 @@ class Server:
         t2 = time.perf_counter()
         self.host_seconds["small"] += t2 - t1
         wide = [j for j in self.resident if j not in small]
-        for job in wide:
-            if job.finishing is None:
-                self._advance_wide(job)
+        launching = sum(1 for j in wide if j.state is not None and j.passes is not None and j.next_pass < len(j.passes))
+        for job in sorted(wide, key=self._priority):
+            if job.finishing is not None:
+                continue
+            if job.state is None and job.method == "statevector":
+                if launching >= self.max_launching:
+                    continue
+                launching += 1
+            self._advance_wide(job)
         t3 = time.perf_counter()
         self.host_seconds["wide_launch"] += t3 - t2
         for job in wide:
@@ -261,6 +274,16 @@ class Server:
         self._sampling = [f for f in self._sampling if not f.done()]
         self.host_seconds["wide_finish"] += time.perf_counter() - t4
         return True
+
+    @staticmethod
+    def _priority(job: _Job) -> tuple:
+        """Start order of wide requests: cheap ones (under about 16 passes' worth of a 24-qubit
+        state) first, since they finish at once; then count requests, widest first, whose host
+        sampling is longest and should overlap the rest; then the others by admission."""
+        n = job.c.num_qubits
+        work = len(job.c.passes or ()) << n
+        count = job.terms is None and job.method == "statevector"
+        return (0 if work < (16 << 24) else 1, -(1 << n) if count else 0, job.seq)
 
     def _run_small(self, jobs: list[_Job]) -> None:
         by_key: dict[tuple, list[_Job]] = {}

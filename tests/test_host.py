@@ -426,6 +426,28 @@ def test_blocks_split_when_host_fusion_would_outcost_a_pass() -> None:
     assert large == [items]
 
 
+def test_server_starts_cheap_then_widest_count_requests_first() -> None:
+    """The serving loop's start order for wide requests: cheap ones first, then count requests
+    widest first (their host sampling overlaps the rest), then the others by admission."""
+    from types import SimpleNamespace
+
+    from lucivant_sim.serving import Server
+
+    def job(seq: int, n: int, passes: int, terms=None):  # type: ignore[no-untyped-def]
+        return SimpleNamespace(seq=seq, c=SimpleNamespace(num_qubits=n, passes=[0] * passes), terms=terms, method="statevector")
+
+    jobs = [
+        job(0, 26, 300, {"Z" * 26: 1.0}),
+        job(1, 24, 200),
+        job(2, 14, 80),
+        job(3, 26, 310),
+        job(4, 25, 90, {"Z" * 25: 1.0}),
+        job(5, 16, 40, {"Z" * 16: 1.0}),
+    ]
+    order = [j.seq for j in sorted(jobs, key=Server._priority)]
+    assert order == [2, 5, 3, 1, 0, 4]
+
+
 class _HostBatch:
     """The trajectory batch's four device operations restated on the host (the oracle's
     tensordot for operators), so its queueing, fusion and kept reduced density matrices can be
