@@ -47,6 +47,11 @@ int lsim_abi_version(void);
 /* Device 0 facts: compute units, HBM bytes, gfx architecture name. */
 int lsim_device_info(int* cu_count, long long* hbm_bytes, char* arch, int arch_len);
 
+/* HOST arrays: out[i] = in[0] + ... + in[i], added strictly left to right in float64 -- the
+ * numbers numpy.cumsum gives, bit for bit. Replaces the cumulative distribution of _sample
+ * (simulator.py:267); needs no GPU. */
+int lsim_host_cumsum(const double* in, double* out, size_t n);
+
 /* |index> in every state: replaces Statevector.__init__ / DensityMatrix.__init__
  * (engines.py:139-148, 216-225; for a density matrix pass nbits=2n, index=0). */
 int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,

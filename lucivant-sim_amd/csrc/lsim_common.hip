@@ -112,6 +112,22 @@ const char* lsim_last_error(void) { return lsim::g_error; }
 
 int lsim_abi_version(void) { return 1; }
 
+// A plain sequential loop: the same float64 additions in the same order as NumPy's accumulate
+// (out[0] = in[0], then one addition per element), which is all bit-exact sampling needs, at
+// about twice NumPy's speed (its accumulate loop carries per-element overhead that the
+// dependent-add chain doesn't need). No reassociation: the build has no fast-math flags.
+int lsim_host_cumsum(const double* in, double* out, size_t n) {
+  if (n == 0) return LSIM_OK;
+  LSIM_REQUIRE(in != nullptr && out != nullptr, "the input or output pointer is NULL");
+  double c = in[0];
+  out[0] = c;
+  for (size_t i = 1; i < n; ++i) {
+    c += in[i];
+    out[i] = c;
+  }
+  return LSIM_OK;
+}
+
 int lsim_device_info(int* cu_count, long long* hbm_bytes, char* arch, int arch_len) {
   hipDeviceProp_t p;
   int dev = 0;

@@ -35,6 +35,7 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "lsim_last_error": ([], ctypes.c_char_p),
     "lsim_abi_version": ([], _c_int),
     "lsim_device_info": ([_c_int_p, ctypes.POINTER(_c_ll), ctypes.c_char_p, _c_int], _c_int),
+    "lsim_host_cumsum": ([_c_void, _c_void, ctypes.c_size_t], _c_int),
     "lsim_sv_set_basis": ([_c_void, _c_int, _c_int, _c_ll, _c_ll, _c_void], _c_int),
     "lsim_sv_apply_dense": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_void, _c_void, _c_void], _c_int),
     "lsim_set_dense_tiling": ([_c_int], _c_int),
