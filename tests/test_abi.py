@@ -93,6 +93,22 @@ def test_variant_switches_read_set_and_clamp(lib: ctypes.CDLL) -> None:
         lib.lsim_set_dense_tiling(old_t)
 
 
+def test_host_cumsum_is_numpys_bit_for_bit(lib: ctypes.CDLL) -> None:
+    """The sampler's host running sum: the same float64 additions in the same order as
+    numpy.cumsum, including a leading negative zero, single elements and empty input."""
+    import numpy as np
+
+    rng = np.random.default_rng(8)
+    for x in (rng.random(1000) ** 6, rng.normal(size=4097), np.array([-0.0, 1e-300, 1.0]), np.array([2.5]), np.zeros(0)):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.full(x.size, np.nan)
+        assert lib.lsim_host_cumsum(x.ctypes.data, out.ctypes.data, x.size) == 0
+        want = np.cumsum(x)
+        assert np.array_equal(out, want) and np.array_equal(np.signbit(out), np.signbit(want))
+    assert lib.lsim_host_cumsum(None, None, 3) == 1
+    assert "NULL" in _err(lib)
+
+
 def test_engines_refuse_to_run_without_a_gpu() -> None:
     import torch
 
