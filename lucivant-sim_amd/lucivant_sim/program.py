@@ -128,13 +128,6 @@ def _compile_static_vectorised(circuit: Circuit) -> Program:
     return prog
 
 
-def from_table(num_qubits: int, table: np.ndarray, pool: np.ndarray) -> Program:
-    """A Program around an op table and matrix pool built elsewhere (e.g. in a worker process)."""
-    prog = Program(num_qubits)
-    prog.table, prog.pool, prog.mat_size = table, pool, pool.size
-    return prog
-
-
 def compile_trajectory(circuit: Circuit, noise: NoiseModel | None) -> Program:
     """A dynamic or noisy circuit as one shot's program (draw-balanced if/else arms only)."""
     from lucivant_sim.trajectories import compile_program
