@@ -99,25 +99,16 @@ def compile_static(circuit: Circuit, fuse_width: int = 0) -> Program:
     calls per circuit whatever its length), which keeps host time per batched circuit small."""
     if not fuse_width:
         return _compile_static_vectorised(circuit)
+    from lucivant_sim.plan import plan_statevector
+
     prog = Program(circuit.num_qubits)
-    if fuse_width:
-        from lucivant_sim.plan import plan_statevector
-
-        for p in plan_statevector(circuit.operations, circuit.num_qubits, fuse_width):
-            if p.kind == "scale":
-                prog.op(OP_PHASE, (), mat=prog.add_matrix(p.matrix))
-            elif p.kind == "diag":
-                prog.op(OP_DIAG, p.bits, mat=prog.add_matrix(p.matrix))
-            else:
-                prog.op(OP_DENSE, p.bits, mat=prog.add_matrix(p.matrix))
-        return prog
-    from lucivant_sim.circuit import GlobalPhase
-
-    for op in circuit.operations:
-        if isinstance(op, Gate):
-            _gate(prog, op.matrix, op.qubits, (0, 0))
-        elif isinstance(op, GlobalPhase):
-            prog.op(OP_PHASE, (), mat=prog.add_matrix(np.array([np.exp(1j * op.angle)])))
+    for p in plan_statevector(circuit.operations, circuit.num_qubits, fuse_width):
+        if p.kind == "scale":
+            prog.op(OP_PHASE, (), mat=prog.add_matrix(p.matrix))
+        elif p.kind == "diag":
+            prog.op(OP_DIAG, p.bits, mat=prog.add_matrix(p.matrix))
+        else:
+            prog.op(OP_DENSE, p.bits, mat=prog.add_matrix(p.matrix))
     return prog
 
 
