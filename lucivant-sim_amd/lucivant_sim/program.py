@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from lucivant_sim import _native as nat
-from lucivant_sim.circuit import Circuit, Gate
+from lucivant_sim.circuit import Circuit
 from lucivant_sim.engines import ptr, require_device, stream_handle
 from lucivant_sim.noise import NoiseModel
 from lucivant_sim.plan import is_diagonal
