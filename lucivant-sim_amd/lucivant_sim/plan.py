@@ -96,7 +96,7 @@ class _Block:
             self.matrix = out
             self.qubits.extend(new)
 
-    def absorb(self, matrix: Matrix, qubits: tuple[int, ...]) -> None:
+    def absorb(self, matrix: Matrix, qubits: tuple[int, ...], count: int = 1) -> None:
         self.extend(qubits)
         k = len(self.qubits)
         if len(qubits) == 1:
@@ -108,26 +108,55 @@ class _Block:
         else:
             pos = [self.qubits.index(q) for q in qubits]
             self.matrix = embed(matrix, pos, k) @ self.matrix
-        self.gates += 1
+        self.gates += count
+
+
+def _one_qubit_runs(items: list[tuple[Matrix, tuple[int, ...]]]) -> list[tuple[Matrix, tuple[int, ...], int]]:
+    """The same operators with each run of one-qubit gates on a qubit (no other gate on that
+    qubit in between) multiplied into one 2x2 matrix, placed where the run's first gate was: the
+    later gates of the run commute with everything in between, which acts on other qubits. The
+    greedy then forms the same blocks in the same order, from fewer operators. Returns (matrix,
+    qubits, gates it stands for)."""
+    out: list[tuple[Matrix, tuple[int, ...], int]] = []
+    run: dict[int, int] = {}  # qubit -> index in ``out`` of its open run
+    for matrix, qubits in items:
+        if len(qubits) == 1:
+            q = qubits[0]
+            at = run.get(q)
+            if at is None:
+                run[q] = len(out)
+                out.append((matrix, qubits, 1))
+            else:
+                m, _, count = out[at]
+                out[at] = (matrix @ m, qubits, count + 1)
+            continue
+        for q in qubits:
+            run.pop(q, None)
+        out.append((matrix, qubits, 1))
+    return out
 
 
 def _greedy(
-    items: list[tuple[Matrix, tuple[int, ...]]], width: int, low_bits: int = 0, low_width: int = 0
+    items: list[tuple[Matrix, tuple[int, ...]]] | list[tuple[Matrix, tuple[int, ...], int]],
+    width: int,
+    low_bits: int = 0,
+    low_width: int = 0,
 ) -> list[_Block]:
     """Merge each item into the latest block touching its qubits while the union has at most
     ``width`` qubits -- at most ``low_width`` when the union includes a qubit below ``low_bits``
     (operators on the lowest index bits cost more per pass; the autotuner sets both)."""
     blocks: list[_Block] = []
     last: dict[int, int] = {}  # qubit -> index of the latest block touching it
-    for matrix, qubits in items:
+    for matrix, qubits, *rest in items:
+        count = rest[0] if rest else 1
         j = max((last.get(q, -1) for q in qubits), default=-1)
         union = set(blocks[j].qubits) | set(qubits) if j >= 0 else set()
         limit = low_width if low_width and low_bits and min(union, default=low_bits) < low_bits else width
         if j >= 0 and len(union) <= limit:
-            blocks[j].absorb(matrix, qubits)
+            blocks[j].absorb(matrix, qubits, count)
         else:
             blk = _Block(list(qubits), np.eye(1 << len(qubits), dtype=np.complex128))
-            blk.absorb(matrix, qubits)
+            blk.absorb(matrix, qubits, count)
             blocks.append(blk)
             j = len(blocks) - 1
         for q in qubits:
@@ -158,7 +187,7 @@ def plan_statevector(
         else:
             raise ValueError(f"a static statevector can't run {type(op).__name__}")
     passes: list[Pass] = []
-    blocks = _greedy(items, width, low_bits, low_width)
+    blocks = _greedy(_one_qubit_runs(items), width, low_bits, low_width)
     if merge:
         blocks = _merge(blocks, width, low_bits, low_width)
     for blk in blocks:
