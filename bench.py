@@ -363,7 +363,7 @@ def bench_trajectories(shots: int = 4096) -> dict:
     pa = dict(zip(ua.tolist(), (ca / shots).tolist()))
     pb = dict(zip(ub.tolist(), (cb / shots).tolist()))
     keys = set(pa) | set(pb)
-    out["workload"] = f"20-qubit teleport chain, {shots} shots in one batch ({shots} x 16 MiB complex128 states)"
+    out["workload"] = f"20-qubit teleport chain, {shots} shots in one batch (room for {shots} x 16 MiB complex128 states; shots with equal histories share one)"
     out["complex64_vs_complex128"] = {
         "shots_with_different_bits": int(np.sum(a != b)),
         "tvd": round(0.5 * sum(abs(pa.get(k, 0) - pb.get(k, 0)) for k in keys), 6),

@@ -57,6 +57,14 @@ int lsim_host_cumsum(const double* in, double* out, size_t n);
 int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,
                       void* stream);
 
+/* State dst[j] <- state src[j] for j < count, within one batch (src and dst: DEVICE int arrays of
+ * `count` indices; an index outside 0..batch-1 skips its copy; no state may be both a source and
+ * a destination). No reference counterpart: the reference runs one fresh state per trajectory
+ * (simulator.py:364-371); the batched trajectories keep shots with equal histories in one state
+ * and copy it where their choices part. */
+int lsim_sv_copy_states(void* states, int dtype, int nbits, long long batch, long long count,
+                        const int* src, const int* dst, void* stream);
+
 /* psi <- M psi on k <= 5 target bits, in place: replaces _apply (engines.py:38-42) and
  * _apply_slices (engines.py:45-119), and for a density matrix DensityMatrix.apply and
  * apply_channel (engines.py:233-250) given the superoperator U (x) conj(U) or sum K (x) conj(K)

@@ -375,6 +375,27 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // y <- y + a x over whole buffers (the accumulation of an out-of-place Kraus sum).
+// state dst[j] <- state src[j] of a batch, for j = first + blockIdx.y; indices outside the batch
+// are skipped
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_copy_states(typename Cx<R>::V* states, uint32_t nbits, uint32_t batch,
+                  const int* __restrict__ src, const int* __restrict__ dst, uint32_t first) {
+  using V = typename Cx<R>::V;
+  const uint32_t j = first + blockIdx.y;
+  const int from = src[j], to = dst[j];
+  if (from < 0 || to < 0 || (uint32_t)from >= batch || (uint32_t)to >= batch) return;
+  const uint64_t size = 1ull << nbits;
+  const V* __restrict__ in = states + ((size_t)from << nbits);
+  V* __restrict__ out = states + ((size_t)to << nbits);
+  const uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint64_t i = i0 + (uint64_t)r * kBlock;
+    if (i < size) out[i] = in[i];
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_axpy(typename Cx<R>::V* __restrict__ y, const typename Cx<R>::V* __restrict__ x,
@@ -619,6 +640,27 @@ int lsim_sv_scale(void* states, int dtype, int nbits, long long batch, double re
       k_scale<float, false><<<grid, kBlock, 0, s>>>((float2*)states, total, re, im);
   }
   return check_launch("scale");
+}
+
+int lsim_sv_copy_states(void* states, int dtype, int nbits, long long batch, long long count,
+                        const int* src, const int* dst, void* stream) {
+  if (int rc = check_state(states, dtype, nbits, batch)) return rc;
+  LSIM_REQUIRE(count >= 0 && count <= batch, "count must be between 0 and the batch size");
+  if (count == 0) return LSIM_OK;
+  LSIM_REQUIRE(src != nullptr && dst != nullptr, "the index pointer is NULL");
+  const uint64_t size = 1ull << nbits;
+  hipStream_t s = as_stream(stream);
+  for (long long c0 = 0; c0 < count; c0 += kMaxGridY) {
+    const uint32_t gy = (uint32_t)(count - c0 < kMaxGridY ? count - c0 : kMaxGridY);
+    const dim3 grid(blocks_for(size), gy);
+    if (dtype == LSIM_C128)
+      k_copy_states<double><<<grid, kBlock, 0, s>>>((double2*)states, nbits, (uint32_t)batch, src,
+                                                    dst, (uint32_t)c0);
+    else
+      k_copy_states<float><<<grid, kBlock, 0, s>>>((float2*)states, nbits, (uint32_t)batch, src,
+                                                   dst, (uint32_t)c0);
+  }
+  return check_launch("copy states");
 }
 
 int lsim_sv_axpy(void* y, const void* x, int dtype, int nbits, long long batch, double re,

@@ -43,6 +43,7 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "lsim_set_dense_streaming": ([_c_int], _c_int),
     "lsim_sv_apply_diag": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_void, _c_void, _c_void], _c_int),
     "lsim_sv_scale": ([_c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),
+    "lsim_sv_copy_states": ([_c_void, _c_int, _c_int, _c_ll, _c_ll, _c_void, _c_void, _c_void], _c_int),
     "lsim_sv_axpy": ([_c_void, _c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),
     "lsim_sv_probabilities": ([_c_void, _c_int, _c_int, _c_ll, _c_void, _c_void], _c_int),
     "lsim_reduce_workspace_bytes": ([_c_int, _c_ll, _c_int], ctypes.c_size_t),

@@ -8,10 +8,10 @@ reset; an if/else contributes the draws of the arm it takes (engines.py:159-205)
 When every if/else has arms with the same number of draws, each shot consumes a constant number D
 of draws and shot k reads stream positions [kD, (k+1)D). This module then draws
 ``rng.random(shots * D)`` once (the same stream as the scalar draws, in the same order) and runs
-all shots together in a batch of states ``[B][2^n]`` in HBM (:class:`_Batch`): gates, noise
-branches, projections and conditioned gates are queued as per-shot choices and fused into shared
-passes, and decisions read per-shot reduced density matrices, computed on the device or carried
-on the host. Each decision uses the reference's float64 rule (cumulative sums in the same order),
+all shots together in a batch of states ``[B][2^n]`` in HBM (:class:`_Batch`), where shots with
+equal histories share one state: gates, noise branches, projections and conditioned gates are
+queued as per-shot choices and fused into shared passes, and decisions read per-shot reduced
+density matrices, computed on the device or carried on the host. Each decision uses the reference's float64 rule (cumulative sums in the same order),
 so counts equal the reference's except where a draw falls within rounding of a branch boundary.
 
 Circuits whose arms draw differently are run shot after shot with the same device engine
@@ -300,10 +300,19 @@ class _Batch:
     operators in between stay inside the set or miss it -- and then kept on the host, where each
     queued operator inside the set transforms it, so the following decisions read it without
     another pass. An operator that straddles the set, or a projection or Kraus operator outside it,
-    drops it."""
+    drops it.
+
+    Shots with the same history share one device state, a slot (``slot[b]`` for shot b; slots
+    0..s-1 of the buffer are in use, and every pass covers those only). When the shots of one slot
+    choose differently, the slot's state is first copied to a new slot for each further choice.
+    The device work then follows the number of distinct histories, not the number of shots: at
+    1e-3..1e-2 noise most shots take the same branches for most of the circuit. A slot's state
+    is the state each of its shots would hold on its own: the same operators in the same order."""
 
     def __init__(self, n: int, batch: int, dtype: int, nflags: int, num_clbits: int) -> None:
         self.n, self.b, self.dtype = n, batch, dtype
+        self.slot = np.zeros(batch, dtype=np.int64)  # the slot holding each shot's state
+        self.s = 1  # slots in use
         self._allocate()
         self.wide = num_clbits > 62
         self.clbits = np.zeros(batch, dtype=object if self.wide else np.int64)
@@ -314,36 +323,61 @@ class _Batch:
     # -- device operations -------------------------------------------------------------------
 
     def _allocate(self) -> None:
+        """Room for one state per shot; slot 0 starts as |0...0>."""
         self.buf = torch.empty(self.b << self.n, dtype=_TORCH_DTYPE[self.dtype], device=require_device())
-        nat.call("lsim_sv_set_basis", ptr(self.buf), self.dtype, self.n, self.b, 0, stream_handle())
+        nat.call("lsim_sv_set_basis", ptr(self.buf), self.dtype, self.n, 1, 0, stream_handle())
 
     def run_pass(self, p: Pass) -> None:
-        run_passes(self.buf, self.dtype, self.n, self.b, [p])
+        run_passes(self.buf, self.dtype, self.n, self.s, [p])
 
     def finish(self) -> None:
         torch.cuda.current_stream().synchronize()
 
     def apply_selected(self, kind: str, qubits: tuple[int, ...], table: np.ndarray, sel: np.ndarray) -> None:
-        """Per-state operator: state b gets table[sel[b]] (dense matrices or diagonals), or is
-        left alone when sel[b] < 0."""
+        """Per-slot operator: slot j gets table[sel[j]] (dense matrices or diagonals), or is left
+        alone when sel[j] < 0."""
         if not np.any(sel >= 0):
             return
         dev, (addr,) = upload([table], self.dtype)
         sel_dev = torch.from_numpy(np.ascontiguousarray(sel, dtype=np.int32)).to(self.buf.device)
         fn = "lsim_sv_apply_dense" if kind == "dense" else "lsim_sv_apply_diag"
-        nat.call(fn, ptr(self.buf), self.dtype, self.n, self.b, len(qubits), nat.int_array(qubits), addr, ptr(sel_dev), stream_handle())
+        nat.call(fn, ptr(self.buf), self.dtype, self.n, self.s, len(qubits), nat.int_array(qubits), addr, ptr(sel_dev), stream_handle())
         dev.record_stream(torch.cuda.current_stream())
         sel_dev.record_stream(torch.cuda.current_stream())
 
     def reduced(self, qubits: tuple[int, ...]) -> np.ndarray:
-        """Per-state reduced density matrices [B][2^k][2^k] on ``qubits`` (float64 sums)."""
+        """Per-slot reduced density matrices [s][2^k][2^k] on ``qubits`` (float64 sums)."""
         k = len(qubits)
-        out = torch.empty(self.b << (2 * k), dtype=torch.complex128, device=self.buf.device)
-        ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(self.n, self.b, k))
+        out = torch.empty(self.s << (2 * k), dtype=torch.complex128, device=self.buf.device)
+        ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(self.n, self.s, k))
         nat.call(
-            "lsim_sv_reduced_dm", ptr(self.buf), self.dtype, self.n, self.b, k, nat.int_array(qubits), ptr(out), ptr(ws), stream_handle()
+            "lsim_sv_reduced_dm", ptr(self.buf), self.dtype, self.n, self.s, k, nat.int_array(qubits), ptr(out), ptr(ws), stream_handle()
         )
-        return out.cpu().numpy().reshape(self.b, 1 << k, 1 << k)
+        return out.cpu().numpy().reshape(self.s, 1 << k, 1 << k)
+
+    def copy_slots(self, src: np.ndarray, dst: np.ndarray) -> None:
+        """State dst[j] <- state src[j] (slots in use; no slot is both a source and a target)."""
+        both = torch.from_numpy(np.ascontiguousarray(np.stack([src, dst]), dtype=np.int32)).to(self.buf.device)
+        nat.call("lsim_sv_copy_states", ptr(self.buf), self.dtype, self.n, self.s, len(src), ptr(both[0]), ptr(both[1]), stream_handle())
+        both.record_stream(torch.cuda.current_stream())
+
+    def _to_slots(self, sel: np.ndarray, choices: int) -> np.ndarray:
+        """The per-slot selection for a per-shot one (sel[b] in -1..choices-1). Within a slot, the
+        shots of the lowest choice stay; those of each further choice move to a new slot that
+        starts as a copy of theirs."""
+        key = self.slot * (choices + 1) + (sel.astype(np.int64) + 1)
+        uniq, inverse = np.unique(key, return_inverse=True)
+        owner, pick = uniq // (choices + 1), uniq % (choices + 1) - 1
+        moved = np.flatnonzero(np.r_[False, owner[1:] == owner[:-1]])
+        target = owner.copy()
+        if moved.size:
+            target[moved] = self.s + np.arange(moved.size)
+            self.s += int(moved.size)
+            self.copy_slots(owner[moved], target[moved])
+        self.slot = target[np.asarray(inverse).reshape(-1)]
+        out = np.full(self.s, -1, dtype=np.int64)
+        out[target] = pick
+        return out
 
     # -- the queue ---------------------------------------------------------------------------
 
@@ -387,7 +421,7 @@ class _Batch:
 
     def _pass_seconds(self) -> float:
         """About what one pass over the batch costs on the device (read and write at 5 TB/s)."""
-        return 2.0 * (self.b << self.n) * np.dtype(_NP_DTYPE[self.dtype]).itemsize / 5e12
+        return 2.0 * (self.s << self.n) * np.dtype(_NP_DTYPE[self.dtype]).itemsize / 5e12
 
     def flush(self) -> None:
         pending, self.pending = self.pending, []
@@ -406,7 +440,7 @@ class _Batch:
             # shots whose choices are all the identity are left alone
             sel = np.where(np.all(combos < 0, axis=1)[sel], -1, sel)
         mats, bits = _descending_many(_fuse(qubits, items, combos), tuple(qubits))
-        self.apply_selected("dense", bits, mats, sel)
+        self.apply_selected("dense", bits, mats, self._to_slots(sel, len(mats)))
 
     def density(self, qubits: tuple[int, ...], ahead: tuple[int, ...]) -> np.ndarray:
         """Unnormalised per-shot reduced density matrices on ``qubits`` of the state with every
@@ -424,12 +458,12 @@ class _Batch:
         trace = np.trace(rho, axis1=1, axis2=2).real
         low = trace < _RENORMALISE_BELOW[self.dtype]
         if np.any(low & (trace > 0)) and self.n:
-            # keep unnormalised states away from underflow: rescale those shots to norm 1
+            # keep unnormalised states away from underflow: rescale those slots to norm 1
             fix = low & (trace > 0)
             s = np.where(fix, 1.0 / np.sqrt(np.where(fix, trace, 1.0)), 1.0)
-            self.apply_selected("diag", (0,), np.stack([s, s], axis=1).astype(np.complex128), np.where(fix, np.arange(self.b), -1))
+            self.apply_selected("diag", (0,), np.stack([s, s], axis=1).astype(np.complex128), np.where(fix, np.arange(self.s), -1))
             rho = rho * np.where(fix, 1.0 / np.where(fix, trace, 1.0), 1.0)[:, None, None]
-        self.known = (ahead, rho)
+        self.known = (ahead, rho[self.slot])
         for it in tail:
             self.pending.append(it)
             self._follow(it)

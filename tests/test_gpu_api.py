@@ -229,6 +229,16 @@ def test_reproducibility_and_verification(tmp_path) -> None:  # type: ignore[no-
     assert not ls.verify(a.to_dict() | {"counts": {"00": 1}}, BELL, noise)
 
 
+def test_an_unseeded_run_records_its_seed() -> None:
+    """test_sim.py:269-271: without a seed, run draws one and records it; rerunning with the
+    recorded seed gives the same counts and fingerprint."""
+    noise = ls.NoiseModel.uniform(single=0.01, two=0.03)
+    a = ls.run(BELL, shots=500, noise=noise)
+    assert isinstance(a.seed, int) and 0 <= a.seed < 2**63
+    b = ls.run(BELL, shots=500, seed=a.seed, noise=noise)
+    assert a.counts == b.counts and a.fingerprint == b.fingerprint
+
+
 def test_limits_and_physical_qubits() -> None:
     with pytest.raises(ls.SimulationError, match="up to 26"):
         ls.run("OPENQASM 3; qubit[30] q; h q; t q;", shots=1)

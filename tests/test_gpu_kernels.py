@@ -350,3 +350,23 @@ def test_dense_streaming_and_matrix_core_variants_agree(k: int, dtype: int, extr
     finally:
         lib.lsim_set_dense_mfma(old_m)
         lib.lsim_set_dense_streaming(old_s)
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+def test_copy_states_within_a_batch(dtype: int) -> None:
+    """lsim_sv_copy_states copies whole states to other slots of the batch, bit for bit, leaves
+    every other state alone and skips pairs with an index outside the batch."""
+    n, batch = 11, 7
+    psi = states(n, batch, 21).astype(NPDT[dtype])
+    s = dev(psi.reshape(-1))
+    src = np.array([0, 2, 2, 1, 9], dtype=np.int32)
+    dst = np.array([4, 5, 6, -1, 3], dtype=np.int32)
+    d_src, d_dst = dev(src), dev(dst)
+    nat.call("lsim_sv_copy_states", ptr(s), dtype, n, batch, len(src), ptr(d_src), ptr(d_dst), stream_handle())
+    got = s.cpu().numpy().reshape(batch, -1)
+    want = psi.copy()
+    want[4], want[5], want[6] = psi[0], psi[2], psi[2]
+    assert np.array_equal(got, want)
+    nat.call("lsim_sv_copy_states", ptr(s), dtype, n, batch, 0, 0, 0, stream_handle())
+    with pytest.raises(nat.NativeError, match="count"):
+        nat.call("lsim_sv_copy_states", ptr(s), dtype, n, batch, batch + 1, ptr(d_src), ptr(d_dst), stream_handle())

@@ -449,9 +449,9 @@ def test_server_starts_cheap_then_widest_count_requests_first() -> None:
 
 
 class _HostBatch:
-    """The trajectory batch's four device operations restated on the host (the oracle's
-    tensordot for operators), so its queueing, fusion and kept reduced density matrices can be
-    checked against the reference's sequential rule on the CPU."""
+    """The trajectory batch's five device operations restated on the host (the oracle's
+    tensordot for operators), so its queueing, fusion, kept reduced density matrices and shared
+    states (slots) can be checked against the reference's sequential rule on the CPU."""
 
     @staticmethod
     def make(n: int, nshots: int, nflags: int, num_clbits: int):  # type: ignore[no-untyped-def]
@@ -462,13 +462,14 @@ class _HostBatch:
             reductions = 0
 
             def _allocate(self) -> None:
-                self.states = np.zeros((self.b, 1 << self.n), dtype=np.complex128)
-                self.states[:, 0] = 1.0
+                self.states = np.full((self.b, 1 << self.n), np.nan, dtype=np.complex128)
+                self.states[0] = 0.0
+                self.states[0, 0] = 1.0
 
             def run_pass(self, p) -> None:  # type: ignore[no-untyped-def]
                 Batch.passes += 1
                 m = np.diag(p.matrix) if p.kind == "diag" else p.matrix
-                for b in range(self.b):
+                for b in range(self.s):
                     self.states[b] = orc.apply_sv(self.states[b], m, p.bits, self.n)
 
             def apply_selected(self, kind, qubits, table, sel) -> None:  # type: ignore[no-untyped-def]
@@ -483,12 +484,16 @@ class _HostBatch:
             def reduced(self, qubits):  # type: ignore[no-untyped-def]
                 Batch.reductions += 1
                 k = len(qubits)
-                out = np.zeros((self.b, 1 << k, 1 << k), dtype=np.complex128)
+                out = np.zeros((self.s, 1 << k, 1 << k), dtype=np.complex128)
                 axes = [self.n - 1 - q for q in qubits]
-                for b in range(self.b):
+                for b in range(self.s):
                     t = np.moveaxis(self.states[b].reshape([2] * self.n), axes, list(range(k))).reshape(1 << k, -1)
                     out[b] = t @ t.conj().T
                 return out
+
+            def copy_slots(self, src, dst) -> None:  # type: ignore[no-untyped-def]
+                assert np.all(dst < self.s) and not set(src.tolist()) & set(dst.tolist())
+                self.states[dst] = self.states[src]
 
             def finish(self) -> None:
                 pass
@@ -504,7 +509,7 @@ def _host_trajectory_counts(circuit, noise, shots: int, seed: int):  # type: ign
     batch = _HostBatch.make(circuit.num_qubits, shots, nflags, circuit.num_clbits)
     bits = tr._run_chunk(steps, circuit, draws, 0, nflags, batch)
     uniq, cnt = np.unique(bits, return_counts=True)
-    return {circuit.format_bits(int(v)): int(c) for v, c in zip(uniq, cnt)}, type(batch)
+    return {circuit.format_bits(int(v)): int(c) for v, c in zip(uniq, cnt)}, batch
 
 
 @pytest.mark.parametrize("which", ["dynamic", "teleport"])
@@ -528,6 +533,8 @@ def test_trajectory_batch_logic_equals_the_reference_rule(which: str) -> None:
     assert got == want
     ops = sum(1 for s in compile_program(circuit, noise)[0] if s.kind in ("gate", "measure", "reset"))
     assert batch.passes < ops
+    # shots with equal histories share states: slots in use stay below the shot count
+    assert batch.s < shots, batch.s
 
 
 def test_search_over_normalised_cumulative_sums_without_dividing_all() -> None:
