@@ -77,7 +77,8 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
 /* Select the dense kernel variant: with tiling on (the default), operators whose targets include
  * low index bits run through LDS-staged tiles; off, every placement uses the direct kernel.
  * Returns the previous setting; a negative argument only reads it. No reference counterpart (a
- * tuning switch). */
+ * tuning switch). This and the next two switches are process-wide atomics: a change applies to
+ * the launches issued after it, from any thread. */
 int lsim_set_dense_tiling(int enabled);
 
 /* Select which complex128 operator widths run on the FP64 matrix cores (v_mfma_f64_16x16x4f64,
@@ -90,8 +91,9 @@ int lsim_set_dense_mfma(int widths);
 
 /* Streaming (non-temporal) loads and stores in the dense, diagonal and scale kernels: 0 never;
  * 1 (the default) unless the launch's states together fit the 256 MiB last-level cache but not
- * the 32 MiB of L2 (cached accesses win there); 2 at every size. The dense kernels use them only for placements whose load instructions cover whole
- * 128-byte lines (every target at bit 3 or above). Returns the previous setting; a negative
+ * the 32 MiB of L2 (cached accesses win there); 2 at every size. The dense kernels use them only
+ * for placements whose load instructions cover whole 128-byte lines (every target at bit 3 or
+ * above). Returns the previous setting; a negative
  * argument only reads it. Results are identical either way. No reference counterpart. */
 int lsim_set_dense_streaming(int enabled);
 

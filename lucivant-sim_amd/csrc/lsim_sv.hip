@@ -10,17 +10,19 @@
 // non-target indices, so for targets at bit 6 and above each load instruction of a wavefront
 // reads 64 consecutive amplitudes; lower targets make the lanes stride, and the partner loads of
 // the same wavefront complete the cache lines.
+#include <atomic>
+
 #include "lsim_internal.h"
 
 namespace lsim {
 
 // bit k set: complex128 k-qubit operators (k = 3..5) run on the matrix cores (k_dense_mfma);
 // for k = 3, 4 only placements with a target below bit 3, where the vector kernel's lanes stride
-static int g_mfma_widths = (1 << 3) | (1 << 4) | (1 << 5);
+static std::atomic<int> g_mfma_widths{(1 << 3) | (1 << 4) | (1 << 5)};
 // streaming (non-temporal) loads and stores: 0 never; 1 (the default) unless the launch's states
 // fit the last-level cache but not the L2s; 2 at every size. The dense kernels also need a
 // placement whose load instructions cover whole lines.
-static int g_dense_streaming = 1;
+static std::atomic<int> g_dense_streaming{1};
 // MI355X: 8 XCDs with 4 MiB of L2 each, and a 256 MiB Infinity Cache (MALL) in front of HBM.
 // Complex128 passes with 40 random 1-5 qubit operators, per pass:
 //   - 20 qubits (16 MiB): 10.6 us streaming, 14.4 us cached. The state sits in the L2s, and a
@@ -33,8 +35,9 @@ constexpr uint64_t kL2Bytes = 32ull << 20;
 constexpr uint64_t kLastLevelCacheBytes = 256ull << 20;
 
 static bool streams(uint64_t bytes) {
-  if (g_dense_streaming == 2) return true;
-  return g_dense_streaming == 1 && (bytes <= kL2Bytes || bytes > kLastLevelCacheBytes);
+  const int mode = g_dense_streaming.load(std::memory_order_relaxed);
+  if (mode == 2) return true;
+  return mode == 1 && (bytes <= kL2Bytes || bytes > kLastLevelCacheBytes);
 }
 
 // loads and stores with the streaming (non-temporal) hint, or plain ones
@@ -526,7 +529,7 @@ static unsigned blocks_for(uint64_t total) {
   return (unsigned)((total + kBlock * 4 - 1) / (kBlock * 4));
 }
 
-static int g_tiling_enabled = 1;
+static std::atomic<int> g_tiling_enabled{1};
 
 }  // namespace lsim
 
@@ -535,24 +538,19 @@ using namespace lsim;
 extern "C" {
 
 int lsim_set_dense_tiling(int enabled) {
-  const int old = g_tiling_enabled;
-  if (enabled >= 0) g_tiling_enabled = enabled != 0;
-  return old;
+  if (enabled >= 0) return g_tiling_enabled.exchange(enabled != 0 ? 1 : 0);
+  return g_tiling_enabled.load();
 }
 
 int lsim_set_dense_mfma(int widths) {
-  const int old = g_mfma_widths;
-  if (widths >= 0) g_mfma_widths = widths & ((1 << 3) | (1 << 4) | (1 << 5));
-  return old;
+  if (widths >= 0) return g_mfma_widths.exchange(widths & ((1 << 3) | (1 << 4) | (1 << 5)));
+  return g_mfma_widths.load();
 }
 
 int lsim_set_dense_streaming(int enabled) {
-  const int old = g_dense_streaming;
-  if (enabled >= 0) g_dense_streaming = enabled > 2 ? 2 : enabled;
-  return old;
+  if (enabled >= 0) return g_dense_streaming.exchange(enabled > 2 ? 2 : enabled);
+  return g_dense_streaming.load();
 }
-
-
 
 int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,
                       void* stream) {
@@ -576,13 +574,13 @@ int lsim_sv_apply_dense(void* states, int dtype, int nbits, long long batch, int
   Targets t;
   if (int rc = make_targets(nbits, k, targets, &t)) return rc;
   Tiling tl;
-  if (k >= 2 && g_tiling_enabled && make_tiling(nbits, k, targets, &tl)) {
+  if (k >= 2 && g_tiling_enabled.load(std::memory_order_relaxed) && make_tiling(nbits, k, targets, &tl)) {
     if (dtype == LSIM_C128)
       return launch_dense_tiled<double>(states, nbits, batch, k, tl, mats, sel, as_stream(stream));
     return launch_dense_tiled<float>(states, nbits, batch, k, tl, mats, sel, as_stream(stream));
   }
   if (dtype == LSIM_C128) {
-    if (k >= 3 && ((g_mfma_widths >> k) & 1) && nbits >= k + 5 && (k == 5 || t.sorted[0] < 3))
+    if (k >= 3 && ((g_mfma_widths.load(std::memory_order_relaxed) >> k) & 1) && nbits >= k + 5 && (k == 5 || t.sorted[0] < 3))
       return launch_dense_mfma(states, nbits, batch, k, t, mats, sel, as_stream(stream));
     return launch_dense<double>(states, nbits, batch, k, t, mats, sel, as_stream(stream));
   }
