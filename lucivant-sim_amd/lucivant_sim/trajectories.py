@@ -309,10 +309,11 @@ class _Batch:
     1e-3..1e-2 noise most shots take the same branches for most of the circuit. A slot's state
     is the state each of its shots would hold on its own: the same operators in the same order."""
 
-    def __init__(self, n: int, batch: int, dtype: int, nflags: int, num_clbits: int) -> None:
+    def __init__(self, n: int, batch: int, dtype: int, nflags: int, num_clbits: int, share: bool = True) -> None:
         self.n, self.b, self.dtype = n, batch, dtype
-        self.slot = np.zeros(batch, dtype=np.int64)  # the slot holding each shot's state
-        self.s = 1  # slots in use
+        # the slot holding each shot's state, and the slots in use (share=False: one per shot)
+        self.slot = np.zeros(batch, dtype=np.int64) if share else np.arange(batch, dtype=np.int64)
+        self.s = 1 if share else batch
         self._allocate()
         self.wide = num_clbits > 62
         self.clbits = np.zeros(batch, dtype=object if self.wide else np.int64)
@@ -323,9 +324,9 @@ class _Batch:
     # -- device operations -------------------------------------------------------------------
 
     def _allocate(self) -> None:
-        """Room for one state per shot; slot 0 starts as |0...0>."""
+        """Room for one state per shot; the slots in use start as |0...0>."""
         self.buf = torch.empty(self.b << self.n, dtype=_TORCH_DTYPE[self.dtype], device=require_device())
-        nat.call("lsim_sv_set_basis", ptr(self.buf), self.dtype, self.n, 1, 0, stream_handle())
+        nat.call("lsim_sv_set_basis", ptr(self.buf), self.dtype, self.n, self.s, 0, stream_handle())
 
     def run_pass(self, p: Pass) -> None:
         run_passes(self.buf, self.dtype, self.n, self.s, [p])
@@ -572,9 +573,11 @@ _PROJECT = [np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 0], [
 _RESET = [np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 1], [0, 0]], dtype=np.complex128)]
 
 
-def _run_chunk(steps: list[_Step], circuit: Circuit, draws: np.ndarray, dtype: int, nflags: int, batch: _Batch | None = None) -> np.ndarray:
+def _run_chunk(
+    steps: list[_Step], circuit: Circuit, draws: np.ndarray, dtype: int, nflags: int, batch: _Batch | None = None, share: bool = True
+) -> np.ndarray:
     nshots = draws.shape[0]
-    st = batch if batch is not None else _Batch(circuit.num_qubits, nshots, dtype, nflags, circuit.num_clbits)
+    st = batch if batch is not None else _Batch(circuit.num_qubits, nshots, dtype, nflags, circuit.num_clbits, share)
     for t, step in enumerate(steps):
         mask = st.active(step.guard)
         if step.kind == "gate":
@@ -626,9 +629,11 @@ def run_batched(
     *,
     dtype: int = nat.C128,
     batch_bytes: int | None = None,
+    share: bool = True,
 ) -> np.ndarray:
     """Classical bits of every shot (int64, or Python ints beyond 62 clbits), all shots batched.
-    Requires draw-balanced if/else arms (see :func:`draw_count`)."""
+    Requires draw-balanced if/else arms (see :func:`draw_count`). ``share=False`` gives every shot
+    its own state from the start (the same results, with more device work)."""
     steps, per_shot, nflags = compile_program(circuit, noise)
     item = np.dtype(_NP_DTYPE[dtype]).itemsize
     limit = batch_bytes if batch_bytes is not None else BATCH_BYTES
@@ -637,7 +642,7 @@ def run_batched(
     for start in range(0, shots, chunk):
         b = min(chunk, shots - start)
         draws = rng.random(b * per_shot).reshape(b, per_shot) if per_shot else np.zeros((b, 0))
-        out.append(_run_chunk(steps, circuit, draws, dtype, nflags))
+        out.append(_run_chunk(steps, circuit, draws, dtype, nflags, share=share))
     return np.concatenate(out) if out else np.zeros(0, dtype=np.int64)
 
 

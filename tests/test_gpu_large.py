@@ -12,6 +12,7 @@ import pytest
 import torch
 
 import lucivant_sim as ls
+from lucivant_sim import _native as nat
 from lucivant_sim import workloads as wl
 from lucivant_sim.engines import DensityMatrix, Statevector
 from lucivant_sim.plan import plan_density, plan_statevector
@@ -108,3 +109,20 @@ def test_20_qubit_trajectory_batch_is_normalised_and_reproducible() -> None:
     a = run_batched(c, wl.noise_c4(), 64, np.random.default_rng(5))
     b = run_batched(c, wl.noise_c4(), 64, np.random.default_rng(5), batch_bytes=16 << 20 << 4)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+def test_shared_trajectory_states_give_the_bits_of_one_state_per_shot(dtype: int) -> None:
+    """Shots with equal histories share one device state, copied where their choices part; every
+    shot's bits equal those of the batch that keeps one state per shot from the start (noisy
+    teleport chain with mid-circuit measurements, and random layers with strong noise)."""
+    from lucivant_sim.trajectories import run_batched
+
+    cases = [
+        (ls.parse(wl.teleport_chain(16, 3)), wl.noise_c4(), 512),
+        (ls.parse(wl.random_layers(12, 8, 77, measure=True)), ls.NoiseModel.uniform(single=0.02, two=0.05, readout=0.02), 300),
+    ]
+    for c, noise, shots in cases:
+        shared = run_batched(c, noise, shots, np.random.default_rng(17), dtype=dtype)
+        alone = run_batched(c, noise, shots, np.random.default_rng(17), dtype=dtype, share=False)
+        assert np.array_equal(shared, alone)

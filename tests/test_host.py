@@ -454,7 +454,7 @@ class _HostBatch:
     states (slots) can be checked against the reference's sequential rule on the CPU."""
 
     @staticmethod
-    def make(n: int, nshots: int, nflags: int, num_clbits: int):  # type: ignore[no-untyped-def]
+    def make(n: int, nshots: int, nflags: int, num_clbits: int, share: bool = True):  # type: ignore[no-untyped-def]
         from lucivant_sim import trajectories as tr
 
         class Batch(tr._Batch):
@@ -463,8 +463,8 @@ class _HostBatch:
 
             def _allocate(self) -> None:
                 self.states = np.full((self.b, 1 << self.n), np.nan, dtype=np.complex128)
-                self.states[0] = 0.0
-                self.states[0, 0] = 1.0
+                self.states[: self.s] = 0.0
+                self.states[: self.s, 0] = 1.0
 
             def run_pass(self, p) -> None:  # type: ignore[no-untyped-def]
                 Batch.passes += 1
@@ -498,15 +498,15 @@ class _HostBatch:
             def finish(self) -> None:
                 pass
 
-        return Batch(n, nshots, 0, nflags, num_clbits)
+        return Batch(n, nshots, 0, nflags, num_clbits, share)
 
 
-def _host_trajectory_counts(circuit, noise, shots: int, seed: int):  # type: ignore[no-untyped-def]
+def _host_trajectory_counts(circuit, noise, shots: int, seed: int, share: bool = True):  # type: ignore[no-untyped-def]
     from lucivant_sim import trajectories as tr
 
     steps, per_shot, nflags = tr.compile_program(circuit, noise)
     draws = np.random.default_rng(seed).random(shots * per_shot).reshape(shots, per_shot)
-    batch = _HostBatch.make(circuit.num_qubits, shots, nflags, circuit.num_clbits)
+    batch = _HostBatch.make(circuit.num_qubits, shots, nflags, circuit.num_clbits, share)
     bits = tr._run_chunk(steps, circuit, draws, 0, nflags, batch)
     uniq, cnt = np.unique(bits, return_counts=True)
     return {circuit.format_bits(int(v)): int(c) for v, c in zip(uniq, cnt)}, batch
@@ -533,8 +533,11 @@ def test_trajectory_batch_logic_equals_the_reference_rule(which: str) -> None:
     assert got == want
     ops = sum(1 for s in compile_program(circuit, noise)[0] if s.kind in ("gate", "measure", "reset"))
     assert batch.passes < ops
-    # shots with equal histories share states: slots in use stay below the shot count
+    # shots with equal histories share states: slots in use stay below the shot count, and the
+    # counts equal those of one state per shot
     assert batch.s < shots, batch.s
+    alone, _ = _host_trajectory_counts(circuit, noise, shots, 31, share=False)
+    assert alone == got
 
 
 def test_search_over_normalised_cumulative_sums_without_dividing_all() -> None:
