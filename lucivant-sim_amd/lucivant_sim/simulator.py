@@ -503,27 +503,49 @@ def outcome_probabilities(
         return {}
     state = _exact_state(circuit, noise)
     measured = sorted({q for q, _ in pairs})
-    position = {q: i for i, q in enumerate(measured)}
     dist = marginal(state.probabilities(), circuit.num_qubits, measured)
-    out: dict[int, float] = {}
-    for index, p in enumerate(dist):
-        if p <= threshold:
-            continue
-        branches = [(0, float(p))]
-        for q, c in pairs:
-            bit = (index >> position[q]) & 1
-            err = noise.readout_error(q) if noise else None
-            flip = (err.p10 if bit else err.p01) if err is not None else 0.0
-            nxt = []
-            for bits, w in branches:
-                if flip < 1:
-                    nxt.append((bits | (bit << c), w * (1 - flip)))
-                if flip > 0:
-                    nxt.append((bits | ((bit ^ 1) << c), w * flip))
-            branches = nxt
-        for bits, w in branches:
-            out[bits] = out.get(bits, 0.0) + w
-    return {circuit.format_bits(b): w for b, w in sorted(out.items()) if w > threshold}
+    recorded = _recorded_distribution(dist, measured, pairs, noise, threshold)
+    return {circuit.format_bits(bits): w for bits, w in recorded}
+
+
+def _recorded_distribution(
+    dist: NDArray[np.float64], measured: list[int], pairs: list[tuple[int, int]], noise: NoiseModel | None, threshold: float
+) -> list[tuple[int, float]]:
+    """(bits, probability) of every recorded outcome above ``threshold``, in increasing bits, for
+    the marginal ``dist`` over ``measured`` (index bit i = measured[i]) and the final (qubit,
+    clbit) ``pairs`` (simulator.py:467-486). Each recorded bit is its qubit's bit, flipped with
+    probability p01 (a 0) or p10 (a 1) of that qubit's readout error, independently of the other
+    recorded bits; entries of ``dist`` at or below ``threshold`` are left out first.
+
+    One contraction per measured qubit: its axis of the [2]*m distribution is replaced by one
+    axis per clbit it is recorded in, through the 2 x 2 flip matrix of its readout error, so the
+    cost is O(m 2^m) instead of a loop over every outcome and flip pattern."""
+    m = len(measured)
+    w = np.where(dist > threshold, dist, 0.0).reshape([2] * m)
+    clbits: list[int] = []  # the clbit of each read axis, in the order they are appended
+    for q in reversed(measured):  # C-order axis 0 is the highest index bit: measured[m-1] first
+        err = noise.readout_error(q) if noise is not None else None
+        flip = np.eye(2) if err is None else np.array([[1.0 - err.p01, err.p01], [err.p10, 1.0 - err.p10]])
+        mine = [c for qq, c in pairs if qq == q]
+        factor = np.ones(2)
+        for j, _ in enumerate(mine):
+            factor = factor[..., None] * flip.reshape((2,) + (1,) * j + (2,))
+        w = np.tensordot(w, factor, axes=([0], [0]))
+        clbits += mine
+    # order the read axes by clbit, highest first, so the flat index packs them in increasing order
+    order = sorted(range(len(clbits)), key=lambda a: -clbits[a])
+    flat = np.transpose(w, order).reshape(-1) if clbits else w.reshape(-1)
+    ascending = sorted(clbits)
+    keep = np.flatnonzero(flat > threshold)
+    if ascending and ascending[-1] < 63:
+        bits = np.zeros(keep.size, dtype=np.int64)
+        for i, c in enumerate(ascending):
+            bits |= ((keep >> i) & 1) << c
+        return list(zip(bits.tolist(), flat[keep].tolist()))
+    out = []
+    for x, v in zip(keep.tolist(), flat[keep].tolist()):
+        out.append((sum(((x >> i) & 1) << c for i, c in enumerate(ascending)), v))
+    return out
 
 
 def expectation(

@@ -5,7 +5,8 @@ Follows lucivant-sim 0.3.0:
 - trajectory channel choice, measurement, collapse, reset: engines.py:159-205;
 - density matrix, channels, analytic depolarising: engines.py:122-135, 215-264;
 - marginal: engines.py:267-275;
-- static evolution, sampling, trajectories, expectation: simulator.py:229-317, 489-533.
+- static evolution, sampling, trajectories, expectation: simulator.py:229-317, 489-533;
+- recorded outcome probabilities with readout error: simulator.py:467-486.
 
 The circuit comes in as lucivant_sim's Circuit IR (plain host data: gates already resolved to
 matrices). Every state update here is written independently of the device engine: tensordot on
@@ -257,6 +258,33 @@ class _Shot:
         if p > 0:
             self.psi = self.psi / np.sqrt(p)
         return o
+
+
+def recorded_distribution(
+    dist: np.ndarray, measured: list[int], pairs: list[tuple[int, int]], readout: dict[int, tuple[float, float]], threshold: float = 1e-12
+) -> dict[int, float]:
+    """Probabilities of the recorded clbits with readout flips (simulator.py:467-486), from the
+    definition rather than the reference's branch expansion: for every true index t of ``dist``
+    (bit i = measured[i]) above ``threshold`` and every read pattern r over ``pairs``, the weight
+    is dist[t] times, per pair (q, c), 1 - f or f as r_c equals q's true bit or not, where f is
+    p01 for a true 0 and p10 for a true 1 of q's readout error (``readout``: qubit -> (p01, p10);
+    no entry: no flips). Outcomes at or below ``threshold`` are dropped. Brute force over
+    2^m x 2^len(pairs) patterns: small cases only."""
+    out: dict[int, float] = {}
+    for t, p in enumerate(dist):
+        if p <= threshold:
+            continue
+        truth = {q: (t >> i) & 1 for i, q in enumerate(measured)}
+        for r in itertools.product((0, 1), repeat=len(pairs)):
+            w = float(p)
+            bits = 0
+            for (q, c), read in zip(pairs, r):
+                p01, p10 = readout.get(q, (0.0, 0.0))
+                f = p10 if truth[q] else p01
+                w *= f if read != truth[q] else 1.0 - f
+                bits |= read << c
+            out[bits] = out.get(bits, 0.0) + w
+    return {b: w for b, w in sorted(out.items()) if w > threshold}
 
 
 def trajectory_counts(circuit, noise, shots: int, rng: np.random.Generator) -> dict[str, int]:  # type: ignore[no-untyped-def]

@@ -557,3 +557,31 @@ def test_search_over_normalised_cumulative_sums_without_dividing_all() -> None:
         u[:100] = normalised[rng.integers(0, len(cum), 100)]
         u[100] = 0.0
         assert np.array_equal(_search_normalised(cum, u), np.searchsorted(normalised, u, side="right")), n
+
+
+def test_recorded_outcome_distribution_equals_the_definition() -> None:
+    """outcome_probabilities' readout expansion (one contraction per measured qubit) gives the
+    oracle's brute-force definition: a qubit recorded in two clbits, clbits never written, qubits
+    with and without readout error, p01 != p10, and marginal entries at the threshold dropped."""
+    from lucivant_sim.simulator import _recorded_distribution
+
+    rng = np.random.default_rng(12)
+    measured = [0, 2, 3, 5]
+    pairs = [(3, 0), (0, 2), (5, 1), (3, 4), (2, 6)]
+    noise = ls.NoiseModel().add_readout_error(ls.ReadoutError(p01=0.05, p10=0.2), qubits=[3]).add_readout_error(ls.ReadoutError(p01=0.3, p10=0.01), qubits=[5])
+    readout = {3: (0.05, 0.2), 5: (0.3, 0.01)}
+    dist = rng.random(16) ** 3
+    dist[[1, 6]] = 0.0
+    dist[9] = 1e-13
+    dist /= dist.sum()
+    got = _recorded_distribution(dist, measured, pairs, noise, 1e-12)
+    want = orc.recorded_distribution(dist, measured, pairs, readout, 1e-12)
+    assert [b for b, _ in got] == list(want)
+    assert np.allclose([w for _, w in got], list(want.values()), rtol=1e-13, atol=0)
+    # no readout error: the recorded bits are the measured bits, moved to their clbits
+    plain = _recorded_distribution(dist, measured, pairs, None, 1e-12)
+    assert dict(plain) == pytest.approx(orc.recorded_distribution(dist, measured, pairs, {}, 1e-12), rel=1e-14)
+    # a uniform symmetric error on every qubit, and a single measured qubit
+    sym = ls.NoiseModel().add_readout_error(0.1)
+    got1 = dict(_recorded_distribution(np.array([0.25, 0.75]), [4], [(4, 1)], sym, 1e-12))
+    assert got1 == pytest.approx({0: 0.25 * 0.9 + 0.75 * 0.1, 2: 0.25 * 0.1 + 0.75 * 0.9}, rel=1e-15)

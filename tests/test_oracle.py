@@ -188,3 +188,20 @@ def test_oracle_trajectories_agree_with_the_oracle_density_matrix() -> None:
     exact = {format(i, "02b"): float(v) for i, v in enumerate(p) if v > 1e-12}
     traj = orc.trajectory_counts(bell, noise, 4000, np.random.default_rng(9))
     assert orc.tvd(traj, exact) < 0.03
+
+
+def test_recorded_distribution_known_answers() -> None:
+    """The oracle's readout expansion: closed forms for one qubit (p01 applies to a true 0, p10 to
+    a true 1), a fan-out into two clbits (both reads flip independently), a pure relabelling of
+    bits without error, and total probability kept."""
+    one = orc.recorded_distribution(np.array([0.4, 0.6]), [2], [(2, 0)], {2: (0.1, 0.3)})
+    assert one == pytest.approx({0: 0.4 * 0.9 + 0.6 * 0.3, 1: 0.4 * 0.1 + 0.6 * 0.7}, rel=1e-15)
+    fan = orc.recorded_distribution(np.array([1.0, 0.0]), [0], [(0, 0), (0, 3)], {0: (0.1, 0.0)})
+    assert fan == pytest.approx({0: 0.81, 1: 0.09, 8: 0.09, 9: 0.01}, rel=1e-14)
+    moved = orc.recorded_distribution(np.array([0.1, 0.2, 0.3, 0.4]), [1, 4], [(4, 0), (1, 5)], {})
+    assert moved == pytest.approx({0: 0.1, 32: 0.2, 1: 0.3, 33: 0.4}, rel=1e-15)
+    rng = np.random.default_rng(3)
+    d = rng.random(8)
+    d /= d.sum()
+    total = orc.recorded_distribution(d, [0, 1, 2], [(0, 1), (1, 0), (2, 2), (2, 3)], {0: (0.2, 0.05), 2: (0.01, 0.3)}, threshold=-1.0)
+    assert sum(total.values()) == pytest.approx(1.0, rel=1e-14)
