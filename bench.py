@@ -273,6 +273,7 @@ def bench_c2_outputs() -> dict:
         "probabilities_entries": entries,
         "probability_sum_minus_1": float(f"{total - 1:.3g}"),
         "expectation_8_calls_s": round(each_s, 3),
+        "expectation_note": "every call simulates the whole circuit again, as the reference's does; the text is read and planned once and reused by later calls (recent-program cache)",
         "expectation_one_weighted_call_s": round(one_s, 4),
         "expectation_values": [round(v, 12) for v in values],
         "weighted_equals_sum_of_calls": linear,

@@ -18,6 +18,7 @@ import os
 import secrets
 import threading
 import time
+from collections import OrderedDict
 from collections.abc import Mapping, Sequence
 from dataclasses import dataclass, field
 from pathlib import Path
@@ -63,6 +64,15 @@ FUSION_WIDTH = 5
 Program = str | os.PathLike[str] | Circuit | Any
 
 
+#: The circuits of recently read OpenQASM text and the statevector plans of recently simulated
+#: circuits (least recently used dropped first). Repeated calls on one program, such as several
+#: expectation() calls, read and plan it once; every call still runs the whole simulation.
+_RECENT = 8
+_READ: OrderedDict[tuple[str, tuple[tuple[str, float], ...]], Circuit] = OrderedDict()
+_PLANS: OrderedDict[tuple[int, int, int, int], tuple[Circuit, list[Pass]]] = OrderedDict()
+_RECENT_LOCK = threading.Lock()
+
+
 def to_circuit(program: Program, params: Mapping[str, float] | None = None) -> Circuit:
     """A Circuit from OpenQASM text, a .qasm path, or an already built Circuit."""
     if isinstance(program, Circuit):
@@ -72,7 +82,18 @@ def to_circuit(program: Program, params: Mapping[str, float] | None = None) -> C
     if isinstance(program, os.PathLike):
         return load(Path(program), params)
     if isinstance(program, str):
-        return parse(program, params)
+        key = (program, tuple(sorted((str(k), float(v)) for k, v in (params or {}).items())))
+        with _RECENT_LOCK:
+            circuit = _READ.get(key)
+            if circuit is not None:
+                _READ.move_to_end(key)
+                return circuit
+        circuit = parse(program, params)  # a Circuit is immutable, so callers may share it
+        with _RECENT_LOCK:
+            _READ[key] = circuit
+            while len(_READ) > _RECENT:
+                _READ.popitem(last=False)
+        return circuit
     module = type(program).__module__
     if module.startswith("qiskit"):
         from qiskit import qasm3  # type: ignore[import-not-found]
@@ -191,15 +212,29 @@ def choose_method(circuit: Circuit, noise: NoiseModel | None, method: str = "aut
     return method
 
 
+def _statevector_plan(circuit: Circuit, width: int, low_bits: int, low_width: int) -> list[Pass]:
+    """The fused passes of ``circuit``, planned once per recently simulated circuit (an entry
+    holds the circuit itself, so an identity match cannot confuse two circuits)."""
+    key = (id(circuit), width, low_bits, low_width)
+    with _RECENT_LOCK:
+        hit = _PLANS.get(key)
+        if hit is not None and hit[0] is circuit:
+            _PLANS.move_to_end(key)
+            return hit[1]
+    passes = plan_statevector(circuit.operations, circuit.num_qubits, width, low_bits, low_width)
+    with _RECENT_LOCK:
+        _PLANS[key] = (circuit, passes)
+        while len(_PLANS) > _RECENT:
+            _PLANS.popitem(last=False)
+    return passes
+
+
 def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseModel | None, width: int | None = None) -> None:
     """Static evolution (simulator.py:229-251): gates, their noise, resets of untouched qubits;
     measurements deferred. Runs as fused device passes."""
     if isinstance(state, Statevector):
-        if width:
-            state.run(plan_statevector(circuit.operations, circuit.num_qubits, width))
-        else:
-            w, low_bits, low_width = _tuned(circuit.num_qubits)
-            state.run(plan_statevector(circuit.operations, circuit.num_qubits, w, low_bits, low_width))
+        w, low_bits, low_width = (width, 0, 0) if width else _tuned(circuit.num_qubits)
+        state.run(_statevector_plan(circuit, w, low_bits, low_width))
         return
     group: list[Pass] = []
     for item in plan_density(circuit.operations, circuit.num_qubits, noise):

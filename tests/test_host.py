@@ -585,3 +585,24 @@ def test_recorded_outcome_distribution_equals_the_definition() -> None:
     sym = ls.NoiseModel().add_readout_error(0.1)
     got1 = dict(_recorded_distribution(np.array([0.25, 0.75]), [4], [(4, 1)], sym, 1e-12))
     assert got1 == pytest.approx({0: 0.25 * 0.9 + 0.75 * 0.1, 2: 0.25 * 0.1 + 0.75 * 0.9}, rel=1e-15)
+
+
+def test_repeated_programs_are_read_and_planned_once() -> None:
+    """The same OpenQASM text with the same parameters reads to the same (immutable) Circuit, and
+    a circuit is planned once per fusion setting; other text or parameters read anew, and only
+    the most recent entries are kept."""
+    from lucivant_sim import simulator as sim
+
+    src = 'OPENQASM 3; include "stdgates.inc"; input float t; qubit[3] q; rx(t) q[0]; cx q[0], q[2];'
+    a = sim.to_circuit(src, {"t": 0.5})
+    assert sim.to_circuit(src, {"t": 0.5}) is a
+    assert sim.to_circuit(src, {"t": 0.25}) is not a
+    p = sim._statevector_plan(a, 5, 0, 0)
+    assert sim._statevector_plan(a, 5, 0, 0) is p
+    assert sim._statevector_plan(a, 4, 0, 0) is not p
+    want = plan_statevector(a.operations, 3, 5)
+    assert len(p) == len(want) and all(np.array_equal(x.matrix, y.matrix) and x.bits == y.bits for x, y in zip(p, want))
+    for i in range(sim._RECENT + 2):
+        sim.to_circuit(f"OPENQASM 3; qubit[{i + 1}] q; x q[0];")
+    assert len(sim._READ) == sim._RECENT
+    assert sim.to_circuit(src, {"t": 0.5}) is not a  # dropped as least recently used
