@@ -448,13 +448,12 @@ static int launch_dense_tiled(void* states, int nbits, long long batch, int k, c
 static int launch_dense_mfma(void* states, int nbits, long long batch, int k, const Targets& t,
                              const void* mats, const int* sel, hipStream_t s) {
   const uint32_t batches = 1u << (nbits - k - 4);  // 16 groups of 2^k amplitudes per wavefront step
-  // two steps per iteration; about 8192 wavefronts in all (enough to fill 256 CUs several times
-  // over), shared among the states of a batch: each wavefront loads its operator fragments once
-  // (16 KiB at k = 5), so a batch of many states gets fewer wavefronts per state, each striding
-  // over more of its state
-  const uint32_t per_state = batch >= 8192 ? 1u : (uint32_t)(8192 / batch);
-  const uint32_t cap = batches / 2 < per_state ? batches / 2 : per_state;
-  const uint32_t waves = cap > 0 ? cap : 1u;
+  // one iteration (two steps) per wavefront: 2^(nbits-k-5) wavefronts per state. Measured at 26
+  // qubits and k = 5 against fewer wavefronts that stride over the state: 8192 -> 394 us per pass,
+  // 16384 -> 389, 32768 -> 384, one iteration each (65536) -> 374, and 416 -> 390 us where two
+  // or more targets are at bit 20 and above. Each wavefront loads its operator fragments (16 KiB
+  // at k = 5) from cache, against 32 KiB of state it reads and writes.
+  const uint32_t waves = batches / 2 > 0 ? batches / 2 : 1u;
   const uint32_t gx = (waves + 3) / 4;
   // Streaming accesses (sizes as for streams()) when each load instruction covers whole 128-byte
   // lines (index bits 0-2):

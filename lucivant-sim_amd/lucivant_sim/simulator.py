@@ -57,7 +57,7 @@ MAX_SHOTS = 10_000_000
 ENGINE = "lucivant-sim-mi355x"
 
 #: Statevector fusion width (qubits per dense pass) when the autotuner has stored no choice for
-#: this device. 5 measured best on gfx950 (26 qubits: 53 ms per simulation at width 5, 67 ms at
+#: this device. 5 measured best on gfx950 (26 qubits: 52-53 ms per simulation at width 5, 67 ms at
 #: 4, 100 ms at 3, with merged blocks and 5-qubit operators on the matrix cores).
 FUSION_WIDTH = 5
 
