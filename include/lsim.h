@@ -83,8 +83,8 @@ int lsim_set_dense_tiling(int enabled);
 
 /* Select which complex128 operator widths run on the FP64 matrix cores (v_mfma_f64_16x16x4f64,
  * 16 groups per wavefront step) instead of the vector kernel: bit k of `widths` for k = 3, 4, 5
- * (states of at least k + 5 bits; other bits are ignored). 3- and 4-qubit operators take the
- * matrix cores only when a target is below bit 3. The default is all three bits. Returns the
+ * (states of at least k + 5 bits; other bits are ignored), on every placement of the targets.
+ * The default is all three bits. Returns the
  * previous mask; a negative argument only reads it. Results agree with the vector kernel to
  * rounding. No reference counterpart (a tuning switch the autotuner sets). */
 int lsim_set_dense_mfma(int widths);
