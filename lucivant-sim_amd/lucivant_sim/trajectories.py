@@ -225,7 +225,7 @@ def _combinations(columns: list[np.ndarray]) -> tuple[np.ndarray, np.ndarray]:
 _FUSE_SECONDS_PER_D3 = 0.7e-9
 
 
-def _split_for_host(items: list[_Queued], k: int, shots: int, pass_seconds: float) -> list[list[_Queued]]:
+def _split_for_host(items: list[_Queued], k: int, pass_seconds: float) -> list[list[_Queued]]:
     """A block's operators in consecutive parts, each one pass, so that fusing a part on the host
     costs no more than a pass on the device: with many per-shot choices (strong noise) every shot
     may need its own fused operator, and one more pass is then cheaper than the host products."""
@@ -427,7 +427,7 @@ class _Batch:
     def flush(self) -> None:
         pending, self.pending = self.pending, []
         for qubits, items in _merge_groups(_group(pending, FUSE_WIDTH), FUSE_WIDTH):
-            for part in _split_for_host(items, len(qubits), self.b, self._pass_seconds()):
+            for part in _split_for_host(items, len(qubits), self._pass_seconds()):
                 self._apply_block(qubits, part)
 
     def _apply_block(self, qubits: list[int], items: list[_Queued]) -> None:

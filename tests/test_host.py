@@ -420,9 +420,9 @@ def test_blocks_split_when_host_fusion_would_outcost_a_pass() -> None:
     rng = np.random.default_rng(4)
     items = [_Queued((i % 4,), [np.eye(2)] * 4, rng.integers(-1, 4, size=4096)) for i in range(8)]
     items.insert(3, _Queued((0, 1), [np.eye(4)]))
-    small = _split_for_host(items, 4, 4096, pass_seconds=1e-5)
+    small = _split_for_host(items, 4, pass_seconds=1e-5)
     assert len(small) > 1 and [it for part in small for it in part] == items
-    large = _split_for_host(items, 4, 4096, pass_seconds=10.0)
+    large = _split_for_host(items, 4, pass_seconds=10.0)
     assert large == [items]
 
 
