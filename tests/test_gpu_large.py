@@ -126,3 +126,20 @@ def test_shared_trajectory_states_give_the_bits_of_one_state_per_shot(dtype: int
         shared = run_batched(c, noise, shots, np.random.default_rng(17), dtype=dtype)
         alone = run_batched(c, noise, shots, np.random.default_rng(17), dtype=dtype, share=False)
         assert np.array_equal(shared, alone)
+
+
+def test_trajectory_rescale_of_small_norms_keeps_the_bits(monkeypatch) -> None:  # type: ignore[no-untyped-def]
+    """Raising the rescale threshold so that the device batch rescales shared states at nearly
+    every decision leaves every shot's bits as they were (dynamic circuit with noise)."""
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).parent))
+    from dynamic_circuits import DYNAMIC, dynamic_noise
+    from lucivant_sim import trajectories as tr
+
+    c = ls.parse(DYNAMIC)
+    want = tr.run_batched(c, dynamic_noise(), 400, np.random.default_rng(19))
+    monkeypatch.setattr(tr, "_RENORMALISE_BELOW", {nat.C128: 0.999, nat.C64: 0.999})
+    got = tr.run_batched(c, dynamic_noise(), 400, np.random.default_rng(19))
+    assert np.array_equal(got, want)

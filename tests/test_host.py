@@ -606,3 +606,17 @@ def test_repeated_programs_are_read_and_planned_once() -> None:
         sim.to_circuit(f"OPENQASM 3; qubit[{i + 1}] q; x q[0];")
     assert len(sim._READ) == sim._RECENT
     assert sim.to_circuit(src, {"t": 0.5}) is not a  # dropped as least recently used
+
+
+def test_trajectory_batch_rescales_small_norms_per_slot(monkeypatch) -> None:  # type: ignore[no-untyped-def]
+    """With the rescale threshold raised so that every decision rescales the states whose squared
+    norm fell below it (one diagonal pass over the slots that need it), the host-state batch still
+    gives the oracle's counts for the same draws."""
+    from dynamic_circuits import DYNAMIC, dynamic_noise
+    from lucivant_sim import trajectories as tr
+
+    monkeypatch.setattr(tr, "_RENORMALISE_BELOW", {0: 0.999, 1: 0.999})
+    circuit = ls.parse(DYNAMIC)
+    noise = dynamic_noise()
+    got, _ = _host_trajectory_counts(circuit, noise, 120, 8)
+    assert got == orc.trajectory_counts(circuit, noise, 120, np.random.default_rng(8))
