@@ -504,11 +504,13 @@ def serve_stream(server: Server, requests: list[Request], arrivals: list[float])
 
 def plain_stream(requests: list[Request], arrivals: list[float]) -> dict[str, float]:
     """The same stream through the plain path: one request at a time, one launch per gate
-    (``lucivant_sim.run`` / ``expectation`` with fusion width 1), no batching or overlap."""
+    (``lucivant_sim.run`` / ``expectation`` with fusion width 1 at every size, whatever the
+    autotuner stored), no batching or overlap."""
     from lucivant_sim import simulator
 
-    old = simulator.FUSION_WIDTH
+    old, old_tuned = simulator.FUSION_WIDTH, simulator._tuned
     simulator.FUSION_WIDTH = 1
+    simulator._tuned = lambda num_qubits: (1, 0, 0)
     t0 = time.perf_counter()
     lat = []
     try:
@@ -525,7 +527,7 @@ def plain_stream(requests: list[Request], arrivals: list[float]) -> dict[str, fl
             req.done = time.perf_counter()
             lat.append(req.done - arrival)
     finally:
-        simulator.FUSION_WIDTH = old
+        simulator.FUSION_WIDTH, simulator._tuned = old, old_tuned
     arr = np.array(lat)
     span = time.perf_counter() - t0
     return {
