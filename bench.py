@@ -190,7 +190,9 @@ def bench_serving(qps: float, count: int = 48, seed: int = 1005) -> dict:
     10..100; half hardware-efficient ansatz expectation requests (16-term Pauli sums, weights in
     [-1, 1]), half random u3 + cx count requests (1000 shots); count requests of at most 12
     qubits carry the uniform noise model (1e-3 / 1e-2) and run as trajectories (wider noisy
-    requests would be 1000 full simulations each, so noise is capped there)."""
+    requests would be 1000 full simulations each, so noise is capped there). The plain stream
+    serves the same requests one at a time with one kernel launch per gate at every width
+    (``serving.plain_stream``)."""
     import numpy as np
 
     from lucivant_sim import workloads
