@@ -183,13 +183,17 @@ int lsim_program_max_qubits(int dtype);
  * with draws from draws[draw_start[b] ..] (draws/draw_start may be NULL when no op draws).
  * Optional outputs (NULL to skip): clbits[b]; probs[b][2^n] = |psi|^2; state[b][2^n] in dtype;
  * expect[b][t] = <psi_b|P_bt|psi_b> for the nterms Pauli strings of each instance, given by the
- * DEVICE arrays pauli_x[b][t] / pauli_z[b][t] (bit masks as for lsim_sv_pauli_expect). */
+ * DEVICE arrays pauli_x[b][t] / pauli_z[b][t] (bit masks as for lsim_sv_pauli_expect);
+ * decisions, laid out as the draws with two doubles per draw: for the draw of each MEASURE or
+ * RESET, {p1, outcome}, and of each KRAUS, {running weight at the branch taken, branch index}
+ * (entries of other draws are left as they are). The record states the error of a complex64
+ * run against complex128 per decision; it has no reference counterpart. */
 int lsim_program_run(int dtype, int n, long long instances, const lsim_prog_op* ops,
                      const int* op_start, const int* op_count, const void* mats,
                      const double* reals, const double* draws, const int* draw_start,
                      unsigned long long* clbits, double* probs, void* state, int nterms,
                      const unsigned long long* pauli_x, const unsigned long long* pauli_z,
-                     double* expect, void* stream);
+                     double* expect, double* decisions, void* stream);
 
 #ifdef __cplusplus
 }

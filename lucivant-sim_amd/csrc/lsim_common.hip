@@ -110,7 +110,7 @@ extern "C" {
 
 const char* lsim_last_error(void) { return lsim::g_error; }
 
-int lsim_abi_version(void) { return 1; }
+int lsim_abi_version(void) { return 2; }
 
 // A plain sequential loop: the same float64 additions in the same order as NumPy's accumulate
 // (out[0] = in[0], then one addition per element), which is all bit-exact sampling needs, at

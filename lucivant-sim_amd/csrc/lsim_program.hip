@@ -239,7 +239,8 @@ __global__ void __launch_bounds__(kProgThreads)
               const int* __restrict__ draw_start, unsigned long long* __restrict__ clbits_out,
               double* __restrict__ probs_out, typename Cx<R>::V* __restrict__ state_out,
               int nterms, const unsigned long long* __restrict__ pauli_x,
-              const unsigned long long* __restrict__ pauli_z, double* __restrict__ expect_out) {
+              const unsigned long long* __restrict__ pauli_z, double* __restrict__ expect_out,
+              double* __restrict__ decisions) {
   using V = typename Cx<R>::V;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   ProgShared& sh = *reinterpret_cast<ProgShared*>(lds_raw);
@@ -256,6 +257,9 @@ __global__ void __launch_bounds__(kProgThreads)
   const lsim_prog_op* prog = ops + op_start[b];
   const int count = op_count[b];
   const double* dr = draws != nullptr ? draws + draw_start[b] : nullptr;
+  // optional record of every state-dependent decision, laid out as the draws (two doubles per
+  // draw column: the probability the draw was compared with, and the branch taken)
+  double* rec = decisions != nullptr ? decisions + 2 * (size_t)draw_start[b] : nullptr;
   unsigned long long clbits = 0;
   uint32_t flags = 0;
   for (int p = 0; p < count; ++p) {
@@ -331,6 +335,10 @@ __global__ void __launch_bounds__(kProgThreads)
           }
           if (pick < 0 && w > 0) pick = op.nb - 1;
           sh.choice = pick;
+          if (rec != nullptr) {
+            rec[2 * op.draw] = acc;
+            rec[2 * op.draw + 1] = pick;
+          }
           if (pick >= 0) {
             const double norm = sqrt(w);
             const double2* K = mats + op.mat + pick * D * D;
@@ -353,6 +361,10 @@ __global__ void __launch_bounds__(kProgThreads)
         wg_sum<1>(v, sh.red);
         const double p1 = fmin(fmax(v[0], 0.0), 1.0);
         const int outcome = dr[op.draw] < p1 ? 1 : 0;
+        if (rec != nullptr && threadIdx.x == 0) {
+          rec[2 * op.draw] = p1;
+          rec[2 * op.draw + 1] = outcome;
+        }
         const double p = outcome ? p1 : 1.0 - p1;
         const double root = sqrt(p);
         for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
@@ -463,7 +475,7 @@ int lsim_program_run(int dtype, int n, long long instances, const lsim_prog_op* 
                      const double* reals, const double* draws, const int* draw_start,
                      unsigned long long* clbits, double* probs, void* state, int nterms,
                      const unsigned long long* pauli_x, const unsigned long long* pauli_z,
-                     double* expect, void* stream) {
+                     double* expect, double* decisions, void* stream) {
   LSIM_REQUIRE(dtype == LSIM_C128 || dtype == LSIM_C64, "dtype must be LSIM_C128 or LSIM_C64");
   LSIM_REQUIRE(n >= 1 && n <= 16, "program states have 1 to 16 qubits, not %d", n);
   LSIM_REQUIRE(instances >= 1 && instances <= 0x7fffffff, "instances must be positive");
@@ -472,6 +484,8 @@ int lsim_program_run(int dtype, int n, long long instances, const lsim_prog_op* 
   LSIM_REQUIRE(nterms >= 0 && nterms <= LSIM_MAX_TERMS, "nterms must be between 0 and %d",
                LSIM_MAX_TERMS);
   LSIM_REQUIRE(nterms == 0 || (pauli_x && pauli_z && expect), "Pauli arrays are NULL");
+  LSIM_REQUIRE(decisions == nullptr || (draws && draw_start),
+               "decisions are recorded per draw: give the draws too");
   const size_t bytes = dtype == LSIM_C64 ? lds_bytes<float>(n) : lds_bytes<double>(n);
   hipStream_t s = as_stream(stream);
   if (dtype == LSIM_C128) {
@@ -480,14 +494,14 @@ int lsim_program_run(int dtype, int n, long long instances, const lsim_prog_op* 
       return fail(LSIM_EHIP, "LDS program: %zu bytes of LDS are not available", bytes);
     k_program<double><<<(unsigned)instances, kProgThreads, bytes, s>>>(
         n, ops, op_start, op_count, (const double2*)mats, reals, draws, draw_start, clbits, probs,
-        (double2*)state, nterms, pauli_x, pauli_z, expect);
+        (double2*)state, nterms, pauli_x, pauli_z, expect, decisions);
   } else {
     if (hipFuncSetAttribute((const void*)k_program<float>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
       return fail(LSIM_EHIP, "LDS program: %zu bytes of LDS are not available", bytes);
     k_program<float><<<(unsigned)instances, kProgThreads, bytes, s>>>(
         n, ops, op_start, op_count, (const double2*)mats, reals, draws, draw_start, clbits, probs,
-        (float2*)state, nterms, pauli_x, pauli_z, expect);
+        (float2*)state, nterms, pauli_x, pauli_z, expect, decisions);
   }
   return check_launch("LDS program");
 }

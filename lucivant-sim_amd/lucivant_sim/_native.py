@@ -56,7 +56,7 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "lsim_program_max_qubits": ([_c_int], _c_int),
     "lsim_program_run": (
         [_c_int, _c_int, _c_ll, _c_void, _c_void, _c_void, _c_void, _c_void, _c_void, _c_void]
-        + [_c_void, _c_void, _c_void, _c_int, _c_void, _c_void, _c_void, _c_void],
+        + [_c_void, _c_void, _c_void, _c_int, _c_void, _c_void, _c_void, _c_void, _c_void],
         _c_int,
     ),
     "lsim_dm_pauli_expect": (

@@ -13,7 +13,8 @@ Four choices set the cost of a statevector simulation on the MI355X:
 on and off) on a scratch state of the same size, and times it with HIP events, then times the
 winner's plan for each matrix-core width mask with streaming on and off; the winner is stored
 per (gfx architecture, qubit count) in ``~/.cache/lucivant_sim/autotune.json`` (or
-``$LSIM_AUTOTUNE``) and applied by :func:`apply`, which ``simulator`` calls on first use. The LDS
+``$LSIM_AUTOTUNE``) and read by :func:`apply`, which ``simulator`` calls on first use of a size;
+every simulation then sets its own size's switches (:func:`set_switches`) before it launches. The LDS
 program path has its own knob, the fusion width of compiled programs (0 = one op per gate), tuned
 the same way by :func:`tune_program_width`.
 """
@@ -141,19 +142,33 @@ def best_for(num_qubits: int) -> dict[str, Any] | None:
     return table[nearest]
 
 
-def apply(num_qubits: int) -> tuple[int, int, int]:
-    """(width, low_bits, low_width) to plan ``num_qubits``-qubit circuits with; also sets the
-    tiling switch to the stored choice."""
+#: the library's own switch settings (tiled kernel on, matrix cores for 3-5 qubit operators,
+#: streaming mode 1), used for sizes the autotuner has no record for
+DEFAULT_SWITCHES = (1, (1 << 3) | (1 << 4) | (1 << 5), 1)
+
+
+def apply(num_qubits: int) -> tuple[int, int, int, tuple[int, int, int]]:
+    """(width, low_bits, low_width, switches) to plan and launch ``num_qubits``-qubit circuits
+    with; ``switches`` = (tiled, matrix-core width mask, streaming mode), set per launch by
+    :func:`set_switches`. Nothing is changed here: the switches are process-wide, so every
+    simulation sets those of its own size before it launches."""
     from lucivant_sim import simulator
 
     rec = best_for(num_qubits)
     if rec is None:
-        return simulator.FUSION_WIDTH, 0, 0
+        return simulator.FUSION_WIDTH, 0, 0, DEFAULT_SWITCHES
+    switches = (int(rec.get("tiled", 1)), int(rec.get("mfma", DEFAULT_SWITCHES[1])), int(rec.get("streaming", 1)))
+    return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0)), switches
+
+
+def set_switches(switches: tuple[int, int, int]) -> None:
+    """Set the library's tiling, matrix-core and streaming switches (bit-for-bit results don't
+    depend on tiling or streaming; the matrix cores round differently from the vector FMAs)."""
     lib = nat.load()
-    lib.lsim_set_dense_tiling(int(rec.get("tiled", 1)))
-    lib.lsim_set_dense_mfma(int(rec.get("mfma", (1 << 3) | (1 << 4) | (1 << 5))))
-    lib.lsim_set_dense_streaming(int(rec.get("streaming", 1)))
-    return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0))
+    tiled, mfma, streaming = switches
+    lib.lsim_set_dense_tiling(int(tiled))
+    lib.lsim_set_dense_mfma(int(mfma))
+    lib.lsim_set_dense_streaming(int(streaming))
 
 
 def tune_program_width(circuits: list[Circuit], widths: tuple[int, ...] = (0, 2, 3), reps: int = 3) -> dict[str, float]:

@@ -127,10 +127,13 @@ def compile_one(
 ) -> Compiled:
     """Read one program (runs in a worker; also callable in-process).
 
-    ``observable`` set: an expectation request (labels checked here, statevector method).
+    ``observable`` set: an expectation request (labels checked here; the statevector method, or
+    the density matrix when ``noise`` has gate errors, as ``expectation`` chooses).
     ``widths`` set: statevectors wider than ``lds_qubits`` also get their fused passes, planned
     with ``widths[num_qubits]`` = (fusion width, low bits, low width). ``digest`` False skips the
     circuit digest (callers that build no Result)."""
+    from lucivant_sim._native import MAX_TERMS
+    from lucivant_sim.engines import SimulationError
     from lucivant_sim.optable import Unsupported, static_table
     from lucivant_sim.plan import plan_statevector
     from lucivant_sim.simulator import choose_method, to_circuit
@@ -139,13 +142,15 @@ def compile_one(
         c = to_circuit(src, params)
         terms = None
         if observable is not None:
+            # the checks of expectation(), in its order (simulator.py:496-502)
+            if c.is_dynamic():
+                raise SimulationError("an expectation value needs a circuit without mid-circuit measurement or conditions")
             terms = {observable: 1.0} if isinstance(observable, str) else {str(k): float(v) for k, v in dict(observable).items()}
             for label in terms:
                 if len(label) != c.num_qubits or set(label.upper()) - set("IXYZ"):
                     raise ValueError(f"{label!r} must be a Pauli string of length {c.num_qubits} (qubit 0 rightmost)")
-            if c.is_dynamic():
-                raise ValueError("an expectation value needs a circuit without mid-circuit measurement")
-            chosen = "statevector"
+            # as expectation() does (simulator.py:503-516): the density matrix under gate noise
+            chosen = "density_matrix" if noise is not None and noise.has_gate_errors() else "statevector"
         else:
             chosen = choose_method(c, noise, method)
         out = Compiled(
@@ -168,7 +173,10 @@ def compile_one(
                     pass
             return out
         out.pairs = tuple(c.final_measurements())
-        if 1 <= c.num_qubits <= lds_qubits:
+        # the program kernel takes 1..MAX_TERMS Pauli strings per instance; other term counts
+        # (none, or more) are evaluated over a state in HBM in chunks of MAX_TERMS
+        lds_terms = terms is None or 0 < len(terms) <= MAX_TERMS
+        if 1 <= c.num_qubits <= lds_qubits and lds_terms:
             try:
                 out.table, out.pool = static_table(c)
                 return out

@@ -191,6 +191,9 @@ class BatchOutputs:
     probs: np.ndarray | None = None
     state: np.ndarray | None = None
     expect: np.ndarray | None = None
+    #: [instances, D, 2]: per draw column, the probability a state-dependent decision compared its
+    #: draw with and the branch it took (NaN where the column decides nothing state-dependent)
+    decisions: np.ndarray | None = None
 
 
 def run_programs(
@@ -203,6 +206,7 @@ def run_programs(
     want_probs: bool = False,
     want_state: bool = False,
     pauli: list[list[tuple[int, int]]] | None = None,
+    want_decisions: bool = False,
 ) -> BatchOutputs:
     """Run every program (each ``repeat`` times, e.g. once per shot) as one launch.
 
@@ -264,6 +268,7 @@ def run_programs(
         want_probs=want_probs,
         want_state=want_state,
         pauli=pauli,
+        want_decisions=want_decisions,
     )
 
 
@@ -282,6 +287,7 @@ def run_table(
     want_probs: bool = False,
     want_state: bool = False,
     pauli: list[list[tuple[int, int]]] | None = None,
+    want_decisions: bool = False,
 ) -> BatchOutputs:
     """:func:`run_programs` over programs already laid end to end: program j is rows
     ``starts[j] : starts[j] + counts[j]`` of ``table``, whose matrix offsets index ``pool``
@@ -296,6 +302,9 @@ def run_table(
     instances = programs * repeat
     if draws is not None and draws.size and draws.shape[0] != instances:
         raise ValueError(f"draws has {draws.shape[0]} rows for {instances} instances")
+    if draws is not None and draws.size >= 1 << 31:
+        # the kernel indexes draws with 32-bit offsets (draw_start[b] + column)
+        raise ValueError(f"{draws.size} draws in one launch exceed the kernel's 32-bit draw offsets; launch fewer instances")
     if pauli is not None:
         if len(pauli) != programs or len({len(t) for t in pauli}) > 1:
             raise ValueError("give every program the same number of Pauli strings")
@@ -338,6 +347,9 @@ def run_table(
         px = up(np.ascontiguousarray(masks[:, :, 0]))
         pz = up(np.ascontiguousarray(masks[:, :, 1]))
         ex = torch.empty(instances * nterms, dtype=torch.float64, device=device)
+    dec = None
+    if want_decisions and d_draws is not None:
+        dec = torch.full((instances * ndraws * 2,), float("nan"), dtype=torch.float64, device=device)
     nat.call(
         "lsim_program_run",
         dtype,
@@ -357,6 +369,7 @@ def run_table(
         ptr(px) if px is not None else None,
         ptr(pz) if pz is not None else None,
         ptr(ex) if ex is not None else None,
+        ptr(dec) if dec is not None else None,
         stream_handle(),
     )
     if clb is not None:
@@ -367,6 +380,8 @@ def run_table(
         out.state = stt.cpu().numpy().reshape(instances, size)
     if ex is not None:
         out.expect = ex.cpu().numpy().reshape(instances, nterms)
+    if dec is not None:
+        out.decisions = dec.cpu().numpy().reshape(instances, ndraws, 2)
     return out
 
 

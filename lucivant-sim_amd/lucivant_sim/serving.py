@@ -41,7 +41,7 @@ from lucivant_sim import program as pg
 from lucivant_sim.engines import MAX_DENSITY_QUBITS, MAX_STATEVECTOR_QUBITS, SimulationError, Statevector, launch_expectations, pauli_masks, ptr, require_device, upload
 from lucivant_sim.frontend import Compiled, compile_one
 from lucivant_sim.noise import NoiseModel
-from lucivant_sim.simulator import FUSION_WIDTH, _sample, _sample_many, _tuned
+from lucivant_sim.simulator import FUSION_WIDTH, _sample, _sample_many, _tuned, _use_switches
 
 _ids = itertools.count()
 
@@ -189,9 +189,16 @@ class Server:
     def _prepare(self, req: Request, c: Compiled) -> _Job:
         if c.method not in ("statevector", "trajectories", "density_matrix"):
             raise ValueError(f"the server runs statevector, density-matrix and trajectory requests, not {c.method!r}")
-        limit = MAX_DENSITY_QUBITS if c.method == "density_matrix" else MAX_STATEVECTOR_QUBITS
-        if c.num_qubits > limit:
-            raise SimulationError(f"{c.num_qubits} qubits are too many for the {c.method} method (up to {limit})")
+        # the engines' own size guards and messages (engines.py:29-31)
+        if c.method == "density_matrix" and c.num_qubits > MAX_DENSITY_QUBITS:
+            raise SimulationError(
+                f"{c.num_qubits} qubits need more memory than the density-matrix engine allows "
+                f"(up to {MAX_DENSITY_QUBITS}); use method='trajectories'"
+            )
+        if c.num_qubits > MAX_STATEVECTOR_QUBITS:
+            raise SimulationError(
+                f"{c.num_qubits} qubits need more memory than the statevector engine allows (up to {MAX_STATEVECTOR_QUBITS})"
+            )
         size = 16 << c.num_qubits
         if c.method == "density_matrix":
             size = 16 << (2 * c.num_qubits)
@@ -319,11 +326,15 @@ class Server:
 
     def _advance_wide(self, job: _Job) -> None:
         if job.method == "density_matrix":
-            from lucivant_sim.simulator import run
+            from lucivant_sim.simulator import expectation, run
 
             try:
-                res = run(job.c.circuit, job.req.shots, seed=job.req.seed, noise=job.req.noise, method="density_matrix")
                 job.passes, job.next_pass = [], 0
+                if job.terms is not None:
+                    # a noisy expectation: Tr(P rho) of the density matrix, as expectation() gives
+                    self._finish(job, expectation(job.c.circuit, job.terms, job.req.noise))
+                    return
+                res = run(job.c.circuit, job.req.shots, seed=job.req.seed, noise=job.req.noise, method="density_matrix")
                 self._finish(job, res.counts)
             except BaseException as exc:  # noqa: BLE001
                 self._finish_error(job.req, exc)
@@ -369,6 +380,7 @@ class Server:
                     job.calls.append((fn, (state, self.dtype, n, 1, p.k, nat.int_array(p.bits), addr, None, hs)))
         assert job.calls is not None
         job.req.stamps.setdefault("launched", time.perf_counter())
+        _use_switches(job.c.num_qubits)
         stop = min(len(job.calls), job.next_pass + self.chunk)
         for fn, args in job.calls[job.next_pass : stop]:
             if fn(*args):
@@ -520,7 +532,7 @@ def plain_stream(requests: list[Request], arrivals: list[float]) -> dict[str, fl
                 time.sleep(at - now)
             arrival = t0 + at
             if req.observable is not None:
-                req.result = simulator.expectation(req.program, req.observable)
+                req.result = simulator.expectation(req.program, req.observable, req.noise)
             else:
                 res = simulator.run(req.program, req.shots, seed=req.seed, noise=req.noise, method=req.method)
                 req.result = res.counts

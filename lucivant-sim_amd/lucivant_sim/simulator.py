@@ -232,6 +232,7 @@ def _statevector_plan(circuit: Circuit, width: int, low_bits: int, low_width: in
 def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseModel | None, width: int | None = None) -> None:
     """Static evolution (simulator.py:229-251): gates, their noise, resets of untouched qubits;
     measurements deferred. Runs as fused device passes."""
+    _use_switches(state.nbits)
     if isinstance(state, Statevector):
         w, low_bits, low_width = (width, 0, 0) if width else _tuned(circuit.num_qubits)
         state.run(_statevector_plan(circuit, w, low_bits, low_width))
@@ -248,21 +249,40 @@ def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseMo
     state.run(group)
 
 
-_TUNED: dict[int, tuple[int, int, int]] = {}
+_TUNED: dict[int, tuple[int, int, int, tuple[int, int, int]]] = {}
+_TUNED_LOCK = threading.Lock()
 
 
-def _tuned(num_qubits: int) -> tuple[int, int, int]:
-    """(width, low_bits, low_width) from the autotuner's stored choice for this size (see
-    autotune.py), else (FUSION_WIDTH, 0, 0)."""
-    if num_qubits not in _TUNED:
+def _tuned_entry(num_qubits: int) -> tuple[int, int, int, tuple[int, int, int]]:
+    """The autotuner's stored choice for this size (see autotune.py), read once per size."""
+    with _TUNED_LOCK:
+        hit = _TUNED.get(num_qubits)
+    if hit is None:
         from lucivant_sim import autotune
 
         try:
-            _TUNED[num_qubits] = autotune.apply(num_qubits)
+            hit = autotune.apply(num_qubits)
         except Exception:  # noqa: BLE001 - no device facts or an unreadable cache: the default
-            _TUNED[num_qubits] = (0, 0, 0)
-    w, low_bits, low_width = _TUNED[num_qubits]
+            hit = (0, 0, 0, autotune.DEFAULT_SWITCHES)
+        with _TUNED_LOCK:
+            _TUNED[num_qubits] = hit
+    return hit
+
+
+def _tuned(num_qubits: int) -> tuple[int, int, int]:
+    """(width, low_bits, low_width) from the autotuner's stored choice for this size, else
+    (FUSION_WIDTH, 0, 0)."""
+    w, low_bits, low_width, _ = _tuned_entry(num_qubits)
     return (w or FUSION_WIDTH, low_bits, low_width)
+
+
+def _use_switches(nbits: int) -> None:
+    """Set the kernel switches (tiling, matrix cores, streaming) stored for states of ``nbits``
+    bits before launching on one. They are process-wide, so a simulation never inherits the
+    choice of the size simulated before it."""
+    from lucivant_sim import autotune
+
+    autotune.set_switches(_tuned_entry(nbits)[3])
 
 
 def _sample(
@@ -436,10 +456,11 @@ def run(
         if circuit.num_qubits > 0:
             Statevector(circuit.num_qubits)  # the size check before any shot
         if precision == "complex64":
-            values, used = shot_values(circuit, noise, shots, rng, dtype=nat.C64)
+            record: dict = {}
+            values, used = shot_values(circuit, noise, shots, rng, dtype=nat.C64, record=record)
             uniq, cnt = np.unique(np.asarray(values, dtype=np.int64), return_counts=True)
             tally = {int(v): int(c) for v, c in zip(uniq, cnt)}
-            checked = complex64_check(circuit, noise, seed, values, used)
+            checked = complex64_check(circuit, noise, seed, values, used, record)
         else:
             tally = run_trajectories(circuit, noise, shots, rng)
         counts = {circuit.format_bits(b): n for b, n in sorted(tally.items())} if circuit.num_clbits else {}

@@ -513,13 +513,15 @@ def _mixture_choice(channel: Channel, r: np.ndarray) -> tuple[np.ndarray, list[n
     return lookup[branch], table
 
 
-def _kraus_index(channel: Channel, rho: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+def _kraus_index(channel: Channel, rho: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """Per-shot Kraus operator index (engines.py:171-183) given normalised reduced density
     matrices: the first i with r < sum of weights so far and w_i > 0, else the last one if its
-    weight is positive, else -1 (the state is left alone). Also returns the chosen weights."""
+    weight is positive, else -1 (the state is left alone). Also returns the chosen weights and
+    the running sum of weights at the chosen branch (what the draw was compared with)."""
     nshots = r.shape[0]
     idx = np.full(nshots, -1, dtype=np.int64)
     chosen_w = np.zeros(nshots)
+    chosen_acc = np.zeros(nshots)
     acc = np.zeros(nshots)
     weight = np.zeros(nshots)
     for i, kraus in enumerate(channel.kraus):
@@ -528,16 +530,18 @@ def _kraus_index(channel: Channel, rho: np.ndarray, r: np.ndarray) -> tuple[np.n
         pick = (idx < 0) & (r < acc) & (weight > 0)
         idx[pick] = i
         chosen_w[pick] = weight[pick]
+        chosen_acc[pick] = acc[pick]
     rest = (idx < 0) & (weight > 0)
     idx[rest] = len(channel.kraus) - 1
     chosen_w[rest] = weight[rest]
-    return idx, chosen_w
+    chosen_acc[rest] = acc[rest]
+    return idx, chosen_w, chosen_acc
 
 
 def _kraus_choice(channel: Channel, rho: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     """Per-shot normalised Kraus operator K_i / sqrt(w_i) for :func:`_kraus_index`'s choice
     (``sel[b] = b``), or ``sel[b] = -1`` when no operator applies."""
-    idx, w = _kraus_index(channel, rho, r)
+    idx, w, _ = _kraus_index(channel, rho, r)
     d = channel.kraus[0].shape[0]
     mats = np.zeros((r.shape[0], d, d), dtype=np.complex128)
     for i, kraus in enumerate(channel.kraus):
@@ -574,8 +578,19 @@ _RESET = [np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 1], [0,
 
 
 def _run_chunk(
-    steps: list[_Step], circuit: Circuit, draws: np.ndarray, dtype: int, nflags: int, batch: _Batch | None = None, share: bool = True
+    steps: list[_Step],
+    circuit: Circuit,
+    draws: np.ndarray,
+    dtype: int,
+    nflags: int,
+    batch: _Batch | None = None,
+    share: bool = True,
+    decisions: np.ndarray | None = None,
 ) -> np.ndarray:
+    """Every shot of one chunk (``draws``: [shots, D]); returns the classical bits. When given,
+    ``decisions`` ([shots, D, 2], NaN-filled) receives per draw column the probability a
+    state-dependent decision compared its draw with and the branch it took, as the program
+    kernel records them."""
     nshots = draws.shape[0]
     st = batch if batch is not None else _Batch(circuit.num_qubits, nshots, dtype, nflags, circuit.num_clbits, share)
     for t, step in enumerate(steps):
@@ -602,11 +617,15 @@ def _run_chunk(
             if step.kind == "channel":
                 ch = step.channel
                 assert ch is not None
-                idx, _ = _kraus_index(ch, rho / norm[:, None, None], r)
+                idx, _, acc = _kraus_index(ch, rho / norm[:, None, None], r)
                 st.queue(step.qubits, list(ch.kraus), idx if mask is None else np.where(mask, idx, -1), unitary=False)
+                if decisions is not None:
+                    _record(decisions, step.column, acc, idx, mask)
                 continue
             p1 = np.minimum(np.maximum(rho[:, 1, 1].real / norm, 0.0), 1.0)
             outcome = (r < p1).astype(np.int64)
+            if decisions is not None:
+                _record(decisions, step.column, p1, outcome, mask)
             table = _PROJECT if step.kind == "measure" else _RESET
             st.queue(step.qubits, table, outcome if mask is None else np.where(mask, outcome, -1), unitary=False)
             if step.kind == "measure":
@@ -621,6 +640,12 @@ def _run_chunk(
     return st.clbits
 
 
+def _record(decisions: np.ndarray, column: int, prob: np.ndarray, choice: np.ndarray, mask: np.ndarray | None) -> None:
+    rows = slice(None) if mask is None else mask
+    decisions[rows, column, 0] = prob[rows]
+    decisions[rows, column, 1] = choice[rows]
+
+
 def run_batched(
     circuit: Circuit,
     noise: NoiseModel | None,
@@ -630,19 +655,28 @@ def run_batched(
     dtype: int = nat.C128,
     batch_bytes: int | None = None,
     share: bool = True,
+    record: dict | None = None,
 ) -> np.ndarray:
     """Classical bits of every shot (int64, or Python ints beyond 62 clbits), all shots batched.
     Requires draw-balanced if/else arms (see :func:`draw_count`). ``share=False`` gives every shot
-    its own state from the start (the same results, with more device work)."""
+    its own state from the start (the same results, with more device work). ``record`` (a dict)
+    receives ``draws`` [shots, D] and ``decisions`` [shots, D, 2] (see :func:`_run_chunk`)."""
     steps, per_shot, nflags = compile_program(circuit, noise)
     item = np.dtype(_NP_DTYPE[dtype]).itemsize
     limit = batch_bytes if batch_bytes is not None else BATCH_BYTES
     chunk = max(1, min(shots, limit // (item << circuit.num_qubits)))
-    out = []
+    out, all_draws, all_dec = [], [], []
     for start in range(0, shots, chunk):
         b = min(chunk, shots - start)
         draws = rng.random(b * per_shot).reshape(b, per_shot) if per_shot else np.zeros((b, 0))
-        out.append(_run_chunk(steps, circuit, draws, dtype, nflags, share=share))
+        dec = np.full((b, per_shot, 2), np.nan) if record is not None else None
+        out.append(_run_chunk(steps, circuit, draws, dtype, nflags, share=share, decisions=dec))
+        if record is not None:
+            all_draws.append(draws)
+            all_dec.append(dec)
+    if record is not None:
+        record["draws"] = np.concatenate(all_draws) if all_draws else np.zeros((0, per_shot))
+        record["decisions"] = np.concatenate(all_dec) if all_dec else np.zeros((0, per_shot, 2))
     return np.concatenate(out) if out else np.zeros(0, dtype=np.int64)
 
 
@@ -651,11 +685,17 @@ LDS_CHUNK = 1 << 16
 
 
 def run_lds(
-    circuit: Circuit, noise: NoiseModel | None, shots: int, rng: np.random.Generator, *, dtype: int = nat.C128
+    circuit: Circuit,
+    noise: NoiseModel | None,
+    shots: int,
+    rng: np.random.Generator,
+    *,
+    dtype: int = nat.C128,
+    record: dict | None = None,
 ) -> np.ndarray | None:
     """All shots as LDS programs, one workgroup per shot, when the state fits a workgroup's LDS
     and the circuit fits the program kernel; None otherwise (the caller uses :func:`run_batched`).
-    Draws are taken from ``rng`` exactly as :func:`run_batched` takes them."""
+    Draws are taken from ``rng`` exactly as :func:`run_batched` takes them; ``record`` as there."""
     from lucivant_sim import program as pg
 
     if circuit.num_qubits < 1 or circuit.num_qubits > pg.max_qubits(dtype) or circuit.num_clbits > 62:
@@ -664,13 +704,21 @@ def run_lds(
         prog = pg.compile_trajectory(circuit, noise)
     except pg.Unsupported:
         return None
-    out = []
-    for start in range(0, shots, LDS_CHUNK):
-        b = min(LDS_CHUNK, shots - start)
+    out, all_draws, all_dec = [], [], []
+    # the kernel's draw offsets are 32-bit: a launch holds fewer than 2^31 draws
+    chunk = max(1, min(LDS_CHUNK, ((1 << 31) - 1) // max(prog.draws, 1)))
+    for start in range(0, shots, chunk):
+        b = min(chunk, shots - start)
         draws = rng.random(b * prog.draws).reshape(b, prog.draws) if prog.draws else None
-        res = pg.run_programs([prog], repeat=b, dtype=dtype, draws=draws, want_clbits=True)
+        res = pg.run_programs([prog], repeat=b, dtype=dtype, draws=draws, want_clbits=True, want_decisions=record is not None)
         assert res.clbits is not None
         out.append(res.clbits.astype(np.int64))
+        if record is not None:
+            all_draws.append(draws if draws is not None else np.zeros((b, 0)))
+            all_dec.append(res.decisions if res.decisions is not None else np.zeros((b, 0, 2)))
+    if record is not None:
+        record["draws"] = np.concatenate(all_draws)
+        record["decisions"] = np.concatenate(all_dec)
     return np.concatenate(out)
 
 
@@ -708,17 +756,24 @@ def run_sequential(circuit: Circuit, noise: NoiseModel | None, shots: int, rng: 
 
 
 def shot_values(
-    circuit: Circuit, noise: NoiseModel | None, shots: int, rng: np.random.Generator, *, dtype: int = nat.C128
+    circuit: Circuit,
+    noise: NoiseModel | None,
+    shots: int,
+    rng: np.random.Generator,
+    *,
+    dtype: int = nat.C128,
+    record: dict | None = None,
 ) -> tuple[list[int] | np.ndarray, int]:
     """The classical bit pattern of every shot, in shot order, and the precision it was computed
-    in (circuits whose if/else arms draw unequally run shot after shot in complex128)."""
+    in (circuits whose if/else arms draw unequally run shot after shot in complex128, and record
+    nothing). ``record`` receives the draws and decisions of a batched run (:func:`run_batched`)."""
     if shots == 0:
         return [], dtype
     if draw_count(circuit.operations, noise) is None:
         return run_sequential(circuit, noise, shots, rng), nat.C128
-    values = run_lds(circuit, noise, shots, rng, dtype=dtype)
+    values = run_lds(circuit, noise, shots, rng, dtype=dtype, record=record)
     if values is None:
-        values = run_batched(circuit, noise, shots, rng, dtype=dtype)
+        values = run_batched(circuit, noise, shots, rng, dtype=dtype, record=record)
     return values, dtype
 
 
@@ -737,19 +792,71 @@ def run_trajectories(
 
 
 #: shots of a complex64 run that are rerun in complex128 to state its error
-PRECISION_CHECK_SHOTS = 256
+PRECISION_CHECK_SHOTS = 4096
 
 
-def complex64_check(circuit: Circuit, noise: NoiseModel | None, seed: int, values: list[int] | np.ndarray, used: int) -> dict[str, object]:
-    """How a complex64 trajectory run compares with complex128: its first shots are rerun in
-    complex128 with the same draws (the same seeded stream) and compared shot by shot. The shots
-    are deterministic given their draws, so any difference is the precision's doing."""
+def complex64_check(
+    circuit: Circuit, noise: NoiseModel | None, seed: int, values: list[int] | np.ndarray, used: int, record: dict | None = None
+) -> dict[str, object]:
+    """The error of a complex64 trajectory run against complex128 (float64 arithmetic, the
+    reference's): its first shots are rerun in complex128 with the same draws (the same seeded
+    stream) and compared decision by decision. Shots are deterministic given their draws, so any
+    difference is the precision's doing. Reported:
+
+    - ``shots_differing_from_complex128`` and ``counts_tvd``, the total variation distance
+      between the two tallies of the checked shots;
+    - ``max_abs_dp``: the largest |p64 - p128| over every measurement probability p1 and Kraus
+      running weight a draw was compared with, up to each shot's first flipped decision (later
+      decisions of that shot follow another history);
+    - ``flipped_decisions``: shots in which some decision took another branch, and for the first
+      few of them ``flip_margins``, the distance of the draw from the complex128 boundary (a flip
+      is a tolerance-level case when it is within about the float32 error of the state)."""
     if used != nat.C64:
         return {"dtype": "complex128", "note": "this circuit runs shot after shot in complex128"}
     m = min(len(values), PRECISION_CHECK_SHOTS)
-    ref, _ = shot_values(circuit, noise, m, np.random.default_rng(seed), dtype=nat.C128)
-    differing = int(np.sum(np.asarray(values[:m], dtype=np.int64) != np.asarray(ref, dtype=np.int64)))
-    return {"dtype": "complex64", "checked_shots": m, "shots_differing_from_complex128": differing}
+    ref_record: dict = {}
+    ref, _ = shot_values(circuit, noise, m, np.random.default_rng(seed), dtype=nat.C128, record=ref_record)
+    mine = np.asarray(values[:m], dtype=np.int64)
+    theirs = np.asarray(ref, dtype=np.int64)
+    u64, c64 = np.unique(mine, return_counts=True)
+    u128, c128 = np.unique(theirs, return_counts=True)
+    tally: dict[int, int] = {}
+    for u, c in zip(u64.tolist(), c64.tolist()):
+        tally[u] = tally.get(u, 0) + c
+    for u, c in zip(u128.tolist(), c128.tolist()):
+        tally[u] = tally.get(u, 0) - c
+    out: dict[str, object] = {
+        "dtype": "complex64",
+        "against": "complex128 rerun of the same shots with the same draws",
+        "checked_shots": m,
+        "shots_differing_from_complex128": int(np.sum(mine != theirs)),
+        "counts_tvd": 0.5 * sum(abs(v) for v in tally.values()) / m if m else 0.0,
+    }
+    if record is None or "decisions" not in record or "decisions" not in ref_record:
+        return out
+    out.update(compare_decisions(record["decisions"][:m], ref_record["decisions"], ref_record["draws"]))
+    return out
 
 
-__all__ = ["complex64_check", "compile_program", "draw_count", "run_batched", "run_sequential", "run_trajectories", "shot_values"]
+def compare_decisions(d64: np.ndarray, d128: np.ndarray, draws: np.ndarray, margins: int = 16) -> dict[str, object]:
+    """Decision-by-decision comparison of two runs of the same shots and draws (records laid out
+    as :func:`_run_chunk` fills them)."""
+    shots, cols = d128.shape[:2]
+    if shots == 0 or cols == 0:
+        return {"decisions_compared": 0, "max_abs_dp": 0.0, "flipped_decisions": 0, "flip_margins": []}
+    made = ~np.isnan(d128[:, :, 1]) & ~np.isnan(d64[:, :, 1])
+    flip = made & (d64[:, :, 1] != d128[:, :, 1])
+    flipped = flip.any(axis=1)
+    first = np.where(flipped, flip.argmax(axis=1), cols)
+    same = made & (np.arange(cols)[None, :] <= first[:, None]) & ~flip
+    dp = np.abs(d64[:, :, 0] - d128[:, :, 0])
+    where = np.flatnonzero(flipped)[:margins]
+    return {
+        "decisions_compared": int(same.sum()),
+        "max_abs_dp": float(dp[same].max()) if same.any() else 0.0,
+        "flipped_decisions": int(flipped.sum()),
+        "flip_margins": [float(abs(draws[s, first[s]] - d128[s, first[s], 0])) for s in where],
+    }
+
+
+__all__ = ["compare_decisions", "complex64_check", "compile_program", "draw_count", "run_batched", "run_sequential", "run_trajectories", "shot_values"]

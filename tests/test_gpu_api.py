@@ -291,11 +291,58 @@ def test_complex64_trajectories_state_their_error() -> None:
     r64 = ls.run(src, 2000, seed=3, noise=noise, method="trajectories", precision="complex64")
     r128 = ls.run(src, 2000, seed=3, noise=noise, method="trajectories")
     assert r64.precision is not None and r64.precision["dtype"] == "complex64"
-    assert r64.precision["checked_shots"] == 256
+    assert r64.precision["checked_shots"] == 2000
     assert r64.precision["shots_differing_from_complex128"] <= 2
+    # the LDS program kernel records every decision: measurement p1 and Kraus running weights
+    assert r64.precision["decisions_compared"] > 2000 * 6
+    assert 0 < r64.precision["max_abs_dp"] < 1e-5
+    assert r64.precision["counts_tvd"] <= r64.precision["shots_differing_from_complex128"] / 2000
     assert r128.precision is None
     tvd = 0.5 * sum(abs(r64.counts.get(k, 0) - r128.counts.get(k, 0)) for k in set(r64.counts) | set(r128.counts)) / 2000
     assert tvd <= 2 / 2000
     assert r64.fingerprint == r128.fingerprint  # precision is not part of what was run
     with pytest.raises(ValueError):
         ls.run(wl.random_layers(5, 2, 1), 10, seed=3, method="statevector", precision="complex64")
+
+
+def test_c4_20_qubit_complex64_against_the_reference_counts(golden_runs: dict, golden_programs: dict) -> None:
+    """The 20-qubit trajectory configuration (teleport chain, 6 mid-circuit measurements with
+    if/else corrections, depolarising noise and amplitude damping), 256 shots at seed 7, in both
+    precisions against lucivant-sim 0.3.0's own counts for it. complex128 equals them exactly.
+    complex64 (float32 states, float64 decisions) may differ only in shots where a decision took
+    another branch because its draw lay within float32 error of the branch boundary: each such
+    shot is listed with the distance of its draw from the complex128 boundary, which must be
+    below 1e-5; decision probabilities that did not flip agree within 1e-5."""
+    from lucivant_sim.trajectories import compare_decisions
+
+    name = "c4_20q_256shots_seed7"
+    rec = golden_runs[name]
+    circuit = ls.parse(golden_programs[name])
+    noise = wl.noise_c4()
+    r128: dict = {}
+    r64: dict = {}
+    b128 = run_batched(circuit, noise, 256, np.random.default_rng(7), record=r128)
+    b64 = run_batched(circuit, noise, 256, np.random.default_rng(7), dtype=nat.C64, record=r64)
+
+    def counts(bits: np.ndarray) -> dict[str, int]:
+        uniq, cnt = np.unique(bits, return_counts=True)
+        return {circuit.format_bits(int(v)): int(c) for v, c in zip(uniq, cnt)}
+
+    assert counts(b128) == rec["counts"]
+    assert np.array_equal(r64["draws"], r128["draws"])
+    stats = compare_decisions(r64["decisions"], r128["decisions"], r128["draws"], margins=256)
+    differing = np.flatnonzero(b64 != b128)
+    for s in differing:
+        print(f"shot {s}: complex64 {int(b64[s]):b} vs complex128 {int(b128[s]):b}")
+    print(f"flipped decisions: {stats['flipped_decisions']}, margins {stats['flip_margins']}, max |dp| {stats['max_abs_dp']:.3e}")
+    assert len(differing) <= stats["flipped_decisions"]
+    assert all(m < 1e-5 for m in stats["flip_margins"])
+    assert stats["decisions_compared"] >= 256 * 6
+    assert stats["max_abs_dp"] < 1e-5
+    got = counts(b64)
+    tvd = 0.5 * sum(abs(got.get(k, 0) - rec["counts"].get(k, 0)) for k in set(got) | set(rec["counts"])) / 256
+    assert tvd <= len(differing) / 256
+    res = ls.run(golden_programs[name], 256, seed=7, noise=noise, method="trajectories", precision="complex64")
+    assert res.precision is not None and res.precision["checked_shots"] == 256
+    assert res.precision["shots_differing_from_complex128"] == len(differing)
+    assert res.precision["max_abs_dp"] == stats["max_abs_dp"]

@@ -95,3 +95,110 @@ def test_bad_requests_fail_alone_and_the_rest_are_served() -> None:
             assert abs(r.result - ls.expectation(r.program, r.observable)) < 1e-10
         else:
             assert r.result == ls.run(r.program, r.shots, seed=r.seed, noise=r.noise).counts
+
+
+def _alone(r: Request) -> Request:
+    server = Server(streams=1)
+    a = server.submit(_fresh(r))
+    server.run_until_idle()
+    return a
+
+
+def test_wide_requests_alone_and_in_the_mix_match_the_oracle() -> None:
+    """Requests of 17-26 qubits beside small ones. The 26-qubit configuration (40 layers of u3 +
+    cx/cz) as a weighted expectation of its 8 Pauli strings is checked against the oracle's own
+    full run (tests/golden/full_c2_oracle.json) within 1e-10; a 20-qubit count request against
+    the oracle's counts for the same seed; an 18-qubit ansatz expectation against the oracle's;
+    17- and 22-qubit requests against the single-circuit API. Each result is also the same alone
+    and in the mix."""
+    import json
+    from pathlib import Path
+
+    from oracle import engine as orc
+
+    rec = json.loads((Path(__file__).parent / "golden" / "full_c2_oracle.json").read_text())
+    src26, labels = wl.config_c2()
+    assert labels == rec["labels"]
+    weights = np.random.default_rng(8).uniform(-1, 1, len(labels))
+    c2 = Request(src26, observable={lbl: float(w) for lbl, w in zip(labels, weights)})
+    c2_want = float(np.dot(weights, rec["expectations"]))
+
+    counts20 = Request(wl.random_layers(20, 6, 123), shots=1000, seed=9)
+    counts20_want = orc.run_counts(ls.parse(counts20.program), 1000, 9)
+
+    rng = np.random.default_rng(21)
+    hea18 = Request(wl.hea_ansatz(18, 12, 5), observable={lbl: float(w) for lbl, w in zip(wl.pauli_strings(18, 16, 3), rng.uniform(-1, 1, 16))})
+    psi18 = orc.evolve_sv(ls.parse(hea18.program))
+    hea18_want = sum(w * orc.expectation_sv(psi18, lbl, 18) for lbl, w in hea18.observable.items())  # type: ignore[union-attr]
+
+    counts17 = Request(wl.random_layers(17, 30, 44), shots=1000, seed=12)
+    hea22 = Request(wl.hea_ansatz(22, 40, 6), observable={lbl: float(w) for lbl, w in zip(wl.pauli_strings(22, 16, 4), rng.uniform(-1, 1, 16))})
+
+    mix = [c2, *_requests()[:4], counts20, hea18, counts17, hea22]
+    server = Server(streams=4, chunk_passes=16)
+    for r in mix:
+        server.submit(r)
+    server.run_until_idle()
+    for r in mix:
+        assert r.error is None, r.error
+    assert abs(c2.result - c2_want) < 1e-10
+    assert counts20.result == counts20_want
+    assert abs(hea18.result - hea18_want) < 1e-10
+    assert counts17.result == ls.run(counts17.program, 1000, seed=12).counts
+    assert abs(hea22.result - ls.expectation(hea22.program, hea22.observable)) < 1e-10  # type: ignore[arg-type]
+    for r in (c2, counts20, hea18, counts17, hea22):
+        a = _alone(r)
+        assert a.error is None
+        if r.observable is not None:
+            assert abs(a.result - r.result) < 1e-12
+        else:
+            assert a.result == r.result
+
+
+def test_noisy_expectation_requests_use_the_density_matrix() -> None:
+    """An expectation request with gate noise is Tr(P rho) of the density matrix, as
+    expectation() computes it (simulator.py:503-516), not the ideal statevector's value; wider
+    than the density-matrix engine's 13 qubits it fails with the engine's error."""
+    from oracle import engine as orc
+
+    noise = wl.noise_c3()
+    src = wl.hea_ansatz(6, 4, 11)
+    labels = wl.pauli_strings(6, 5, 12)
+    obs = {lbl: float(w) for lbl, w in zip(labels, np.linspace(-1, 1, 5))}
+    rho = orc.evolve_dm(ls.parse(src), noise)
+    want = sum(w * orc.expectation_dm(rho, lbl, 6) for lbl, w in obs.items())
+    ideal = ls.expectation(src, obs)
+    assert abs(want - ideal) > 1e-6  # the noise changes the value
+    noisy = Request(src, observable=obs, noise=noise)
+    wide = Request(wl.hea_ansatz(14, 2, 3), observable="Z" * 14, noise=noise)
+    server = Server(streams=2)
+    for r in (noisy, wide, *_requests()[:2]):
+        server.submit(r)
+    server.run_until_idle()
+    assert noisy.error is None and abs(noisy.result - want) < 1e-10
+    assert abs(noisy.result - ls.expectation(src, obs, noise)) < 1e-12
+    assert isinstance(wide.error, ls.SimulationError)
+
+
+def test_expectation_requests_with_many_or_no_terms() -> None:
+    """65 Pauli strings (more than one program launch takes) and an empty observable, on a
+    10-qubit circuit: the same values as expectation() (0.0 for no terms), and the other
+    requests of the mix are unaffected."""
+    src = wl.hea_ansatz(10, 5, 31)
+    labels = wl.pauli_strings(10, 65, 32)
+    obs = {lbl: float(w) for lbl, w in zip(labels, np.random.default_rng(33).uniform(-1, 1, 65))}
+    many = Request(src, observable=obs)
+    none = Request(src, observable={})
+    rest = _requests()[:3]
+    server = Server(streams=2)
+    for r in (many, none, *rest):
+        server.submit(r)
+    server.run_until_idle()
+    assert many.error is None and abs(many.result - ls.expectation(src, obs)) < 1e-10
+    assert none.error is None and none.result == 0.0
+    for r in rest:
+        assert r.error is None
+        if r.observable is not None:
+            assert abs(r.result - ls.expectation(r.program, r.observable)) < 1e-10
+        else:
+            assert r.result == ls.run(r.program, r.shots, seed=r.seed).counts
