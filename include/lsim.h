@@ -134,6 +134,20 @@ int lsim_sv_pauli_expect(const void* states, int dtype, int nbits, long long bat
                          const unsigned long long* xmask, const unsigned long long* zmask,
                          double* out, double* workspace, void* stream);
 
+/* lsim_sv_pauli_expect without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No reference counterpart. It runs the same kernels, in the same summation order (bit-identical
+ * results), with partials in a 4 MiB scratch array the library holds on each GPU until the
+ * process ends, and calls no allocator. It refuses (LSIM_EINVAL) sizes whose partials exceed the
+ * scratch: batch x nterms x (one partial per workgroup, at most 4096) x 8 bytes must stay
+ * within 4 MiB, so one state of any size takes up to 64 strings, and 128 states of 20 bits or
+ * fewer take up to 16 strings each (see lsim_reduce_workspace_bytes). Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (on different streams, or from different
+ * threads) race and must be serialised by the caller. */
+int lsim_sv_pauli_expect_noalloc(const void* states, int dtype, int nbits, long long batch,
+                                 int nterms, const unsigned long long* xmask,
+                                 const unsigned long long* zmask, double* out, void* stream);
+
 /* probs[b][i] = max(Re rho_b[i][i], 0) for n-qubit density matrices: DensityMatrix.probabilities
  * (engines.py:257-260). */
 int lsim_dm_probabilities(const void* rho, int dtype, int n, long long batch, double* probs,

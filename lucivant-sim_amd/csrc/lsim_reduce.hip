@@ -6,6 +6,8 @@
 // state with wavefront shuffles and LDS, writes one partial per value, and a second kernel adds
 // the partials of each state in a fixed order. Repeating a call gives the same bits, which keeps
 // seeded trajectory decisions reproducible run to run. Accumulation is float64 for both dtypes.
+#include <mutex>
+
 #include "lsim_internal.h"
 
 namespace lsim {
@@ -231,6 +233,42 @@ static int load_terms(int nterms, const unsigned long long* xmask, const unsigne
   return LSIM_OK;
 }
 
+static size_t pauli_sv_workspace_bytes(int nbits, long long batch, int nterms) {
+  return (size_t)batch * nterms * reduce_blocks(1ull << (nbits - 1)) * sizeof(double);
+}
+
+int launch_pauli_sv(const void* states, int dtype, int nbits, long long batch, int nterms,
+                    const PauliTerms& terms, double* out, double* workspace, void* stream);
+
+// Scratch of lsim_sv_pauli_expect_noalloc: a file-scope device array, so that entry point calls
+// no allocator. Each GPU the library runs on holds its own copy until the process ends. In host
+// code the array's name is not a device address: the address comes from hipGetSymbolAddress,
+// looked up once per device with that device current, and only that address is passed on.
+constexpr size_t kScratchBytes = 4u << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mutex;
+static void* scratch_on[kMaxDevices] = {};
+
+static int scratch_address(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(LSIM_EHIP, "no current device: %s", hipGetErrorString(e));
+  LSIM_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d is beyond the %d this library serves", dev,
+               kMaxDevices);
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (scratch_on[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(LSIM_EHIP, "scratch address on device %d: %s", dev, hipGetErrorString(e));
+    scratch_on[dev] = p;
+  }
+  *out = static_cast<double*>(scratch_on[dev]);
+  return LSIM_OK;
+}
+
 }  // namespace lsim
 
 using namespace lsim;
@@ -301,6 +339,34 @@ int lsim_sv_pauli_expect(const void* states, int dtype, int nbits, long long bat
   LSIM_REQUIRE(out != nullptr && workspace != nullptr, "output or workspace pointer is NULL");
   PauliTerms terms;
   if (int rc = load_terms(nterms, xmask, zmask, nbits, &terms)) return rc;
+  return launch_pauli_sv(states, dtype, nbits, batch, nterms, terms, out, workspace, stream);
+}
+
+int lsim_sv_pauli_expect_noalloc(const void* states, int dtype, int nbits, long long batch,
+                                 int nterms, const unsigned long long* xmask,
+                                 const unsigned long long* zmask, double* out, void* stream) {
+  if (int rc = check_common(states, dtype, nbits, batch)) return rc;
+  LSIM_REQUIRE(out != nullptr, "the output pointer is NULL");
+  PauliTerms terms;
+  if (int rc = load_terms(nterms, xmask, zmask, nbits, &terms)) return rc;
+  const size_t need = pauli_sv_workspace_bytes(nbits, batch, nterms);
+  LSIM_REQUIRE(need <= kScratchBytes,
+               "%lld states of %d bits with %d strings need %zu bytes of scratch; this entry point "
+               "has %zu (use lsim_sv_pauli_expect with a workspace)",
+               batch, nbits, nterms, need, kScratchBytes);
+  double* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  return launch_pauli_sv(states, dtype, nbits, batch, nterms, terms, out, ws, stream);
+}
+
+}  // extern "C"
+
+namespace lsim {
+
+// The expectation launches shared by both entry points: per-string partials into `workspace`
+// (every slot of batch x nterms x nb is written before it is read), then their fixed-order sums.
+int launch_pauli_sv(const void* states, int dtype, int nbits, long long batch, int nterms,
+                    const PauliTerms& terms, double* out, double* workspace, void* stream) {
   const uint32_t nb = reduce_blocks(1ull << (nbits - 1));
   hipStream_t s = as_stream(stream);
   const long long per_launch = kMaxGridY / nterms;
@@ -317,6 +383,10 @@ int lsim_sv_pauli_expect(const void* states, int dtype, int nbits, long long bat
   sum_rows(workspace, nb, (uint64_t)batch * nterms, out, s);
   return check_launch("Pauli expectation");
 }
+
+}  // namespace lsim
+
+extern "C" {
 
 int lsim_dm_probabilities(const void* rho, int dtype, int n, long long batch, double* probs,
                           void* stream) {

@@ -52,6 +52,10 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
         [_c_void, _c_int, _c_int, _c_ll, _c_int, _c_ull_p, _c_ull_p, _c_void, _c_void, _c_void],
         _c_int,
     ),
+    "lsim_sv_pauli_expect_noalloc": (
+        [_c_void, _c_int, _c_int, _c_ll, _c_int, _c_ull_p, _c_ull_p, _c_void, _c_void],
+        _c_int,
+    ),
     "lsim_dm_probabilities": ([_c_void, _c_int, _c_int, _c_ll, _c_void, _c_void], _c_int),
     "lsim_program_max_qubits": ([_c_int], _c_int),
     "lsim_program_run": (

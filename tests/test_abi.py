@@ -73,6 +73,13 @@ def test_argument_errors_are_reported_without_a_gpu(lib: ctypes.CDLL) -> None:
     assert "beyond" in _err(lib)
     assert lib.lsim_sv_set_basis(fake, 0, 4, 1, 16, None) == 1
     assert "outside the state" in _err(lib)
+    # the allocation-free expectation refuses partials beyond its 4 MiB scratch before any GPU call
+    x64 = nat.ull_array([1] * 64)
+    z64 = nat.ull_array([0] * 64)
+    assert lib.lsim_sv_pauli_expect_noalloc(fake, 0, 20, 1024, 64, x64, z64, fake, None) == 1
+    assert "scratch" in _err(lib)
+    assert lib.lsim_sv_pauli_expect_noalloc(fake, 0, 4, 1, 1, xs, nat.ull_array([0]), fake, None) == 1
+    assert "beyond" in _err(lib)
     assert lib.lsim_reduce_workspace_bytes(26, 1, 1) > 0
 
 
