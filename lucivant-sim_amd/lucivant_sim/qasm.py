@@ -1,15 +1,28 @@
-"""An OpenQASM 2 / 3 reader producing :class:`~lucivant_sim.circuit.Circuit`.
+"""OpenQASM 2.0 and 3 reader: the dialect lucivant-sim 0.3.0 accepts (src/lucivant_sim/qasm.py).
 
-The accepted language and the messages of :class:`QasmError` follow lucivant-sim 0.3.0
-src/lucivant_sim/qasm.py (its docstring, lines 1-9, and ``parse``/``load`` at 842-885):
-register and ``qubit``/``bit`` declarations, the standard gates, user ``gate`` definitions, the
-``ctrl @``/``negctrl @``/``inv @``/``pow(k) @`` modifiers, ``input``/``const`` values,
-broadcasting over registers, slices and index sets, both ``measure`` spellings, ``reset``,
-``barrier``, ``gphase``, ``box``, ``delay`` and ``if``/``else`` on classical bits. Loops,
-subroutines and calibration blocks are refused by name.
+Accepted: ``qreg``/``creg`` and ``qubit``/``bit`` declarations, physical qubits ``$n``, the
+built-in gate table (qelib1.inc and stdgates.inc, which are never read from disk), user ``gate``
+blocks, the ``ctrl @``/``negctrl @``/``inv @``/``pow(k) @`` modifiers, ``input`` parameters and
+classical constants, broadcasting over registers, index lists and ranges, ``measure`` in both
+spellings, ``reset``, ``barrier``, ``gphase``, ``box``, ``delay`` and ``if``/``else`` on classical
+bits. Loops, subroutines, calibrations and classical arithmetic are refused with a message.
+Results, messages and error positions are those of the reference reader
+(tests/golden/reader_cases.json records them).
 
-The reader is written from that description: a regular-expression tokenizer and a
-recursive-descent parser with one token of lookahead (a few more for modifiers).
+How this reader is built:
+
+- *Source.* Pragma and annotation lines are blanked, then the whole text is checked for lexical
+  errors by two regular-expression passes (comments and strings masked, then a search for a
+  character no token may start with), so a bad character is reported before any later error.
+  Tokens carry their source offset; lines and columns are computed only when an error is raised.
+- *Stream.* Tokens are lexed lazily, as the parser looks ahead.
+- *Plain statements.* At a statement boundary one regular expression tries the most common form,
+  a built-in gate with literal parameters on indexed qubits (``u3(0.1, 2, -3) q[0];``). When it
+  matches and every check passes, the gate is made without tokenising the statement; otherwise
+  the statement is read by the general path, which then reports any error.
+- *Expressions* are parsed by precedence climbing into closures that evaluate under a name
+  environment (gate parameters, then declared values, then the constants).
+- *Statements* are dispatched on their first word through a table of handlers.
 """
 
 from __future__ import annotations
@@ -19,39 +32,17 @@ import re
 from collections.abc import Callable, Mapping
 from dataclasses import dataclass
 from pathlib import Path
-from typing import Any, NamedTuple
 
 import numpy as np
 
-from lucivant_sim.circuit import (
-    Barrier,
-    Circuit,
-    Condition,
-    Gate,
-    GlobalPhase,
-    IfElse,
-    Measure,
-    Operation,
-    Register,
-    Reset,
-)
+from lucivant_sim.circuit import Barrier, Circuit, Condition, Gate, GlobalPhase, IfElse, Measure, Operation, Register, Reset
 from lucivant_sim.gates import GATES, Matrix, canonical_name, controlled, gate_matrix, matrix_power, negcontrolled
 
-#: Declared qubits beyond this are refused before anything is allocated.
+#: Declared qubits beyond this are refused before anything is allocated (the reference's limit).
 MAX_QUBITS = 2048
 
-STANDARD_INCLUDES = frozenset({"qelib1.inc", "stdgates.inc"})
-
-#: qelib1.inc gates given there as decompositions rather than matrices.
-QELIB_DECOMPOSED = """
-gate rccx a, b, c { u2(0, pi) c; u1(pi/4) c; cx b, c; u1(-pi/4) c; cx a, c; u1(pi/4) c; cx b, c;
-  u1(-pi/4) c; u2(0, pi) c; }
-gate rc3x a, b, c, d { u2(0, pi) d; u1(pi/4) d; cx c, d; u1(-pi/4) d; u2(0, pi) d; cx a, d;
-  u1(pi/4) d; cx b, d; u1(-pi/4) d; cx a, d; u1(pi/4) d; cx b, d; u1(-pi/4) d; u2(0, pi) d;
-  u1(pi/4) d; cx c, d; u1(-pi/4) d; u2(0, pi) d; }
-"""
-
-UNSUPPORTED = frozenset({"for", "while", "def", "defcal", "cal", "extern", "opaque", "return", "break", "continue"})
+#: Read plain gate statements with one regular expression (off: every statement is tokenised).
+PLAIN_STATEMENTS = True
 
 
 class QasmError(ValueError):
@@ -68,122 +59,97 @@ class QasmError(ValueError):
         return (type(self), (self.message, self.line, self.column))
 
 
-# ----------------------------------------------------------------------------------- lexing
+# ----------------------------------------------------------------------------------- source
+
+_PRAGMA = re.compile(r"(?m)^[ \t]*(#?pragma|@)[^\n]*")
+_MASK = re.compile(r"//[^\n]*|/\*.*?\*/|\"[^\"\n]*\"|'[^'\n]*'", re.DOTALL)
+# characters no token starts with, a '$' not followed by a digit, and quotes left unmatched
+_BAD = re.compile(r"[^ \t\n\r\f\v\w/.$\[\](){};,:=+\-*^%<>!~@&|]|\$(?!\d)|[\"']")
 
 
-class Tok(NamedTuple):
-    kind: str  # id, num, str, phys, op, eof
-    text: str
-    line: int
-    col: int
+class _Source:
+    """Program text (pragmas blanked) and line/column lookup for offsets."""
+
+    __slots__ = ("text",)
+
+    def __init__(self, text: str) -> None:
+        self.text = _PRAGMA.sub("", text)
+
+    def where(self, offset: int) -> tuple[int, int]:
+        line = self.text.count("\n", 0, offset) + 1
+        return line, offset - (self.text.rfind("\n", 0, offset) + 1) + 1
+
+    def first_bad(self) -> int:
+        """Offset of the first character the lexer can't read, or -1."""
+        text = self.text
+        masked = _MASK.sub(lambda m: " " * (m.end() - m.start()), text) if ('"' in text or "'" in text or "/" in text) else text
+        m = _BAD.search(masked)
+        return -1 if m is None else m.start()
 
 
+# Token: (kind, text, offset, source); kinds id, num, str, phys, op, eof
+Token = tuple
+
+
+_SKIP = re.compile(r"(?:[ \t\r\f\v\n]+|//[^\n]*|/\*.*?\*/)*", re.DOTALL)
 _LEX = re.compile(
-    r"(?P<ws>\s+)|(?P<line_comment>//[^\n]*)|(?P<block_comment>/\*.*?\*/)"
-    r"|(?P<num>0[bB][01_]+|0[xX][0-9a-fA-F_]+|0[oO][0-7_]+|(?:\d[\d_]*\.?[\d_]*|\.\d[\d_]*)(?:[eE][+-]?\d+)?)"
-    r"|(?P<str>\"[^\"\n]*\"|'[^'\n]*')|(?P<phys>\$\d+)|(?P<id>[^\W\d]\w*)"
-    r"|(?P<op>->|==|!=|<=|>=|&&|\|\||\*\*|<<|>>|[\[\](){};,:=+\-*/^%<>!~@&|.])",
-    re.DOTALL,
+    r"(0[bB][01_]+|0[xX][0-9a-fA-F_]+|0[oO][0-7_]+|(?:\d[\d_]*\.?[\d_]*|\.\d[\d_]*)(?:[eE][+-]?\d+)?)"
+    r"|(\"[^\"\n]*\"|'[^'\n]*')"
+    r"|(\$\d+)"
+    r"|([^\W\d]\w*)"
+    r"|(->|==|!=|<=|>=|&&|\|\||\*\*|<<|>>|[\[\](){};,:=+\-*/^%<>!~@&|.])"
 )
-_PRAGMA = re.compile(r"(?m)^[ \t]*(?:#?pragma|@)[^\n]*")
-_SKIP = frozenset({"ws", "line_comment", "block_comment"})
-
-# A whole gate-call statement with literal-number arguments and indexed qubits, e.g.
-# ``u3(0.1, -2e-3, 3) q[0];`` or ``cx q[0], q[11];``: lexed as one ``simple`` token when it starts a
-# statement, so the common case costs one regex match instead of a dozen tokens. Statement
-# keywords are excluded; the parser reads such a token straight into a Gate, or lexes its text
-# into ordinary tokens when it is anything else (a user-defined gate, an error).
-_SNUM = r"-?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?"
-_SARG = r"([A-Za-z_]\w*)\s*\[\s*(\d+)\s*\]"
-_SARG_NC = r"[A-Za-z_]\w*\s*\[\s*\d+\s*\]"
-_SIMPLE = re.compile(
-    r"(?!(?:measure|reset|barrier|qreg|creg|qubit|bit|include|gate|opaque|if|else|for|while|box|delay"
-    r"|gphase|input|output|const|float|angle|int|uint|bool|def|defcal|cal|return|let|extern|ctrl|negctrl"
-    r"|inv|pow)\b)"
-    rf"([A-Za-z_]\w*)(?:\s*\(\s*({_SNUM}(?:\s*,\s*{_SNUM})*)\s*\)\s*|\s+)"
-    rf"({_SARG_NC}(?:\s*,\s*{_SARG_NC})*)\s*;"
-)
-_SIMPLE_ARG = re.compile(_SARG)
+_KINDS = (None, "num", "str", "phys", "id", "op")
 
 
-class _SimpleTok(NamedTuple):
-    """A ``simple`` token with its regex groups (name, parameter text, qubit text)."""
+class _Stream:
+    """Tokens of one source, lexed on demand."""
 
-    kind: str
-    text: str
-    line: int
-    col: int
-    groups: tuple[str, str | None, str]
-_BOUNDARY = frozenset({";", "{", "}"})
-#: lex simple gate calls as single tokens (False: every statement takes the general path)
-FAST_STATEMENTS = True
+    __slots__ = ("src", "text", "pos", "ahead")
 
+    def __init__(self, src: _Source) -> None:
+        self.src = src
+        self.text = src.text
+        self.pos = 0  # offset of the first character not yet lexed
+        self.ahead: list[Token] = []  # lexed, not yet consumed
 
-def _lex_span(text: str, line: int, col: int) -> list[Tok]:
-    """Ordinary tokens of ``text``, which starts at ``line``/``col`` of the source."""
-    out: list[Tok] = []
-    line_start = -(col - 1)
-    for m in _LEX.finditer(text):
-        kind = m.lastgroup
-        start = m.start()
-        if kind in _SKIP:
-            part = m.group()
-            nl = part.count("\n")
-            if nl:
-                line += nl
-                line_start = start + part.rfind("\n") + 1
-            continue
-        out.append(Tok(kind, m.group(), line, start - line_start + 1))  # type: ignore[arg-type]
-    return out
+    def _lex(self) -> Token:
+        text = self.text
+        pos = _SKIP.match(text, self.pos).end()
+        if pos >= len(text):
+            self.pos = pos
+            return ("eof", "", len(text), self.src)
+        m = _LEX.match(text, pos)
+        if m is None:  # excluded by the check of the whole source; kept as a guard
+            raise _error_at(self.src, pos, f"unexpected character {text[pos]!r}")
+        self.pos = m.end()
+        return (_KINDS[m.lastindex], m.group(), pos, self.src)
 
+    def peek(self, k: int = 0) -> Token:
+        ahead = self.ahead
+        while len(ahead) <= k:
+            if ahead and ahead[-1][0] == "eof":
+                return ahead[-1]
+            ahead.append(self._lex())
+        return ahead[k]
 
-def tokenize(source: str) -> list[Tok]:
-    """Tokens of ``source`` with 1-based line/column, ending in one ``eof`` token. Pragma and
-    annotation lines are dropped (they don't change the circuit). Simple gate calls at the start
-    of a statement come as single ``simple`` tokens (see ``_SIMPLE``)."""
-    source = _PRAGMA.sub("", source)
-    out: list[Tok] = []
-    append = out.append
-    lex = _LEX.match
-    simple = _SIMPLE.match
-    line, line_start, pos, end = 1, 0, 0, len(source)
-    boundary = FAST_STATEMENTS
-    while pos < end:
-        if boundary:
-            m = simple(source, pos)
-            if m is not None:
-                text = m.group()
-                append(_SimpleTok("simple", text, line, pos - line_start + 1, m.groups()))
-                nl = text.count("\n")
-                if nl:
-                    line += nl
-                    line_start = pos + text.rfind("\n") + 1
-                pos = m.end()
-                continue
-        m = lex(source, pos)
-        if m is None:
-            raise QasmError(f"unexpected character {source[pos]!r}", line, pos - line_start + 1)
-        kind = m.lastgroup
-        start, pos = pos, m.end()
-        text = m.group()
-        if kind in _SKIP:
-            nl = text.count("\n")
-            if nl:
-                line += nl
-                line_start = start + text.rfind("\n") + 1
-            continue
-        append(Tok(kind, text, line, start - line_start + 1))  # type: ignore[arg-type]
-        boundary = FAST_STATEMENTS and kind == "op" and text in _BOUNDARY
-    append(Tok("eof", "", line, pos - line_start + 1))
-    return out
+    def take(self) -> Token:
+        tok = self.peek()
+        if tok[0] != "eof":
+            self.ahead.pop(0)
+        return tok
+
+    def at_boundary(self) -> bool:
+        return not self.ahead
 
 
-def _number(text: str) -> float:
-    t = text.replace("_", "").lower()
-    for prefix, base in (("0b", 2), ("0x", 16), ("0o", 8)):
-        if t.startswith(prefix):
-            return float(int(t[2:], base))
-    return float(t)
+def _error_at(src: _Source, offset: int, message: str) -> QasmError:
+    line, col = src.where(offset)
+    return QasmError(message, line, col)
+
+
+def _tok_error(tok: Token, message: str) -> QasmError:
+    return _error_at(tok[3], tok[2], message)
 
 
 # ------------------------------------------------------------------------------ expressions
@@ -205,167 +171,242 @@ _FUNCTIONS: dict[str, Callable[[float], float]] = {
     "sqrt": math.sqrt,
 }
 
-# An expression tree: ("num", value) | ("var", name, tok) | ("neg", e) | ("call", fn, e, tok)
-# | ("bin", op, a, b, tok)
-Expr = tuple[Any, ...]
+Env = Mapping[str, float]
+Expr = Callable[[Env, "_Reader"], float]
 
 
-@dataclass
-class _Call:
-    """One gate call inside a ``gate`` body, kept unevaluated until the gate is used."""
-
-    modifiers: list[tuple[str, Expr | None]]
-    name: Tok
-    params: list[Expr]
-    args: list[str]
+class _Unevaluable(Exception):
+    """An arithmetic error; reported at the token the reader is on when it evaluates."""
 
 
-@dataclass
-class _GateDef:
-    params: list[str]
-    qubits: list[str]
-    body: list[_Call | Expr]  # an Expr entry is a gphase angle
+def _literal(text: str) -> float:
+    t = text.replace("_", "").lower()
+    if t[:2] in ("0b", "0x", "0o"):
+        return float(int(t[2:], {"0b": 2, "0x": 16, "0o": 8}[t[:2]]))
+    return float(t)
 
 
-# ---------------------------------------------------------------------------------- parsing
+def _arith(op: str, a: float, b: float) -> float:
+    try:
+        if op == "+":
+            return a + b
+        if op == "-":
+            return a - b
+        if op == "*":
+            return a * b
+        if op == "/":
+            return a / b
+        if op == "%":
+            return math.fmod(a, b)
+        return float(a**b)
+    except (ZeroDivisionError, OverflowError, TypeError) as exc:
+        raise _Unevaluable(f"can't evaluate the expression: {exc}") from exc
+
+
+def _const(v: float) -> Expr:
+    return lambda env, rd: v
+
+
+def _name(tok: Token) -> Expr:
+    name = tok[1]
+
+    def look(env: Env, rd: _Reader) -> float:
+        if env and name in env:
+            return env[name]
+        if name in rd.values:
+            return rd.values[name]
+        if name in _CONSTANTS:
+            return _CONSTANTS[name]
+        raise _tok_error(tok, f"unknown name {name!r}")
+
+    return look
+
+
+def _call(tok: Token, arg: Expr) -> Expr:
+    fn = _FUNCTIONS[tok[1]]
+
+    def apply(env: Env, rd: _Reader) -> float:
+        # an error anywhere in the argument is reported as the function's, at its name, as the
+        # reference does (its argument is evaluated inside the handler, qasm.py:319-323)
+        try:
+            return fn(arg(env, rd))
+        except _Unevaluable as exc:
+            raise _tok_error(tok, f"{tok[1]}: {rd.err(str(exc))}") from exc
+        except ValueError as exc:
+            raise _tok_error(tok, f"{tok[1]}: {exc}") from exc
+
+    return apply
+
+
+def _binary(op: str, left: Expr, right: Expr) -> Expr:
+    def apply(env: Env, rd: _Reader) -> float:
+        a = left(env, rd)
+        return _arith(op, a, right(env, rd))
+
+    return apply
+
+
+def _negate(inner: Expr) -> Expr:
+    return lambda env, rd: -inner(env, rd)
+
+
+# binding powers: sums < products < signs < powers (right-associative, may take a sign)
+_INFIX = {"+": 10, "-": 10, "*": 20, "/": 20, "%": 20, "^": 30, "**": 30}
+_SIGN_BP = 25
+_POWER_RIGHT_BP = 24
+
+
+# ------------------------------------------------------------------------------ gate blocks
+
+
+@dataclass(frozen=True)
+class _BodyCall:
+    modifiers: tuple[tuple[str, Expr | None], ...]
+    token: Token
+    params: tuple[Expr, ...]
+    args: tuple[str, ...]
+
+
+@dataclass(frozen=True)
+class _UserGate:
+    name: str
+    params: tuple[str, ...]
+    qubits: tuple[str, ...]
+    body: tuple[_BodyCall | Expr, ...]  # an Expr entry is a gphase angle
+
+
+def _embed_after(op: Matrix, sub: Matrix, positions: tuple[int, ...], k: int) -> Matrix:
+    """``sub`` applied after ``op`` on local qubits ``positions`` of a k-qubit block (local qubit
+    j is matrix bit k-1-j; the first position is ``sub``'s most significant bit): the rows of
+    ``op`` are regrouped so each column of ``sub`` meets the rows it mixes."""
+    m = len(positions)
+    rest = [j for j in range(k) if j not in positions]
+    order = list(positions) + rest
+    t = op.reshape([2] * k + [1 << k]).transpose(order + [k])
+    t = (sub @ t.reshape(1 << m, -1)).reshape([2] * k + [1 << k])
+    back = np.argsort(order).tolist()
+    return np.ascontiguousarray(t.transpose(back + [k]).reshape(1 << k, 1 << k))
+
+
+# ----------------------------------------------------------------------- plain statements
+
+_Q_NUM = r"[+-]?(?:[0-9]+(?:\.[0-9]*)?|\.[0-9]+)(?:[eE][+-]?[0-9]+)?"
+_Q_OPER = r"[A-Za-z_]\w*[ \t]*\[[ \t]*[0-9]+[ \t]*\]"
+_QUICK = re.compile(
+    rf"([A-Za-z_]\w*)(?:[ \t]*\([ \t]*({_Q_NUM}(?:[ \t]*,[ \t]*{_Q_NUM})*)[ \t]*\)[ \t]*|[ \t]+)"
+    rf"({_Q_OPER}(?:[ \t]*,[ \t]*{_Q_OPER})*)[ \t]*;"
+)
+_Q_OPERAND = re.compile(r"([A-Za-z_]\w*)[ \t]*\[[ \t]*([0-9]+)")
+
+_REFUSED = ("for", "while", "def", "defcal", "cal", "extern", "opaque", "return", "break", "continue")
+_CLASSICAL = ("input", "const", "float", "angle", "int", "uint", "bool")
+_STANDARD_INCLUDES = ("qelib1.inc", "stdgates.inc")
+
+# qelib1.inc's relative-phase Toffolis, defined by decomposition rather than as matrices
+_QELIB_BLOCKS = """
+gate rccx a,b,c { u2(0,pi) c; u1(pi/4) c; cx b,c; u1(-pi/4) c; cx a,c; u1(pi/4) c; cx b,c;
+  u1(-pi/4) c; u2(0,pi) c; }
+gate rc3x a,b,c,d { u2(0,pi) d; u1(pi/4) d; cx c,d; u1(-pi/4) d; u2(0,pi) d; cx a,d; u1(pi/4) d;
+  cx b,d; u1(-pi/4) d; cx a,d; u1(pi/4) d; cx b,d; u1(-pi/4) d; u2(0,pi) d; u1(pi/4) d; cx c,d;
+  u1(-pi/4) d; u2(0,pi) d; }
+"""
+_QELIB_GATES: dict[str, _UserGate] | None = None
+
+
+# ------------------------------------------------------------------------------------ reader
+
+Operand = tuple[list[int], bool]  # qubits (or clbits), and whether a whole register was named
 
 
 class _Reader:
-    def __init__(self, source: str, inputs: Mapping[str, float], include: Callable[[str], str] | None) -> None:
-        self.toks = tokenize(source)
-        self.i = 0
+    """One program's state: registers, declared values, gate definitions, and the stream being
+    read (an ``include`` reads another stream into the same state)."""
+
+    def __init__(self, inputs: Mapping[str, float], include: Callable[[str], str] | None) -> None:
         self.inputs = dict(inputs)
+        self.used_inputs: dict[str, float] = {}
         self.include = include
-        # shared with nested includes
         self.qregs: dict[str, Register] = {}
         self.cregs: dict[str, Register] = {}
         self.values: dict[str, float] = {}
-        self.used_inputs: dict[str, float] = {}
-        self.gates: dict[str, _GateDef] = {}
-        self.user_defined: set[str] = set()
         self.num_qubits = 0
         self.num_clbits = 0
-        self.physical = -1
+        self.physical = -1  # highest $n named
+        self.gates: dict[str, _UserGate] = dict(_qelib_gates())
+        self.user_defined: set[str] = set()
+        self.matrices: dict[tuple, Matrix] = {}
+        self.s: _Stream | None = None
 
-    # -- token helpers ---------------------------------------------------------------------
+    # -- token helpers --------------------------------------------------------------------
 
-    def peek(self, ahead: int = 0) -> Tok:
-        j = self.i + ahead
-        return self.toks[j] if j < len(self.toks) else self.toks[-1]
-
-    def take(self) -> Tok:
-        t = self.peek()
-        if t.kind != "eof":
-            self.i += 1
-        return t
-
-    def err(self, message: str, tok: Tok | None = None) -> QasmError:
-        t = tok or self.peek()
-        return QasmError(message, t.line, t.col)
-
-    def is_(self, text: str, ahead: int = 0) -> bool:
-        t = self.peek(ahead)
-        return t.text == text and t.kind in ("op", "id")
+    def err(self, message: str, tok: Token | None = None) -> QasmError:
+        return _tok_error(tok if tok is not None else self.s.peek(), message)  # type: ignore[union-attr]
 
     def accept(self, text: str) -> bool:
-        if self.is_(text):
-            self.i += 1
+        tok = self.s.peek()  # type: ignore[union-attr]
+        if tok[1] == text and tok[0] in ("op", "id"):
+            self.s.take()  # type: ignore[union-attr]
             return True
         return False
 
-    def expect(self, text: str) -> Tok:
-        t = self.peek()
-        if not self.is_(text):
-            found = "the end of the program" if t.kind == "eof" else repr(t.text)
-            raise self.err(f"expected {text!r} but found {found}", t)
-        self.i += 1
-        return t
+    def expect(self, text: str) -> Token:
+        tok = self.s.peek()  # type: ignore[union-attr]
+        if tok[1] != text or tok[0] not in ("op", "id"):
+            found = "the end of the program" if tok[0] == "eof" else repr(tok[1])
+            raise self.err(f"expected {text!r} but found {found}", tok)
+        return self.s.take()  # type: ignore[union-attr]
 
-    def name(self) -> Tok:
-        t = self.take()
-        if t.kind != "id":
-            raise self.err(f"expected a name but found {t.text!r}", t)
-        return t
+    def word(self) -> Token:
+        tok = self.s.take()  # type: ignore[union-attr]
+        if tok[0] != "id":
+            raise self.err(f"expected a name but found {tok[1]!r}", tok)
+        return tok
 
-    # -- expressions -----------------------------------------------------------------------
+    def is_op(self, *texts: str) -> bool:
+        tok = self.s.peek()  # type: ignore[union-attr]
+        return tok[0] == "op" and tok[1] in texts
 
-    def expr(self) -> Expr:
-        node = self.product()
-        while self.peek().kind == "op" and self.peek().text in ("+", "-"):
-            op = self.take()
-            node = ("bin", op.text, node, self.product(), op)
-        return node
+    # -- expressions ----------------------------------------------------------------------
 
-    def product(self) -> Expr:
-        node = self.signed()
-        while self.peek().kind == "op" and self.peek().text in ("*", "/", "%"):
-            op = self.take()
-            node = ("bin", op.text, node, self.signed(), op)
-        return node
-
-    def signed(self) -> Expr:
-        if self.peek().kind == "op" and self.peek().text in ("+", "-"):
-            negative = self.take().text == "-"
-            inner = self.signed()
-            return ("neg", inner) if negative else inner
-        base = self.atom()
-        if self.peek().kind == "op" and self.peek().text in ("**", "^"):
-            op = self.take()
-            return ("bin", "**", base, self.signed(), op)
-        return base
-
-    def atom(self) -> Expr:
-        t = self.take()
-        if t.kind == "num":
-            return ("num", _number(t.text))
-        if t.kind == "op" and t.text == "(":
-            inner = self.expr()
+    def expr(self, rbp: int = 0) -> Expr:
+        s = self.s
+        assert s is not None
+        tok = s.take()
+        kind, text = tok[0], tok[1]
+        if kind == "op" and text in ("-", "+"):
+            inner = self.expr(_SIGN_BP)
+            left = _negate(inner) if text == "-" else inner
+        elif kind == "num":
+            left = _const(_literal(text))
+        elif kind == "op" and text == "(":
+            left = self.expr()
             self.expect(")")
-            return inner
-        if t.kind == "id":
-            if t.text in _FUNCTIONS and self.is_("("):
-                self.take()
+        elif kind == "id":
+            if text in _FUNCTIONS and s.peek()[1] == "(":
+                self.expect("(")
                 arg = self.expr()
                 self.expect(")")
-                return ("call", t.text, arg, t)
-            return ("var", t.text, t)
-        raise self.err(f"expected a number or an expression but found {t.text!r}", t)
+                left = _call(tok, arg)
+            else:
+                left = _name(tok)
+        else:
+            raise self.err(f"expected a number or an expression but found {text!r}", tok)
+        while True:
+            nxt = s.peek()
+            bp = _INFIX.get(nxt[1], 0) if nxt[0] == "op" else 0
+            if bp <= rbp:
+                return left
+            s.take()
+            if bp == 30:  # the exponent is a signed operand and may itself be a power
+                left = _binary("**", left, self.expr(_POWER_RIGHT_BP))
+            else:
+                left = _binary(nxt[1], left, self.expr(bp))
 
-    def value(self, e: Expr, env: Mapping[str, float] | None = None) -> float:
-        kind = e[0]
-        if kind == "num":
-            return float(e[1])
-        if kind == "var":
-            name = e[1]
-            if env is not None and name in env:
-                return env[name]
-            if name in self.values:
-                return self.values[name]
-            if name in _CONSTANTS:
-                return _CONSTANTS[name]
-            raise self.err(f"unknown name {name!r}", e[2])
-        if kind == "neg":
-            return -self.value(e[1], env)
-        if kind == "call":
-            try:
-                return _FUNCTIONS[e[1]](self.value(e[2], env))
-            except ValueError as exc:
-                raise self.err(f"{e[1]}: {exc}", e[3]) from exc
-        op, a, b = e[1], self.value(e[2], env), self.value(e[3], env)
+    def value(self, e: Expr, env: Env | None = None) -> float:
         try:
-            if op == "+":
-                return a + b
-            if op == "-":
-                return a - b
-            if op == "*":
-                return a * b
-            if op == "/":
-                return a / b
-            if op == "%":
-                return math.fmod(a, b)
-            return float(a**b)
-        except (ZeroDivisionError, OverflowError, TypeError) as exc:
-            raise self.err(f"can't evaluate the expression: {exc}", e[4]) from exc
+            return e(env or {}, self)
+        except _Unevaluable as exc:
+            raise self.err(str(exc)) from None
 
     def whole(self, e: Expr, what: str) -> int:
         v = self.value(e)
@@ -373,440 +414,410 @@ class _Reader:
             raise self.err(f"{what} must be a whole number")
         return round(v)
 
-    # -- program ---------------------------------------------------------------------------
+    # -- program --------------------------------------------------------------------------
 
-    def program(self) -> list[Operation]:
-        if self.is_("OPENQASM"):
-            self.take()
-            version = self.take()
-            if version.kind != "num" or version.text.split(".")[0] not in ("2", "3"):
-                raise self.err("only OpenQASM 2 and 3 are supported", version)
-            self.expect(";")
-        ops: list[Operation] = []
-        while self.peek().kind != "eof":
-            ops.extend(self.statement())
-        return ops
+    def read(self, src: _Source) -> list[Operation]:
+        bad = src.first_bad()
+        if bad >= 0:
+            raise _error_at(src, bad, f"unexpected character {src.text[bad]!r}")
+        outer, self.s = self.s, _Stream(src)
+        try:
+            if self.s.peek()[1] == "OPENQASM":
+                self.s.take()
+                tok = self.s.take()
+                if tok[0] != "num" or tok[1].split(".")[0] not in ("2", "3"):
+                    raise self.err("only OpenQASM 2 and 3 are supported", tok)
+                self.expect(";")
+            ops: list[Operation] = []
+            stream = self.s
+            while True:
+                if stream.at_boundary():
+                    plain = self.plain_gate()
+                    if plain is not None:
+                        ops += plain
+                        continue
+                if stream.peek()[0] == "eof":
+                    return ops
+                ops += self.statement(plain_tried=True)
+        finally:
+            self.s = outer
 
-    def body(self) -> list[Operation]:
-        """A braced block or a single statement."""
+    def statement(self, plain_tried: bool = False) -> list[Operation]:
+        s = self.s
+        assert s is not None
+        if not plain_tried and s.at_boundary():
+            plain = self.plain_gate()
+            if plain is not None:
+                return plain
+        tok = s.peek()
+        kind, text = tok[0], tok[1]
+        if kind == "id":
+            handler = _HANDLERS.get(text)
+            if handler is not None:
+                return handler(self, tok)
+            if s.peek(1)[1] in ("=", "[") and text in self.cregs:
+                return self.assignment()
+            return self.gate_statement()
+        if kind == "op" and text == ";":
+            s.take()
+            return []
+        raise self.err(f"unexpected {text!r}" if kind != "eof" else "unexpected end of program", tok)
+
+    def block(self) -> list[Operation]:
+        """A ``{ ... }`` block or a single statement."""
         if not self.accept("{"):
             return self.statement()
         ops: list[Operation] = []
         while not self.accept("}"):
-            if self.peek().kind == "eof":
+            if self.s.peek()[0] == "eof":  # type: ignore[union-attr]
                 raise self.err("this block is never closed with '}'")
-            ops.extend(self.statement())
+            ops += self.statement()
         return ops
 
-    def statement(self) -> list[Operation]:
-        t = self.peek()
-        if t.kind == "simple":
-            return self.simple_statement(t)
-        word = t.text if t.kind == "id" else ""
-        if t.kind == "op" and t.text == ";":
-            self.take()
-            return []
-        if word == "include":
-            return self.include_statement()
-        if word in ("qreg", "creg"):
-            self.take()
-            reg = self.name()
-            self.expect("[")
-            size = self.whole(self.expr(), "a register size")
-            self.expect("]")
-            self.expect(";")
-            self.declare(reg, size, quantum=word == "qreg")
-            return []
-        if word in ("qubit", "bit"):
-            self.take()
-            size = 1
-            if self.accept("["):
-                size = self.whole(self.expr(), "a register size")
-                self.expect("]")
-            reg = self.name()
-            self.declare(reg, size, quantum=word == "qubit")
-            if word == "bit" and self.accept("="):
-                self.expect("measure")
-                qubits = self.qubits()
-                self.expect(";")
-                return self.measures(qubits, self.whole_creg(reg.text))
-            self.expect(";")
-            return []
-        if word == "gate":
-            self.gate_definition()
-            return []
-        if word in ("input", "const", "float", "angle", "int", "uint", "bool"):
-            self.classical()
-            return []
-        if word == "measure":
-            self.take()
-            qubits = self.qubits()
-            if self.accept("->"):
-                bits = self.bits()
-                self.expect(";")
-                return self.measures(qubits, bits)
-            self.expect(";")
-            return [Measure(q, None) for q in qubits[0]]
-        if word == "reset":
-            self.take()
-            qubits = self.qubits()
-            self.expect(";")
-            return [Reset(q) for q in qubits[0]]
-        if word == "barrier":
-            self.take()
-            listed: list[int] = []
-            while not self.is_(";"):
-                listed.extend(self.qubits()[0])
-                if not self.accept(","):
-                    break
-            self.expect(";")
-            return [Barrier(tuple(listed) or tuple(range(self.num_qubits)))]
-        if word == "if":
-            return [self.if_statement()]
-        if word == "box":
-            self.take()
-            if self.accept("["):
-                self.skip_to("]")
-            return self.body()
-        if word == "delay":
-            self.take()
-            self.expect("[")
-            self.skip_to("]")
-            while not self.is_(";") and self.peek().kind != "eof":
-                self.take()
-            self.expect(";")
-            return []
-        if word in UNSUPPORTED:
-            raise self.err(f"'{word}' isn't supported by Lucivant Orrery yet; unroll it or define a gate instead")
-        if word == "gphase":
-            self.take()
-            self.expect("(")
-            angle = self.value(self.expr())
-            self.expect(")")
-            self.expect(";")
-            return [GlobalPhase(angle)]
-        if t.kind == "id" and t.text in self.cregs and (self.is_("=", 1) or self.is_("[", 1)):
-            bits = self.bits()
-            self.expect("=")
-            if not self.accept("measure"):
-                raise self.err("only measurement results can be assigned to bits")
-            qubits = self.qubits()
-            self.expect(";")
-            return self.measures(qubits, bits)
-        if t.kind == "id":
-            fast = self.simple_gate_statement()
-            if fast is not None:
-                return fast
-            return self.gate_statement()
-        raise self.err(f"unexpected {t.text!r}" if t.kind != "eof" else "unexpected end of program")
-
-    def simple_statement(self, tok: Tok) -> list[Operation]:
-        """A ``simple`` token: a built-in gate on in-range, distinct qubits becomes its Gate
-        directly (the same Gate :meth:`gate_statement` makes); anything else is lexed into
-        ordinary tokens and read by the general path, with its usual errors."""
-        name, params_text, args_text = tok.groups  # type: ignore[attr-defined]
-        spec = GATES.get(name)
-        if spec is not None and name not in self.gates:
-            # _SNUM admits only plain decimal literals, which float() reads exactly as _number does
-            params = [float(x) for x in params_text.split(",")] if params_text else []
-            qubits: list[int] = []
-            for reg_name, index_text in _SIMPLE_ARG.findall(args_text):
-                reg = self.qregs.get(reg_name)
-                index = int(index_text)
-                if reg is None or index >= reg.size:
-                    qubits = []
-                    break
-                qubits.append(reg.offset + index)
-            if qubits and len(params) == spec.params and len(qubits) == spec.qubits and len(set(qubits)) == len(qubits):
-                self.i += 1
-                pt = tuple(params)
-                return [Gate(canonical_name(name), tuple(qubits), gate_matrix(name, pt), pt)]
-        # the general path over this statement's ordinary tokens
-        sub = _lex_span(tok.text, tok.line, tok.col)
-        last = sub[-1]
-        saved, saved_i = self.toks, self.i
-        self.toks, self.i = sub + [Tok("eof", "", last.line, last.col + len(last.text))], 0
-        try:
-            ops = self.statement()
-            if self.peek().kind != "eof":
-                raise self.err(f"unexpected {self.peek().text!r}")
-        finally:
-            self.toks, self.i = saved, saved_i
-        self.i += 1
-        return ops
-
-    def simple_gate_statement(self) -> list[Operation] | None:
-        """``name(num, -num, ...) reg[int], reg[int], ...;`` for a built-in gate, read straight
-        from the token list; None (nothing consumed) for anything else, which then takes the
-        general path and its error messages. Produces exactly what :meth:`gate_statement`
-        produces for such a statement."""
-        toks = self.toks
-        j = self.i
-        name = toks[j].text
-        spec = GATES.get(name)
-        if spec is None or name in self.gates:
-            return None
-        j += 1
-        params: list[float] = []
-        tok = toks[j]
-        if tok.text == "(" and tok.kind == "op":
-            j += 1
-            while True:
-                tok = toks[j]
-                negative = tok.text == "-" and tok.kind == "op"
-                if negative:
-                    j += 1
-                    tok = toks[j]
-                if tok.kind != "num":
-                    return None
-                v = _number(tok.text)
-                params.append(-v if negative else v)
-                j += 1
-                sep = toks[j]
-                j += 1
-                if sep.text == ")" and sep.kind == "op":
-                    break
-                if sep.text != "," or sep.kind != "op":
-                    return None
-        qubits: list[int] = []
-        while True:
-            reg_tok = toks[j]
-            reg = self.qregs.get(reg_tok.text) if reg_tok.kind == "id" else None
-            if reg is None or toks[j + 1].text != "[" or toks[j + 2].kind != "num" or toks[j + 3].text != "]":
-                return None
-            text = toks[j + 2].text
-            if not text.isdigit():
-                return None
-            index = int(text)
-            if index >= reg.size:
-                return None
-            qubits.append(reg.offset + index)
-            sep = toks[j + 4]
-            j += 5
-            if sep.kind != "op":
-                return None
-            if sep.text == ";":
-                break
-            if sep.text != ",":
-                return None
-        if len(params) != spec.params or len(qubits) != spec.qubits or len(set(qubits)) != len(qubits):
-            return None
-        self.i = j
-        pt = tuple(params)
-        return [Gate(canonical_name(name), tuple(qubits), gate_matrix(name, pt), pt)]
-
-    def include_statement(self) -> list[Operation]:
-        self.take()
-        file = self.take()
-        if file.kind != "str":
-            raise self.err("include needs a file name in quotes", file)
-        self.expect(";")
-        name = file.text[1:-1]
-        if name in STANDARD_INCLUDES:
-            return []
-        if self.include is None:
-            raise self.err(f"can't include {name!r}: only qelib1.inc and stdgates.inc are built in", file)
-        nested = _Reader(self.include(name), self.inputs, self.include)
-        for attr in ("qregs", "cregs", "values", "gates", "user_defined", "used_inputs"):
-            setattr(nested, attr, getattr(self, attr))
-        nested.num_qubits, nested.num_clbits, nested.physical = self.num_qubits, self.num_clbits, self.physical
-        ops = nested.program()
-        self.num_qubits, self.num_clbits, self.physical = nested.num_qubits, nested.num_clbits, nested.physical
-        return ops
-
-    def skip_to(self, closing: str) -> None:
+    def skip_past(self, closing: str) -> None:
         depth = 0
         while True:
-            t = self.take()
-            if t.kind == "eof":
+            tok = self.s.take()  # type: ignore[union-attr]
+            if tok[0] == "eof":
                 raise self.err(f"expected {closing!r}")
-            if t.text == "[":
+            if tok[1] == "[":
                 depth += 1
-            elif t.text == closing:
+            elif tok[1] == closing:
                 if depth == 0:
                     return
                 depth -= 1
 
-    # -- declarations ----------------------------------------------------------------------
+    # -- the plain form -------------------------------------------------------------------
 
-    def declare(self, reg: Tok, size: int, *, quantum: bool) -> None:
-        if reg.text in self.qregs or reg.text in self.cregs or reg.text in self.values:
-            raise self.err(f"{reg.text!r} is already declared", reg)
+    def plain_gate(self) -> list[Operation] | None:
+        """A built-in gate with literal parameters on indexed qubits, read by one match; None
+        when the statement has another form or fails a check (the general path then reads it
+        and reports the error)."""
+        s = self.s
+        assert s is not None
+        if not PLAIN_STATEMENTS:
+            return None
+        text = s.text
+        start = _SKIP.match(text, s.pos).end()
+        m = _QUICK.match(text, start)
+        if m is None:
+            return None
+        name, args, operands = m.group(1, 2, 3)
+        spec = GATES.get(name)
+        if spec is None or name in _HANDLERS or name in self.gates:
+            return None
+        params = tuple(float(a) for a in args.split(",")) if args else ()
+        if len(params) != spec.params:
+            return None
+        qubits = []
+        for reg_name, idx in _Q_OPERAND.findall(operands):
+            reg = self.qregs.get(reg_name)
+            i = int(idx)
+            if reg is None or i >= reg.size:
+                return None
+            qubits.append(reg.offset + i)
+        if len(qubits) != spec.qubits or len(set(qubits)) != len(qubits):
+            return None
+        s.pos = m.end()
+        return [Gate(canonical_name(name), tuple(qubits), self.builtin(name, params), params)]
+
+    def builtin(self, name: str, params: tuple[float, ...]) -> Matrix:
+        key = (name, tuple(round(p, 14) for p in params))
+        mat = self.matrices.get(key)
+        if mat is None:
+            mat = self.matrices[key] = gate_matrix(name, params)
+        return mat
+
+    # -- declarations ---------------------------------------------------------------------
+
+    def declare(self, name: Token, size: int, quantum: bool) -> None:
+        n = name[1]
+        if n in self.qregs or n in self.cregs or n in self.values:
+            raise self.err(f"{n!r} is already declared", name)
         if size < 1:
-            raise self.err("a register needs at least one bit", reg)
+            raise self.err("a register needs at least one bit", name)
         if quantum:
             if self.physical >= 0:
-                raise self.err("a program can't mix declared qubits with physical qubits ($n)", reg)
+                raise self.err("a program can't mix declared qubits with physical qubits ($n)", name)
             if self.num_qubits + size > MAX_QUBITS:
-                raise self.err(f"more than {MAX_QUBITS} qubits", reg)
-            self.qregs[reg.text] = Register(reg.text, self.num_qubits, size)
+                raise self.err(f"more than {MAX_QUBITS} qubits", name)
+            self.qregs[n] = Register(n, self.num_qubits, size)
             self.num_qubits += size
         else:
-            self.cregs[reg.text] = Register(reg.text, self.num_clbits, size)
+            self.cregs[n] = Register(n, self.num_clbits, size)
             self.num_clbits += size
 
-    def classical(self) -> None:
+    def st_register2(self, tok: Token) -> list[Operation]:
+        """``qreg q[n];`` / ``creg c[n];``"""
+        self.s.take()  # type: ignore[union-attr]
+        name = self.word()
+        self.expect("[")
+        size = self.whole(self.expr(), "a register size")
+        self.expect("]")
+        self.expect(";")
+        self.declare(name, size, tok[1] == "qreg")
+        return []
+
+    def st_register3(self, tok: Token) -> list[Operation]:
+        """``qubit[n] q;`` / ``bit[n] c;`` / ``bit[n] c = measure q;``"""
+        self.s.take()  # type: ignore[union-attr]
+        size = 1
+        if self.accept("["):
+            size = self.whole(self.expr(), "a register size")
+            self.expect("]")
+        name = self.word()
+        self.declare(name, size, tok[1] == "qubit")
+        if tok[1] == "bit" and self.accept("="):
+            self.expect("measure")
+            qubits = self.qubit_operand()
+            self.expect(";")
+            return self.measures(qubits, self.whole_creg(name[1]))
+        self.expect(";")
+        return []
+
+    def st_classical(self, tok: Token) -> list[Operation]:
+        """``input float x;``, ``const float k = ...;``, ``angle a = ...;`` and the like."""
         is_input = self.accept("input")
         self.accept("const")
-        kind = self.name()
-        if kind.text not in ("float", "angle", "int", "uint", "bool"):
-            raise self.err(f"'{kind.text}' parameters aren't supported; use float or angle", kind)
+        kind = self.word()
+        if kind[1] not in ("float", "angle", "int", "uint", "bool"):
+            raise self.err(f"'{kind[1]}' parameters aren't supported; use float or angle", kind)
         if self.accept("["):
-            self.skip_to("]")
-        var = self.name()
-        if var.text in self.values:
-            raise self.err(f"{var.text!r} is already declared", var)
+            self.skip_past("]")
+        name = self.word()
+        if name[1] in self.values:
+            raise self.err(f"{name[1]!r} is already declared", name)
         if is_input:
-            if var.text not in self.inputs:
-                raise self.err(f"the program needs a value for its input {var.text!r}", var)
-            v = float(self.inputs[var.text])
-            self.used_inputs[var.text] = v
+            if name[1] not in self.inputs:
+                raise self.err(f"the program needs a value for its input {name[1]!r}", name)
+            v = float(self.inputs[name[1]])
+            self.used_inputs[name[1]] = v
         else:
             self.expect("=")
             v = self.value(self.expr())
         self.expect(";")
-        self.values[var.text] = v
+        self.values[name[1]] = v
+        return []
 
-    # -- operands --------------------------------------------------------------------------
+    def st_include(self, tok: Token) -> list[Operation]:
+        self.s.take()  # type: ignore[union-attr]
+        name = self.s.take()  # type: ignore[union-attr]
+        if name[0] != "str":
+            raise self.err("include needs a file name in quotes", name)
+        self.expect(";")
+        file = name[1][1:-1]
+        if file in _STANDARD_INCLUDES:
+            return []
+        if self.include is None:
+            raise self.err(f"can't include {file!r}: only qelib1.inc and stdgates.inc are built in", name)
+        return self.read(_Source(self.include(file)))
 
-    def index_list(self, size: int, reg: Tok) -> list[int]:
-        """The inside of ``name[...]`` after the opening bracket, resolved to positions."""
+    # -- operands -------------------------------------------------------------------------
+
+    def index_list(self, size: int, reg: Token) -> list[int]:
+        """The part of ``reg[...]`` after the bracket: an index, a range or a ``{...}`` set."""
         if self.accept("{"):
             picked = [self.whole(self.expr(), "an index")]
             while self.accept(","):
                 picked.append(self.whole(self.expr(), "an index"))
             self.expect("}")
         else:
-            start = self.whole(self.expr(), "an index")
+            first = self.whole(self.expr(), "an index")
             if self.accept(":"):
-                second = self.whole(self.expr(), "an index")
+                bound = self.whole(self.expr(), "an index")
+                step, stop = 1, bound
                 if self.accept(":"):
-                    step, stop = second, self.whole(self.expr(), "an index")
-                else:
-                    step, stop = 1, second
+                    step, stop = bound, self.whole(self.expr(), "an index")
                 if step == 0:
                     raise self.err("a range's step can't be 0", reg)
-                picked = list(range(start, stop + (1 if step > 0 else -1), step))
+                picked = list(range(first, stop + (1 if step > 0 else -1), step))
             else:
-                picked = [start]
+                picked = [first]
         self.expect("]")
         out = []
         for i in picked:
             j = i + size if i < 0 else i
             if not 0 <= j < size:
-                raise self.err(f"index {i} is outside {reg.text!r}, which has {size}", reg)
+                raise self.err(f"index {i} is outside {reg[1]!r}, which has {size}", reg)
             out.append(j)
         return out
 
-    def qubits(self, local: Mapping[str, int] | None = None) -> tuple[list[int], bool]:
-        """Qubits an operand names, and whether it was a whole register (broadcast)."""
-        t = self.take()
-        if t.kind == "phys":
+    def qubit_operand(self) -> Operand:
+        tok = self.s.take()  # type: ignore[union-attr]
+        if tok[0] == "phys":
             if self.qregs:
-                raise self.err("a program can't mix declared qubits with physical qubits ($n)", t)
-            q = int(t.text[1:])
-            if q >= MAX_QUBITS:
-                raise self.err(f"more than {MAX_QUBITS} qubits", t)
-            self.physical = max(self.physical, q)
+                raise self.err("a program can't mix declared qubits with physical qubits ($n)", tok)
+            n = int(tok[1][1:])
+            if n >= MAX_QUBITS:
+                raise self.err(f"more than {MAX_QUBITS} qubits", tok)
+            self.physical = max(self.physical, n)
             self.num_qubits = self.physical + 1
-            return [q], False
-        if t.kind != "id":
-            raise self.err(f"expected a qubit but found {t.text!r}", t)
-        if local is not None:
-            if t.text not in local:
-                raise self.err(f"{t.text!r} isn't an argument of this gate", t)
-            return [local[t.text]], False
-        reg = self.qregs.get(t.text)
+            return [n], False
+        if tok[0] != "id":
+            raise self.err(f"expected a qubit but found {tok[1]!r}", tok)
+        reg = self.qregs.get(tok[1])
         if reg is None:
-            raise self.err(f"unknown qubit register {t.text!r}", t)
+            raise self.err(f"unknown qubit register {tok[1]!r}", tok)
         if self.accept("["):
-            return [reg.offset + i for i in self.index_list(reg.size, t)], False
+            return [reg.offset + i for i in self.index_list(reg.size, tok)], False
         return list(range(reg.offset, reg.offset + reg.size)), True
 
-    def bits(self) -> tuple[list[int], bool]:
-        t = self.name()
-        reg = self.cregs.get(t.text)
+    def creg_operand(self, tok: Token) -> Operand:
+        reg = self.cregs.get(tok[1])
         if reg is None:
-            raise self.err(f"unknown classical register {t.text!r}", t)
+            raise self.err(f"unknown classical register {tok[1]!r}", tok)
         if self.accept("["):
-            return [reg.offset + i for i in self.index_list(reg.size, t)], False
+            return [reg.offset + i for i in self.index_list(reg.size, tok)], False
         return list(range(reg.offset, reg.offset + reg.size)), True
 
-    def whole_creg(self, name: str) -> tuple[list[int], bool]:
-        reg = self.cregs[name]
+    def whole_creg(self, name: str) -> Operand:
+        reg = self.cregs.get(name)
+        if reg is None:
+            raise self.err(f"unknown classical register {name!r}")
         return list(range(reg.offset, reg.offset + reg.size)), True
 
-    def measures(self, qubits: tuple[list[int], bool], bits: tuple[list[int], bool]) -> list[Operation]:
+    def measures(self, qubits: Operand, bits: Operand) -> list[Operation]:
         if len(qubits[0]) != len(bits[0]):
             raise self.err(f"can't measure {len(qubits[0])} qubits into {len(bits[0])} bits")
         return [Measure(q, c) for q, c in zip(qubits[0], bits[0])]
 
-    # -- conditions ------------------------------------------------------------------------
+    # -- measurement, reset, barrier, timing ----------------------------------------------
 
-    def if_statement(self) -> IfElse:
+    def st_measure(self, tok: Token) -> list[Operation]:
+        self.s.take()  # type: ignore[union-attr]
+        qubits = self.qubit_operand()
+        if self.accept("->"):
+            bits = self.creg_operand(self.word())
+            self.expect(";")
+            return self.measures(qubits, bits)
+        self.expect(";")
+        return [Measure(q, None) for q in qubits[0]]
+
+    def assignment(self) -> list[Operation]:
+        """``c = measure q;`` / ``c[0] = measure q[0];``"""
+        bits = self.creg_operand(self.word())
+        self.expect("=")
+        if not self.accept("measure"):
+            raise self.err("only measurement results can be assigned to bits")
+        qubits = self.qubit_operand()
+        self.expect(";")
+        return self.measures(qubits, bits)
+
+    def st_reset(self, tok: Token) -> list[Operation]:
+        self.s.take()  # type: ignore[union-attr]
+        qubits = self.qubit_operand()
+        self.expect(";")
+        return [Reset(q) for q in qubits[0]]
+
+    def st_barrier(self, tok: Token) -> list[Operation]:
+        s = self.s
+        assert s is not None
+        s.take()
+        qubits: list[int] = []
+        while s.peek()[1] != ";":
+            qubits += self.qubit_operand()[0]
+            if not self.accept(","):
+                break
+        self.expect(";")
+        return [Barrier(tuple(qubits) or tuple(range(self.num_qubits)))]
+
+    def st_box(self, tok: Token) -> list[Operation]:
+        self.s.take()  # type: ignore[union-attr]
+        if self.accept("["):
+            self.skip_past("]")
+        return self.block()
+
+    def st_delay(self, tok: Token) -> list[Operation]:
+        s = self.s
+        assert s is not None
+        s.take()
+        self.expect("[")
+        self.skip_past("]")
+        while s.peek()[1] != ";" and s.peek()[0] != "eof":
+            s.take()
+        self.expect(";")
+        return []
+
+    def st_refused(self, tok: Token) -> list[Operation]:
+        raise self.err(f"'{tok[1]}' isn't supported by Lucivant Orrery yet; unroll it or define a gate instead", tok)
+
+    def st_gphase(self, tok: Token) -> list[Operation]:
+        self.s.take()  # type: ignore[union-attr]
+        self.expect("(")
+        angle = self.value(self.expr())
+        self.expect(")")
+        self.expect(";")
+        return [GlobalPhase(angle)]
+
+    # -- conditions -----------------------------------------------------------------------
+
+    def st_if(self, tok: Token) -> list[Operation]:
+        s = self.s
+        assert s is not None
         self.expect("if")
         self.expect("(")
         negate = self.accept("!")
-        reg = self.name()
-        if reg.text not in self.cregs:
-            raise self.err(f"unknown classical register {reg.text!r}", reg)
-        self.i -= 1
-        bits = self.bits()[0]
+        reg = self.word()
+        bits, _ = self.creg_operand(reg)
         value = 1
-        if self.is_("==") or self.is_("!="):
-            if self.take().text == "!=":
+        if s.peek()[1] in ("==", "!="):
+            if s.take()[1] == "!=":
                 negate = not negate
-            v = self.take()
-            if v.kind == "num":
-                value = int(_number(v.text))
-            elif v.text in ("true", "false"):
-                value = int(v.text == "true")
+            rhs = s.take()
+            if rhs[0] == "num":
+                value = int(_literal(rhs[1]))
+            elif rhs[1] in ("true", "false"):
+                value = int(rhs[1] == "true")
             else:
-                raise self.err("a condition compares with a whole number", v)
+                raise self.err("a condition compares with a whole number", rhs)
         elif len(bits) != 1:
             raise self.err("compare a register with a value, e.g. if (c == 1)", reg)
         self.expect(")")
-        then = self.body()
+        then = self.block()
         otherwise: list[Operation] = []
         if self.accept("else"):
-            otherwise = self.body()
-        return IfElse(Condition(tuple(bits), value, negate), tuple(then), tuple(otherwise))
+            otherwise = self.block()
+        return [IfElse(Condition(tuple(bits), value, negate), tuple(then), tuple(otherwise))]
 
-    # -- gates -----------------------------------------------------------------------------
+    # -- gate applications ----------------------------------------------------------------
 
-    def modifiers(self) -> list[tuple[str, Expr | None]]:
+    def modifier_list(self) -> list[tuple[str, Expr | None]]:
+        s = self.s
+        assert s is not None
         mods: list[tuple[str, Expr | None]] = []
-        while self.peek().kind == "id" and self.peek().text in ("ctrl", "negctrl", "inv", "pow"):
-            # only a modifier when an '@' follows (after an optional parenthesised argument)
-            ahead = 1
-            if self.is_("(", 1):
+        while True:
+            head = s.peek()
+            if head[0] != "id" or head[1] not in ("ctrl", "negctrl", "inv", "pow"):
+                return mods
+            # a modifier, with its optional (argument), is followed by '@'
+            k = 1
+            if s.peek(1)[1] == "(":
                 depth = 0
                 while True:
-                    t = self.peek(ahead)
-                    if t.kind == "eof":
-                        break
-                    if t.text == "(":
+                    t = s.peek(k)
+                    if t[1] == "(":
                         depth += 1
-                    elif t.text == ")":
+                    elif t[1] == ")":
                         depth -= 1
                         if depth == 0:
                             break
-                    ahead += 1
-                ahead += 1
-            if not self.is_("@", ahead):
-                break
-            word = self.take().text
+                    elif t[0] == "eof":
+                        break
+                    k += 1
+                k += 1
+            if s.peek(k)[1] != "@":
+                return mods
+            s.take()
             arg = None
             if self.accept("("):
                 arg = self.expr()
                 self.expect(")")
             self.expect("@")
-            mods.append((word, arg))
-        return mods
+            mods.append((head[1], arg))
 
-    def call_head(self) -> tuple[list[tuple[str, Expr | None]], Tok, list[Expr]]:
-        mods = self.modifiers()
-        name = self.name()
+    def call_head(self) -> tuple[list[tuple[str, Expr | None]], Token, list[Expr]]:
+        mods = self.modifier_list()
+        name = self.word()
         params: list[Expr] = []
         if self.accept("(") and not self.accept(")"):
             params.append(self.expr())
@@ -816,40 +827,41 @@ class _Reader:
         return mods, name, params
 
     def gate_statement(self) -> list[Operation]:
+        s = self.s
+        assert s is not None
         mods, name, exprs = self.call_head()
         params = tuple(self.value(e) for e in exprs)
-        operands: list[tuple[list[int], bool]] = []
-        while not self.is_(";"):
-            operands.append(self.qubits())
+        operands: list[Operand] = []
+        while s.peek()[1] != ";":
+            operands.append(self.qubit_operand())
             if not self.accept(","):
                 break
         self.expect(";")
-        if name.text == "gphase" and not operands:
+        if name[1] == "gphase" and not operands:
             if len(params) != 1:
                 raise self.err("gphase takes one angle", name)
             return [GlobalPhase(params[0])]
         matrix, arity = self.resolve(mods, name, params)
         if len(operands) != arity:
-            raise self.err(f"{name.text} acts on {arity} qubit(s) but was given {len(operands)}", name)
-        widths = {len(qs) for qs, whole in operands if whole or len(qs) > 1}
-        if len(widths) > 1:
+            raise self.err(f"{name[1]} acts on {arity} qubit(s) but was given {len(operands)}", name)
+        sizes = {len(qs) for qs, whole in operands if whole or len(qs) > 1}
+        if len(sizes) > 1:
             raise self.err("registers of different sizes can't be combined in one gate", name)
-        times = widths.pop() if widths else 1
-        label = "".join(f"{m}@" for m, _ in mods) + canonical_name(name.text)
-        ops: list[Operation] = []
+        times = sizes.pop() if sizes else 1
+        label = "".join(f"{m}@" for m, _ in mods) + canonical_name(name[1])
+        out: list[Operation] = []
         for i in range(times):
-            qs = tuple(q[i] if len(q) > 1 else q[0] for q, _ in operands)
-            if len(set(qs)) != len(qs):
-                raise self.err(f"{name.text} is given the same qubit twice", name)
-            ops.append(Gate(label, qs, matrix, params))
-        return ops
+            qubits = tuple(qs[i] if len(qs) > 1 else qs[0] for qs, _ in operands)
+            if len(set(qubits)) != len(qubits):
+                raise self.err(f"{name[1]} is given the same qubit twice", name)
+            out.append(Gate(label, qubits, matrix, params))
+        return out
 
     def resolve(
-        self, mods: list[tuple[str, Expr | None]], name: Tok, params: tuple[float, ...], env: Mapping[str, float] | None = None
+        self, mods: list[tuple[str, Expr | None]] | tuple, name: Token, params: tuple[float, ...], env: Env | None = None
     ) -> tuple[Matrix, int]:
-        """The matrix and arity of ``mods name(params)``; the modifier nearest the name applies
-        first."""
-        matrix, arity = self.base(name, params)
+        """The matrix and width of a gate with its modifiers, applied innermost first."""
+        matrix, arity = self.base_gate(name, params)
         for word, arg in reversed(mods):
             if word == "inv":
                 matrix = matrix.conj().T
@@ -858,133 +870,147 @@ class _Reader:
                     raise self.err("pow needs an exponent, e.g. pow(2) @", name)
                 matrix = matrix_power(matrix, self.value(arg, env))
             else:
-                count = 1 if arg is None else self.whole(arg, "the number of controls")
+                if arg is None:
+                    count = 1
+                else:
+                    # evaluated among the declared values only, not the gate's parameters, as
+                    # the reference does (qasm.py:742)
+                    count = self.whole(arg, "the number of controls")
+                grow = controlled if word == "ctrl" else negcontrolled
                 for _ in range(count):
-                    matrix = controlled(matrix) if word == "ctrl" else negcontrolled(matrix)
+                    matrix = grow(matrix)
                 arity += count
         return matrix, arity
 
-    def base(self, name: Tok, params: tuple[float, ...]) -> tuple[Matrix, int]:
-        user = self.gates.get(name.text)
-        if user is not None:
+    def base_gate(self, name: Token, params: tuple[float, ...]) -> tuple[Matrix, int]:
+        n = name[1]
+        user = self.gates.get(n)
+        if user is not None and (n in self.user_defined or n not in GATES):
             if len(params) != len(user.params):
-                raise self.err(f"{name.text} takes {len(user.params)} parameter(s) but was given {len(params)}", name)
-            return self.user_unitary(user, params), len(user.qubits)
-        spec = GATES.get(name.text)
+                raise self.err(f"{n} takes {len(user.params)} parameter(s) but was given {len(params)}", name)
+            key = (n, tuple(round(p, 14) for p in params))
+            mat = self.matrices.get(key)
+            if mat is None:
+                mat = self.matrices[key] = self.block_unitary(user, params)
+            return mat, len(user.qubits)
+        spec = GATES.get(n)
         if spec is None:
-            raise self.err(f"unknown gate {name.text!r}", name)
+            raise self.err(f"unknown gate {n!r}", name)
         if len(params) != spec.params:
-            raise self.err(f"{name.text} takes {spec.params} parameter(s) but was given {len(params)}", name)
-        return gate_matrix(name.text, params), spec.qubits
+            raise self.err(f"{n} takes {spec.params} parameter(s) but was given {len(params)}", name)
+        return self.builtin(n, params), spec.qubits
 
-    def user_unitary(self, gate: _GateDef, params: tuple[float, ...]) -> Matrix:
-        """The unitary of a user gate: its body applied in order to the identity."""
+    def block_unitary(self, gate: _UserGate, params: tuple[float, ...]) -> Matrix:
         k = len(gate.qubits)
         env = dict(zip(gate.params, params))
-        where = {q: i for i, q in enumerate(gate.qubits)}
-        op = np.eye(2**k, dtype=np.complex128)
+        local = {q: i for i, q in enumerate(gate.qubits)}
+        op = np.eye(1 << k, dtype=np.complex128)
         for item in gate.body:
-            if not isinstance(item, _Call):
+            if not isinstance(item, _BodyCall):
                 op = op * np.exp(1j * self.value(item, env))
                 continue
             values = tuple(self.value(p, env) for p in item.params)
-            sub, arity = self.resolve(item.modifiers, item.name, values, env)
+            sub, arity = self.resolve(item.modifiers, item.token, values, env)
             if len(item.args) != arity:
-                raise self.err(f"{item.name.text} acts on {arity} qubit(s) but was given {len(item.args)}", item.name)
-            positions = [where[a] for a in item.args]
+                raise self.err(f"{item.token[1]} acts on {arity} qubit(s) but was given {len(item.args)}", item.token)
+            positions = tuple(local[a] for a in item.args)
             if len(set(positions)) != len(positions):
-                raise self.err(f"{item.name.text} is given the same qubit twice", item.name)
-            op = embed(sub, positions, k) @ op
+                raise self.err(f"{item.token[1]} is given the same qubit twice", item.token)
+            op = _embed_after(op, sub, positions, k)
         return op
 
-    def gate_definition(self) -> None:
+    def st_gate(self, tok: Token) -> list[Operation]:
+        """A ``gate name(params) a, b { ... }`` definition."""
+        s = self.s
+        assert s is not None
         self.expect("gate")
-        name = self.name()
-        if name.text in self.user_defined:
-            raise self.err(f"gate {name.text!r} is already defined", name)
+        name = self.word()
+        if name[1] in self.user_defined:
+            raise self.err(f"gate {name[1]!r} is already defined", name)
         params: list[str] = []
         if self.accept("(") and not self.accept(")"):
-            params.append(self.name().text)
+            params.append(self.word()[1])
             while self.accept(","):
-                params.append(self.name().text)
+                params.append(self.word()[1])
             self.expect(")")
-        qubits = [self.name().text]
+        qubits = [self.word()[1]]
         while self.accept(","):
-            qubits.append(self.name().text)
+            qubits.append(self.word()[1])
         if len(set(qubits)) != len(qubits):
             raise self.err("a gate's qubit arguments must have different names", name)
         self.expect("{")
-        body: list[_Call | Expr] = []
+        body: list[_BodyCall | Expr] = []
         while not self.accept("}"):
-            if self.peek().kind == "eof":
-                raise self.err(f"gate {name.text!r} is never closed with '}}'", name)
-            if self.is_("barrier"):
-                while self.take().text != ";":
-                    pass
+            head = s.peek()
+            if head[0] == "eof":
+                raise self.err(f"gate {name[1]!r} is never closed with '}}'", name)
+            if head[1] == "barrier":
+                while s.take()[1] != ";":
+                    if s.peek()[0] == "eof":
+                        raise self.err(f"gate {name[1]!r} is never closed with '}}'", name)
                 continue
-            mods, callee, exprs = self.call_head()
-            if callee.text == "gphase":
+            mods, call, exprs = self.call_head()
+            if call[1] == "gphase":
                 self.expect(";")
                 if len(exprs) != 1:
-                    raise self.err("gphase takes one angle", callee)
+                    raise self.err("gphase takes one angle", call)
                 body.append(exprs[0])
                 continue
             args: list[str] = []
-            while not self.is_(";"):
-                a = self.name()
-                if a.text not in qubits:
-                    raise self.err(f"{a.text!r} isn't an argument of gate {name.text!r}", a)
-                args.append(a.text)
+            while s.peek()[1] != ";":
+                arg = self.word()
+                if arg[1] not in qubits:
+                    raise self.err(f"{arg[1]!r} isn't an argument of gate {name[1]!r}", arg)
+                args.append(arg[1])
                 if not self.accept(","):
                     break
             self.expect(";")
-            if callee.text == name.text:
-                raise self.err("a gate can't call itself", callee)
-            body.append(_Call(mods, callee, exprs, args))
-        self.gates[name.text] = _GateDef(params, qubits, body)
-        self.user_defined.add(name.text)
-
-    def define_prelude(self) -> None:
-        """The relative-phase Toffolis qelib1.inc defines by decomposition (not user gates, so a
-        program may redefine them); parsed once per process."""
-        global _PRELUDE_GATES
-        if _PRELUDE_GATES is None:
-            saved = (self.toks, self.i)
-            self.toks, self.i = tokenize(QELIB_DECOMPOSED), 0
-            while self.peek().kind != "eof":
-                self.gate_definition()
-            self.toks, self.i = saved
-            self.user_defined.clear()
-            _PRELUDE_GATES = dict(self.gates)
-        self.gates.update(_PRELUDE_GATES)
+            if call[1] == name[1]:
+                raise self.err("a gate can't call itself", call)
+            body.append(_BodyCall(tuple(mods), call, tuple(exprs), tuple(args)))
+        self.gates[name[1]] = _UserGate(name[1], tuple(params), tuple(qubits), tuple(body))
+        self.user_defined.add(name[1])
+        return []
 
 
-def embed(sub: Matrix, positions: list[int], k: int) -> Matrix:
-    """``sub`` acting on local qubits ``positions`` (first = its most significant bit) of a
-    k-qubit register whose local qubit j is bit k-1-j of the index, as a 2^k x 2^k matrix."""
-    m = len(positions)
-    d = 2**k
-    rows = np.arange(d)
-    # index bits of the positions, most significant first
-    sub_index = np.zeros(d, dtype=np.int64)
-    for i, p in enumerate(positions):
-        sub_index |= ((rows >> (k - 1 - p)) & 1) << (m - 1 - i)
-    mask = 0
-    for p in positions:
-        mask |= 1 << (k - 1 - p)
-    rest = rows & ~mask
-    same_rest = rest[:, None] == rest[None, :]
-    return np.where(same_rest, sub[sub_index[:, None], sub_index[None, :]], 0).astype(np.complex128)
+_HANDLERS: dict[str, Callable[[_Reader, Token], list[Operation]]] = {
+    "include": _Reader.st_include,
+    "qreg": _Reader.st_register2,
+    "creg": _Reader.st_register2,
+    "qubit": _Reader.st_register3,
+    "bit": _Reader.st_register3,
+    "gate": _Reader.st_gate,
+    "measure": _Reader.st_measure,
+    "reset": _Reader.st_reset,
+    "barrier": _Reader.st_barrier,
+    "if": _Reader.st_if,
+    "box": _Reader.st_box,
+    "delay": _Reader.st_delay,
+    "gphase": _Reader.st_gphase,
+    **{w: _Reader.st_classical for w in _CLASSICAL},
+    **{w: _Reader.st_refused for w in _REFUSED},
+}
 
 
-_PRELUDE_GATES: dict[str, _GateDef] | None = None
+def _qelib_gates() -> dict[str, _UserGate]:
+    """qelib1.inc's decomposed gates, read once per process. They are not user definitions:
+    a program may define gates of the same names."""
+    global _QELIB_GATES
+    if _QELIB_GATES is None:
+        _QELIB_GATES = {}
+        rd = _Reader({}, None)
+        rd.read(_Source(_QELIB_BLOCKS))
+        _QELIB_GATES = dict(rd.gates)
+    return _QELIB_GATES
+
 
 # --------------------------------------------------------------------------------------- API
 
 
 def parse(source: str, params: Mapping[str, float] | None = None, *, include_dir: str | Path | None = None) -> Circuit:
-    """Read OpenQASM 2 or 3 source into a :class:`Circuit`. ``params`` gives the values of the
-    program's ``input`` parameters; ``include_dir`` permits includes from inside that folder."""
+    """Read OpenQASM 2 or 3 source into a :class:`Circuit` (qasm.py:842-879). ``params`` gives
+    the values of the program's ``input`` parameters; ``include_dir`` permits includes of files
+    inside that folder (never outside it)."""
     include: Callable[[str], str] | None = None
     if include_dir is not None:
         root = Path(include_dir).resolve()
@@ -995,26 +1021,29 @@ def parse(source: str, params: Mapping[str, float] | None = None, *, include_dir
                 raise QasmError(f"include {name!r} is outside {root}")
             return path.read_text(encoding="utf-8")
 
-    reader = _Reader(source, params or {}, include)
-    reader.define_prelude()
-    ops = reader.program()
-    unused = set(params or {}) - set(reader.used_inputs)
+    rd = _Reader(params or {}, include)
+    ops = rd.read(_Source(source))
+    unused = set(params or {}) - set(rd.used_inputs)
     if unused:
         raise QasmError(f"the program has no input named {', '.join(sorted(unused))}")
-    qregs = tuple(reader.qregs.values())
-    if reader.physical >= 0:
-        qregs = (Register("$", 0, reader.num_qubits),)
+    qregs = tuple(rd.qregs.values())
+    if rd.physical >= 0:
+        qregs = (Register("$", 0, rd.num_qubits),)
     return Circuit(
-        num_qubits=reader.num_qubits,
-        num_clbits=reader.num_clbits,
+        num_qubits=rd.num_qubits,
+        num_clbits=rd.num_clbits,
         operations=tuple(ops),
         qregs=qregs,
-        cregs=tuple(reader.cregs.values()),
-        inputs=dict(reader.used_inputs),
+        cregs=tuple(rd.cregs.values()),
+        inputs=dict(rd.used_inputs),
     )
 
 
 def load(path: str | Path, params: Mapping[str, float] | None = None) -> Circuit:
-    """Read an OpenQASM file; its includes resolve beside it."""
+    """Read an OpenQASM file; includes resolve beside it (qasm.py:882-885)."""
     p = Path(path)
     return parse(p.read_text(encoding="utf-8"), params, include_dir=p.parent)
+
+
+__all__ = ["MAX_QUBITS", "QasmError", "load", "parse"]
+
