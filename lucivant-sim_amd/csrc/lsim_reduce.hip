@@ -6,6 +6,8 @@
 // state with wavefront shuffles and LDS, writes one partial per value, and a second kernel adds
 // the partials of each state in a fixed order. Repeating a call gives the same bits, which keeps
 // seeded trajectory decisions reproducible run to run. Accumulation is float64 for both dtypes.
+#include <string.h>
+
 #include <mutex>
 
 #include "lsim_internal.h"
@@ -166,6 +168,120 @@ __global__ void __launch_bounds__(kBlock)
   block_sum_store<1>(acc, partial + ((size_t)b * nterms + t) * nb + blockIdx.x);
 }
 
+// ---- Pauli strings in groups: one read of the state for every string of a group ----------
+//
+// <psi|P|psi> = sum_j Re[i^w conj(psi[j ^ X]) psi[j]] (-1)^popcount(j & Z), w = popcount(X & Z).
+// The strings of a group have X masks whose span V has at most kGroupDims dimensions above the
+// low `low` bits. Take W = V + span(bits 0..low-1), with a basis b_i of V reduced to zero low
+// bits and distinct pivot bits. Every address is j = r ^ B(c) ^ l: r a coset representative (zero
+// at the low bits and the pivots), B(c) = xor of the b_i selected by c, l < 2^low. One workgroup
+// stages a coset, 2^(low + dims) = 4096 amplitudes read as runs of 2^low contiguous ones, in LDS;
+// for every string the partner j ^ X is r ^ B(c ^ c_t) ^ (l ^ l_t), inside the same tile, at
+// tile index e ^ tx with e = (c << low) | l. The sign factors as well: parity(j & Z) =
+// parity(r & Z) ^ parity(e & tz), where tz holds Z's low bits and parity(b_i & Z) at bit low + i.
+constexpr int kTileBits = 12;   // amplitudes per coset tile: 4096 (64 KiB complex128)
+constexpr int kGroupDims = 8;   // span dimensions above the low bits, at most
+constexpr int kGroupTerms = 16; // strings per group (accumulators per thread)
+
+struct PauliGroup {
+  uint64_t basis[kGroupDims];  // b_i, ordered by pivot bit
+  uint64_t z[kGroupTerms];     // each string's Z mask (the coset sign)
+  uint32_t tx[kGroupTerms];    // tile-index X mask (c_t << low | l_t)
+  uint32_t tz[kGroupTerms];    // tile-index Z mask
+  uint32_t kpar[kGroupTerms];  // bit k: parity of tz & (k * kBlock), the sign of entry k of a thread
+  int32_t slot[kGroupTerms];   // the string's position in the call's output (-1: padding)
+  uint32_t wodd, wneg;         // bit t: w_t odd (Re[i^w z] is +-Im z), Re[i^w z] negated (w = 1, 2)
+  uint8_t zeros[kTileBits];    // ascending bit positions a coset representative leaves zero
+  int nterms, low, dims;
+};
+
+// NT = the group's string count rounded up to a power of two (padding strings read a partner and
+// are never written out), so every loop below unrolls and every accumulator is a register.
+template <typename R, int NT>
+__global__ void __launch_bounds__(kBlock)
+    k_pauli_group(const typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t nb,
+                  int call_terms, PauliGroup g, double* __restrict__ partial, uint32_t batch_base) {
+  using V = typename Cx<R>::V;
+  constexpr int PER = (1 << kTileBits) / kBlock;
+  constexpr uint32_t kTileMask = (1u << kTileBits) - 1u;
+  __shared__ V tile[1 << kTileBits];
+  const uint32_t b = batch_base + blockIdx.y;
+  const V* __restrict__ st = states + ((size_t)b << nbits);
+  const uint64_t cosets = 1ull << (nbits - kTileBits);
+  const uint32_t lmask = (1u << g.low) - 1u;
+  // offset B(c) ^ l of each of this thread's tile entries e = tid + k * kBlock (the same in
+  // every coset)
+  uint64_t off[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t e = threadIdx.x + k * kBlock;
+    const uint32_t c = e >> g.low;
+    uint64_t o = e & lmask;
+    for (int i = 0; i < g.dims; ++i)
+      if ((c >> i) & 1u) o ^= g.basis[i];
+    off[k] = o;
+  }
+  // the sign (-1)^parity(e & tz) splits into this thread's part, parity(tid & tz), applied at the
+  // end, and the entry's part kpar (e = tid + k * kBlock: the two use disjoint bits)
+  uint32_t tpar = 0;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) tpar |= (uint32_t)(__popc(threadIdx.x & g.tz[t]) & 1) << t;
+  double acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = 0;
+  for (uint64_t q = blockIdx.x; q < cosets; q += nb) {
+    uint64_t r = q;
+    for (int i = 0; i < kTileBits; ++i) {
+      const uint32_t p = g.zeros[i];
+      r = ((r >> p) << (p + 1)) | (r & ((1ull << p) - 1));
+    }
+    {
+      V a[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) a[k] = st[r ^ off[k]];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) tile[threadIdx.x + k * kBlock] = a[k];
+    }
+    __syncthreads();
+    uint32_t cpar = 0;  // coset sign of each string, parity(r & Z) (wavefront-uniform)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) cpar |= (uint32_t)(__popcll(r & g.z[t]) & 1) << t;
+#pragma unroll 2
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t e = threadIdx.x + k * kBlock;
+      const V own = tile[e];
+      const double ax = (double)own.x, ay = (double)own.y;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const V p = tile[(e ^ g.tx[t]) & kTileMask];
+        // Re or Im of conj(p) a, by the parity of w
+        const bool odd = (g.wodd >> t) & 1u;
+        const double u1 = odd ? ay : ax, u2 = odd ? -ax : ay;
+        const double v = (double)p.x * u1 + (double)p.y * u2;
+        const double sg = (((g.kpar[t] >> k) ^ (cpar >> t)) & 1u) ? -1.0 : 1.0;
+        acc[t] = fma(sg, v, acc[t]);
+      }
+    }
+    __syncthreads();  // the tile is refilled by the next coset
+  }
+  // workgroup sums in a fixed order; string t's partial goes to its own row
+  __shared__ double part[kBlock / 64][NT];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const double v = ((tpar ^ g.wneg) >> t) & 1u ? -acc[t] : acc[t];
+    const double sum = wave_sum(v);
+    if (lane == 0) part[wave][t] = sum;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < g.nterms) {
+    double sum = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) sum += part[w][threadIdx.x];
+    partial[((size_t)b * call_terms + g.slot[threadIdx.x]) * nb + blockIdx.x] = sum;
+  }
+}
+
 // Tr(P rho) = sum_m i^nY (-1)^popcount(m&z) rho[m][m^x] (real part) for an n-qubit rho.
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
@@ -233,8 +349,128 @@ static int load_terms(int nterms, const unsigned long long* xmask, const unsigne
   return LSIM_OK;
 }
 
+template <typename R>
+static void launch_group(int m, dim3 grid, hipStream_t s, const typename Cx<R>::V* states,
+                         uint32_t nbits, uint32_t nb, int call_terms, const PauliGroup& g,
+                         double* partial, uint32_t b0) {
+  if (m <= 1)
+    k_pauli_group<R, 1><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
+  else if (m <= 2)
+    k_pauli_group<R, 2><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
+  else if (m <= 4)
+    k_pauli_group<R, 4><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
+  else if (m <= 8)
+    k_pauli_group<R, 8><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
+  else
+    k_pauli_group<R, 16><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
+}
+
+// Workgroups per state, i.e. partials per (state, string), of the expectation path for nbits.
+static uint32_t pauli_sv_blocks(int nbits) {
+  if (nbits < kTileBits) return reduce_blocks(1ull << (nbits - 1));
+  const uint64_t cosets = 1ull << (nbits - kTileBits);
+  return (uint32_t)(cosets < 4096 ? cosets : 4096);
+}
+
 static size_t pauli_sv_workspace_bytes(int nbits, long long batch, int nterms) {
-  return (size_t)batch * nterms * reduce_blocks(1ull << (nbits - 1)) * sizeof(double);
+  return (size_t)batch * nterms * pauli_sv_blocks(nbits) * sizeof(double);
+}
+
+// Rank over GF(2) of the masks v >> low.
+static int rank_above(const uint64_t* xs, int count, int low) {
+  uint64_t basis[64] = {};
+  int rank = 0;
+  for (int i = 0; i < count; ++i) {
+    uint64_t v = xs[i] >> low;
+    for (int bit = 63; bit >= 0 && v; --bit)
+      if ((v >> bit) & 1) {
+        if (basis[bit]) {
+          v ^= basis[bit];
+        } else {
+          basis[bit] = v;
+          ++rank;
+          break;
+        }
+      }
+  }
+  return rank;
+}
+
+// Splits the call's strings into groups, in order: a string joins the current group while the
+// group has fewer than kGroupTerms strings and its X masks keep a span of at most kGroupDims
+// dimensions above bit 4 (runs of at least 16 contiguous amplitudes). Returns the number of
+// groups, or -1 if a group breaks an invariant the kernel's indexing relies on (low + dims ==
+// kTileBits, every tile mask below 2^kTileBits, every address bit below nbits).
+static int make_groups(const PauliTerms& terms, int nterms, int nbits, PauliGroup* groups) {
+  int count = 0, first = 0;
+  while (first < nterms) {
+    int last = first + 1;  // [first, last) is the group
+    while (last < nterms && last - first < kGroupTerms &&
+           rank_above(terms.x + first, last + 1 - first, 4) <= kGroupDims)
+      ++last;
+    const int m = last - first;
+    const uint64_t* xs = terms.x + first;
+    // the widest low part that keeps the tile at kTileBits: low + rank_above(low) == kTileBits
+    int low = 4;
+    for (int l = 4; l <= kTileBits; ++l)
+      if (l + rank_above(xs, m, l) <= kTileBits) low = l;
+    PauliGroup& g = groups[count++];
+    memset(&g, 0, sizeof g);
+    g.nterms = m;
+    g.low = low;
+    // reduced basis of the masks with their low bits cleared: distinct pivots, each pivot bit in
+    // its own vector only
+    uint64_t basis[64] = {};
+    for (int i = 0; i < m; ++i) {
+      uint64_t v = xs[i] & ~((1ull << low) - 1);
+      for (int bit = 63; bit >= 0 && v; --bit)
+        if (((v >> bit) & 1) && basis[bit]) v ^= basis[bit];
+      if (!v) continue;
+      const int pivot = 63 - __builtin_clzll(v);
+      for (int bit = 0; bit < 64; ++bit)
+        if (basis[bit] && ((basis[bit] >> pivot) & 1)) basis[bit] ^= v;
+      basis[pivot] = v;
+    }
+    int pivots[kGroupDims];
+    for (int bit = 0; bit < 64; ++bit)
+      if (basis[bit]) {
+        pivots[g.dims] = bit;
+        g.basis[g.dims++] = basis[bit];
+      }
+    // zero positions of a coset representative: the low bits, then the pivots (ascending)
+    int nz = 0;
+    for (int bit = 0; bit < low; ++bit) g.zeros[nz++] = (uint8_t)bit;
+    for (int i = 0; i < g.dims; ++i) g.zeros[nz++] = (uint8_t)pivots[i];
+    for (int i = 0; i < m; ++i) {
+      const uint64_t x = xs[i], z = terms.z[first + i];
+      uint32_t c = 0;
+      uint64_t rest = x;
+      for (int d = 0; d < g.dims; ++d)
+        if ((x >> pivots[d]) & 1) {
+          c |= 1u << d;
+          rest ^= g.basis[d];
+        }
+      uint32_t tz = (uint32_t)(z & ((1ull << low) - 1));
+      for (int d = 0; d < g.dims; ++d) tz |= (uint32_t)(__builtin_popcountll(g.basis[d] & z) & 1) << (low + d);
+      if (rest >> low) return -1;  // the mask lies outside the group's span
+      const int w = __builtin_popcountll(x & z) & 3;
+      g.tx[i] = (c << low) | (uint32_t)rest;
+      g.tz[i] = tz;
+      g.z[i] = z;
+      for (int k = 0; k < (1 << kTileBits) / kBlock; ++k)
+        g.kpar[i] |= (uint32_t)(__builtin_popcount(tz & (uint32_t)(k * kBlock)) & 1) << k;
+      g.wodd |= (uint32_t)(w & 1) << i;
+      g.wneg |= (uint32_t)(((w + 1) >> 1) & 1) << i;
+      g.slot[i] = first + i;
+    }
+    if (g.low + g.dims != kTileBits) return -1;
+    for (int d = 0; d < g.dims; ++d)
+      if (nbits < 64 && (g.basis[d] >> nbits)) return -1;
+    for (int i = 0; i < kTileBits; ++i)
+      if (g.zeros[i] >= nbits) return -1;
+    first = last;
+  }
+  return count;
 }
 
 int launch_pauli_sv(const void* states, int dtype, int nbits, long long batch, int nterms,
@@ -367,8 +603,28 @@ namespace lsim {
 // (every slot of batch x nterms x nb is written before it is read), then their fixed-order sums.
 int launch_pauli_sv(const void* states, int dtype, int nbits, long long batch, int nterms,
                     const PauliTerms& terms, double* out, double* workspace, void* stream) {
-  const uint32_t nb = reduce_blocks(1ull << (nbits - 1));
+  const uint32_t nb = pauli_sv_blocks(nbits);
   hipStream_t s = as_stream(stream);
+  if (nbits >= kTileBits) {
+    PauliGroup groups[LSIM_MAX_TERMS];
+    const int ng = make_groups(terms, nterms, nbits, groups);
+    if (ng < 0) return fail(LSIM_EINVAL, "Pauli strings: grouping failed its own checks");
+    for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
+      const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
+      for (int i = 0; i < ng; ++i) {
+        const dim3 grid(nb, gy);
+        const int m = groups[i].nterms;
+        if (dtype == LSIM_C128)
+          launch_group<double>(m, grid, s, (const double2*)states, nbits, nb, nterms, groups[i],
+                               workspace, (uint32_t)b0);
+        else
+          launch_group<float>(m, grid, s, (const float2*)states, nbits, nb, nterms, groups[i],
+                              workspace, (uint32_t)b0);
+      }
+    }
+    sum_rows(workspace, nb, (uint64_t)batch * nterms, out, s);
+    return check_launch("Pauli expectation");
+  }
   const long long per_launch = kMaxGridY / nterms;
   for (long long b0 = 0; b0 < batch; b0 += per_launch) {
     const long long nbatch = batch - b0 < per_launch ? batch - b0 : per_launch;

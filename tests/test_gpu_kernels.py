@@ -370,3 +370,54 @@ def test_copy_states_within_a_batch(dtype: int) -> None:
     nat.call("lsim_sv_copy_states", ptr(s), dtype, n, batch, 0, 0, 0, stream_handle())
     with pytest.raises(nat.NativeError, match="count"):
         nat.call("lsim_sv_copy_states", ptr(s), dtype, n, batch, batch + 1, ptr(d_src), ptr(d_dst), stream_handle())
+
+
+def _structured_labels(n: int, seed: int) -> list[str]:
+    """Strings that exercise the grouping of the expectation kernel: X masks only in the low
+    bits, only in the high bits, spans of exactly 8 dimensions above bit 4 and beyond (a group
+    split), all-Z and identity strings, Y on both ends, and random strings."""
+    from lucivant_sim import workloads as wl
+
+    rng = np.random.default_rng(seed)
+
+    def label(xmask: int, zmask: int) -> str:
+        out = []
+        for q in range(n):
+            x, z = (xmask >> q) & 1, (zmask >> q) & 1
+            out.append("Y" if x and z else "X" if x else "Z" if z else "I")
+        return "".join(reversed(out))
+
+    labels = ["I" * n, "Z" * n, "Y" + "I" * (n - 2) + "Y", label(0b1011, 0b0110), label(1 << (n - 1), 1)]
+    for i in range(10):  # ten independent high masks: more than 8 dimensions above bit 4
+        labels.append(label((1 << (4 + i % (n - 4))) | (1 << (n - 1 - i % 3)), int(rng.integers(1 << n))))
+    labels += wl.pauli_strings(n, 40, seed)
+    return labels
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize("n", [12, 13, 16])
+def test_grouped_pauli_strings_equal_the_oracle(dtype: int, n: int) -> None:
+    """From 12 qubits on, the strings of a call are read in groups (one pass over the state per
+    group of up to 16 strings whose X masks span at most 8 dimensions above bit 4): every value
+    of a 3-state batch against the oracle."""
+    print(f"grouped Pauli expectation: n={n} dtype={dtype}", flush=True)
+    batch = 3
+    psi = states(n, batch, 40 + n)
+    labels = _structured_labels(n, n)
+    assert len(labels) <= nat.MAX_TERMS
+    s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+    masks = [pauli_masks(lbl) for lbl in labels]
+    out = torch.empty(batch * len(labels), dtype=torch.float64, device="cuda")
+    ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(n, batch, len(labels)))
+    nat.call(
+        "lsim_sv_pauli_expect",
+        ptr(s), dtype, n, batch, len(labels),
+        nat.ull_array([m[0] for m in masks]), nat.ull_array([m[1] for m in masks]),
+        ptr(out), ptr(ws), stream_handle(),
+    )  # fmt: skip
+    got = out.cpu().numpy().reshape(batch, -1)
+    for b in range(batch):
+        v = psi[b].astype(NPDT[dtype]).astype(np.complex128)
+        want = np.array([orc.expectation_sv(v, lbl, n) for lbl in labels])
+        assert np.abs(got[b] - want).max() < TOL[dtype]
+    assert np.all(np.abs(got[:, 0] - 1) < TOL[dtype])  # the identity
