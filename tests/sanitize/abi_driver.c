@@ -1,0 +1,51 @@
+/* Host-side checks of liblsim's C ABI under AddressSanitizer and UndefinedBehaviorSanitizer:
+ * every entry point refuses bad arguments with LSIM_EINVAL before any device work, and the host
+ * cumulative sum adds left to right. Needs no GPU. Built and run by tests/test_host_sanitizers.py
+ * against a copy of the library whose host code is sanitized. */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "lsim.h"
+static int fails = 0;
+#define EXPECT(call, code) do { int rc_ = (call); if (rc_ != (code)) { printf("FAIL %s -> %d: %s\n", #call, rc_, lsim_last_error()); fails++; } } while (0)
+int main(void) {
+  void* fake = (void*)0x1000;
+  int t2[2] = {0, 0}, t1[1] = {0}, t6[6] = {0};
+  unsigned long long x1[1] = {1ull << 5}, z1[1] = {0}, x64[64], z64[64];
+  for (int i = 0; i < 64; ++i) { x64[i] = 1; z64[i] = 0; }
+  EXPECT(lsim_sv_apply_dense(NULL, 0, 4, 1, 1, t1, fake, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_apply_dense(fake, 0, 4, 1, 6, t6, fake, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_apply_dense(fake, 0, 4, 1, 2, t2, fake, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_apply_diag(fake, 7, 4, 1, 1, t1, fake, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_apply_dense(fake, 0, 40, 1, 1, t1, fake, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_reduced_dm(fake, 0, 4, 1, 3, t6, fake, fake, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_pauli_expect(fake, 0, 4, 1, 1, x1, z1, fake, fake, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_pauli_expect(fake, 0, 4, 1, 65, x64, z64, fake, fake, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_pauli_expect_noalloc(fake, 0, 20, 1024, 64, x64, z64, fake, NULL), LSIM_EINVAL);
+  EXPECT(lsim_dm_pauli_expect(fake, 0, 2, 1, 1, x1, z1, fake, fake, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_set_basis(fake, 0, 4, 1, 16, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_set_basis(fake, 0, 4, 0, 0, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_scale(NULL, 0, 4, 1, 1.0, 0.0, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_axpy(fake, NULL, 0, 4, 1, 1.0, 0.0, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_probabilities(fake, 0, 4, 1, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_dm_probabilities(fake, 0, 2, 1, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_sv_copy_states(fake, 0, 4, 2, -1, NULL, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_program_run(0, 0, 1, fake, fake, fake, fake, NULL, NULL, NULL, NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_program_run(0, 5, 1, fake, fake, fake, fake, NULL, NULL, NULL, NULL, NULL, NULL, 65, NULL, NULL, NULL, NULL, NULL), LSIM_EINVAL);
+  EXPECT(lsim_program_run(0, 5, 1, fake, fake, fake, fake, NULL, NULL, NULL, NULL, NULL, NULL, 0, NULL, NULL, NULL, fake, NULL), LSIM_EINVAL);
+  if (lsim_reduce_workspace_bytes(26, 1, 1) == 0) { printf("FAIL workspace bytes\n"); fails++; }
+  if (lsim_reduce_workspace_bytes(0, 1, 1) != 0) { printf("FAIL workspace bytes of 0 bits\n"); fails++; }
+  /* the host cumulative sum, on sizes around its blocking */
+  for (size_t n = 0; n < 5000; n += 37) {
+    double* in = (double*)malloc((n + 1) * sizeof(double));
+    double* out = (double*)malloc((n + 1) * sizeof(double));
+    double acc = 0;
+    for (size_t i = 0; i < n; ++i) in[i] = (double)((i * 2654435761u) % 1000) / 997.0;
+    EXPECT(lsim_host_cumsum(in, out, n), LSIM_OK);
+    for (size_t i = 0; i < n; ++i) { acc += in[i]; if (out[i] != acc) { printf("FAIL cumsum %zu\n", i); fails++; break; } }
+    free(in); free(out);
+  }
+  EXPECT(lsim_host_cumsum(NULL, NULL, 4), LSIM_EINVAL);
+  printf("%s: %d failures\n", fails ? "FAILED" : "ok", fails);
+  return fails != 0;
+}

@@ -127,6 +127,24 @@ def test_analytic_values_of_the_reference_tests() -> None:
     assert orc.probabilities_dm(orc.evolve_dm(one, relax))[1] == pytest.approx(math.exp(-0.3))
 
 
+def test_pauli_labels_put_qubit_0_rightmost() -> None:
+    """Closed forms that fix the label order (simulator.py:511, 524: qubit 0 is the rightmost
+    letter): after x q[0] the state is |01>, so <IZ> = -1 and <ZI> = +1; after rx(pi/2) q[0],
+    <IY> = -sin(pi/2) = -1 and <YI> = 0; after h q[1], <XI> = 1 and <IX> = 0."""
+    one = orc.evolve_sv(ls.parse("OPENQASM 3; qubit[2] q; x q[0];"))
+    assert orc.expectation_sv(one, "IZ", 2) == pytest.approx(-1.0)
+    assert orc.expectation_sv(one, "ZI", 2) == pytest.approx(1.0)
+    rot = orc.evolve_sv(ls.parse('OPENQASM 3; include "stdgates.inc"; qubit[2] q; rx(pi/2) q[0];'))
+    assert orc.expectation_sv(rot, "IY", 2) == pytest.approx(-1.0)
+    assert orc.expectation_sv(rot, "YI", 2) == pytest.approx(0.0, abs=1e-15)
+    had = orc.evolve_sv(ls.parse('OPENQASM 3; include "stdgates.inc"; qubit[2] q; h q[1];'))
+    assert orc.expectation_sv(had, "XI", 2) == pytest.approx(1.0)
+    assert orc.expectation_sv(had, "IX", 2) == pytest.approx(0.0, abs=1e-15)
+    rho = orc.evolve_dm(ls.parse("OPENQASM 3; qubit[2] q; x q[0];"), None)
+    assert orc.expectation_dm(rho, "IZ", 2) == pytest.approx(-1.0)
+    assert orc.expectation_dm(rho, "ZI", 2) == pytest.approx(1.0)
+
+
 def test_expectations_match_dense_pauli_matrices() -> None:
     n = 3
     psi = _state(n, 5)
