@@ -136,6 +136,7 @@ def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
     inproc = counts_many(circuits, 1000, seeds)
     dt_in = time.perf_counter() - t0
     same = same and all(inproc[i] == res[i].counts for i in range(count))
+    kernel = program_roofline(circuits)
     return {
         "workload": f"{count} distinct 12-qubit circuits, 20 layers u3 + cx, 1000 shots each, complex128, OpenQASM text to Results",
         "plain_circuits_per_s": round(plain, 1),
@@ -148,6 +149,50 @@ def bench_batched(count: int = 1024, plain_count: int = 48) -> dict:
         "in_process_circuits_per_s": round(count / dt_in, 1),
         "in_process_note": "from already-read circuits, counts only (no reading, digests or Results): one thread of host work",
         "results_equal_plain": same,
+        "roofline": kernel,
+    }
+
+
+#: MI355X dense FP64 peak (vector and matrix cores alike)
+FP64_PEAK_TFLOPS = 78.6
+
+
+def program_roofline(circuits, reps: int = 5) -> dict:  # type: ignore[no-untyped-def]
+    """Duration of the one ``k_program`` launch that simulates every circuit of the batch (each
+    workgroup holds one 12-qubit state in LDS and applies its op table), from HIP events recorded
+    on the launch stream right around the launch; the flop count is that of the op table the
+    kernel runs: 8 x 2^k flop per amplitude for a dense k-qubit operator, 6 for a diagonal one
+    (one complex multiply), 3 per amplitude for the probabilities at the end."""
+    import numpy as np
+
+    from lucivant_sim import program as pg
+    from lucivant_sim.optable import static_tables
+
+    n = circuits[0].num_qubits
+    table, pool, starts, rows = static_tables(circuits)
+    kind, k = table["kind"].astype(np.int64), table["k"].astype(np.int64)
+    per_amp = np.where(kind == 0, 8 * (1 << k), np.where(kind == 1, 6, 0))
+    flops = float(per_amp.sum() + 3 * len(circuits)) * (1 << n)
+    times = []
+    for _ in range(reps):
+        timer: list = []
+        pg.run_table(n, table, pool, starts, rows, want_probs=True, timer=timer)
+        times.append(timer[0].elapsed_time(timer[1]) * 1e-3)
+    t = float(np.median(times))
+    achieved = flops / t / 1e12
+    dense_rows = int(np.sum(kind == 0))
+    return {
+        "bound": "fp64",
+        "kernel": "lsim::k_program<double>: one workgroup per circuit, state in LDS",
+        "achieved": round(achieved, 2),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+        "launch_ms": round(t * 1e3, 3),
+        "ops": int(len(table)),
+        "dense_ops": dense_rows,
+        "mean_dense_width": round(float(k[kind == 0].mean()), 2) if dense_rows else 0,
+        "flop_per_launch": flops,
     }
 
 
@@ -393,14 +438,236 @@ def bench_density_matrix() -> dict:
     res = ls.run(circuit, 4000, seed=7, noise=noise, method="density_matrix")
     t_run = time.perf_counter() - t0
     passes = [p for p in plan_density(circuit.operations, 13, noise) if isinstance(p, Pass)]
+    kernel = passes_roofline(passes, 26)
     return {
         "workload": "13-qubit density matrix, depth 30 u3 + cx/cz, depolarising 1e-3/1e-2, thermal relaxation on u3, readout 0.01/0.03",
         "superoperator_passes": len(passes),
+        "roofline": kernel,
         "probabilities_seconds": round(t_probs, 3),
         "run_4000_shots_seconds": round(t_run, 3),
         "distinct_outcomes": len(res.counts),
         "probability_sum": round(sum(probs.values()), 12),
     }
+
+
+def passes_roofline(passes, nbits: int) -> dict:  # type: ignore[no-untyped-def]
+    """Per-launch durations (HIP events on the launch stream) of ``passes`` run once on a
+    scratch vector of 2^nbits complex128 entries from |0>: the dense and diagonal launches,
+    each a read and a write of the vector, 2 x 16 B x 2^nbits."""
+    import numpy as np
+    import torch
+
+    from lucivant_sim import _native as nat
+    from lucivant_sim.engines import Statevector, plan_launches
+
+    sv = Statevector(nbits)
+    stream = torch.cuda.current_stream()
+    keep, calls = plan_launches(sv.buffer, nat.C128, nbits, 1, passes, stream.cuda_stream)
+    events = []
+    for fn, a in calls:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        assert fn(*a) == 0
+        e1.record(stream)
+        events.append((e0, e1))
+    torch.cuda.synchronize()
+    durations = [a.elapsed_time(b) * 1e-3 for a, b in events]
+    del keep, sv
+    torch.cuda.empty_cache()
+    nbytes = 2 * 16 * (1 << nbits)
+    avg = float(np.mean(durations))
+    return {
+        "bound": "hbm",
+        "kernel": "lsim::k_dense / k_dense_mfma / k_diag over the 2^26-entry vector (superoperators on row and column bits)",
+        "achieved": round(nbytes / avg / 1e9, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(nbytes / avg / 1e9 / HBM_PEAK_GBS, 4),
+        "avg_launch_ms": round(avg * 1e3, 4),
+        "launches": len(durations),
+        "evolution_device_ms": round(sum(durations) * 1e3, 2),
+        "bytes_per_launch": nbytes,
+    }
+
+
+def bench_pauli(buf, labels, reps: int = 20) -> dict:  # type: ignore[no-untyped-def]
+    """Pauli expectations on the resident 26-qubit state (2^26 complex128 amplitudes, 1 GiB):
+    the 26-qubit configuration's 8 strings in one ``lsim_sv_pauli_expect`` call (strings whose X
+    masks span few enough bits share one read of the state), and the read-only ceiling: a single
+    Z string, whose reduction reads every amplitude once and writes nothing. HIP events on the
+    launch stream bracket each call; GB/s counts 16 B x 2^26 per read of the state."""
+    import numpy as np
+    import torch
+
+    from lucivant_sim import _native as nat
+    from lucivant_sim.engines import launch_expectations
+
+    stream = torch.cuda.current_stream()
+    state_bytes = 16 * (1 << N)
+
+    def timed(lbls) -> float:  # type: ignore[no-untyped-def]
+        launch_expectations("lsim_sv_pauli_expect", buf, nat.C128, N, N, lbls)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch_expectations("lsim_sv_pauli_expect", buf, nat.C128, N, N, lbls)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+
+    t8 = timed(labels)
+    t1 = timed(["Z" + "I" * (N - 1)])
+    return {
+        "workload": "26-qubit state resident in HBM: 8 Pauli strings (seed 1003) in one call, and one Z string",
+        "strings_8_ms": round(t8 * 1e3, 4),
+        "string_1_ms": round(t1 * 1e3, 4),
+        "read_only_GBps": round(state_bytes / t1 / 1e9, 1),
+        "strings_8_over_1": round(t8 / t1, 2),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "lsim::k_pauli_group: one read of the state per group of strings",
+            "achieved": round(state_bytes / t1 / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(state_bytes / t1 / 1e9 / HBM_PEAK_GBS, 4),
+        },
+    }
+
+
+def bench_tile_plan(circuit, steps: int, dense_ms: float, budget: float = 48.0, low: int = 4) -> dict:  # type: ignore[no-untyped-def]
+    """The same 26-qubit simulation planned into tile passes instead: each pass loads tiles of
+    4096 amplitudes (12 state bits, the 4 lowest among them) into registers, applies a run of
+    up to 64 one- and two-qubit operators to them (regrouping registers through LDS when an
+    operator's bits are not held per thread) and stores them back, so one sweep of HBM covers
+    more gates than a 5-qubit dense pass. Per-launch durations of ``k_tile`` are measured with
+    HIP events on the launch stream; GB/s counts 2 x 16 B x 2^26 per pass."""
+    import numpy as np
+    import torch
+
+    from lucivant_sim import _native as nat
+    from lucivant_sim.engines import Statevector, launch_all, plan_launches
+    from lucivant_sim.plan import TilePass, plan_tiles
+
+    plan = plan_tiles(circuit.operations, N, budget=budget, low=low)
+    tiles = [p for p in plan if isinstance(p, TilePass)]
+    sv = Statevector(N)
+    stream = torch.cuda.current_stream()
+    hs = stream.cuda_stream
+    keep, calls = plan_launches(sv.buffer, nat.C128, N, 1, plan, hs)
+    lib = nat.load()
+    state = sv.buffer.data_ptr()
+
+    def step() -> None:
+        assert lib.lsim_sv_set_basis(state, nat.C128, N, 1, 0, hs) == 0
+        launch_all(calls)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    events = []
+    lib.lsim_sv_set_basis(state, nat.C128, N, 1, 0, hs)
+    for fn, a in calls:
+        if fn is lib.lsim_sv_apply_tile:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn(*a)
+            e1.record(stream)
+            events.append((e0, e1))
+        else:
+            fn(*a)
+    torch.cuda.synchronize()
+    avg = float(np.mean([a.elapsed_time(b) * 1e-3 for a, b in events])) if events else float("nan")
+    norm = float(torch.linalg.vector_norm(sv.buffer).item())
+    del keep, sv
+    torch.cuda.empty_cache()
+    ops = [len(t.ops) for t in tiles]
+    return {
+        "workload": "the same 26-qubit simulation, planned into tile passes (planner budget "
+        f"{budget:g}, {low} lowest bits always in the tile)",
+        "passes": len(plan),
+        "tile_passes": len(tiles),
+        "ops_per_tile_pass": round(float(np.mean(ops)), 1) if ops else 0,
+        "ms_per_step": round(ms, 3),
+        "speedup_vs_dense_plan": round(dense_ms / ms, 3),
+        "k_tile_avg_launch_ms": round(avg * 1e3, 4),
+        "k_tile_GBps": round(BYTES_PER_PASS / avg / 1e9, 1),
+        "k_tile_frac_of_hbm_peak": round(BYTES_PER_PASS / avg / 1e9 / HBM_PEAK_GBS, 4),
+        "norm_minus_1": float(f"{norm - 1:.3g}"),
+        "note": "fewer sweeps of HBM at a lower rate per sweep: the headline line stays on the dense plan, whose "
+        "kernel runs at the HBM roofline; the library's autotuner picks whichever plan is faster per size",
+    }
+
+
+PROFILE_LEGS = ("dense", "tile", "pauli", "density", "trajectories", "program")
+
+
+def profile_leg(leg: str, circuit, labels, steps: int) -> None:  # type: ignore[no-untyped-def]
+    """The kernels of one bench leg, ``steps`` times, for a profiler run (rocprofv3 names the
+    kernels; each leg that sweeps a 1 GiB vector adds one k_scale pass per step, a streaming
+    read and write of exactly 2^30 B, to calibrate the HBM counters):
+    tile = the 26-qubit simulation as tile passes (budget 48, 4 low bits);
+    pauli = the 8 Pauli strings and one Z string on the resident 26-qubit state;
+    density = the 13-qubit density-matrix probabilities (superoperator passes over 4^13 entries);
+    trajectories = 1024 shots of the 20-qubit teleport chain with noise in one batch;
+    program = 1024 distinct 12-qubit circuits in one LDS-program launch."""
+    import numpy as np
+    import torch
+
+    import lucivant_sim as ls
+    from lucivant_sim import _native as nat
+    from lucivant_sim import workloads
+    from lucivant_sim.engines import Statevector, launch_all, launch_expectations, plan_launches, run_plan
+    from lucivant_sim.plan import plan_statevector, plan_tiles
+
+    lib = nat.load()
+    hs = torch.cuda.current_stream().cuda_stream
+    if leg in ("tile", "pauli"):
+        sv = Statevector(N)
+        state = sv.buffer.data_ptr()
+        if leg == "tile":
+            plan = plan_tiles(circuit.operations, N, budget=48.0, low=4)
+            keep, calls = plan_launches(sv.buffer, nat.C128, N, 1, plan, hs)
+            for _ in range(steps):
+                assert lib.lsim_sv_set_basis(state, nat.C128, N, 1, 0, hs) == 0
+                launch_all(calls)
+                assert lib.lsim_sv_scale(state, nat.C128, N, 1, 1.0, 0.0, hs) == 0
+            del keep
+            what = f"{steps} steps of {len(plan)} tile-plan passes"
+        else:
+            run_plan(sv.buffer, nat.C128, N, 1, plan_statevector(circuit.operations, N, 5))
+            for _ in range(steps):
+                launch_expectations("lsim_sv_pauli_expect", sv.buffer, nat.C128, N, N, labels)
+                launch_expectations("lsim_sv_pauli_expect", sv.buffer, nat.C128, N, N, ["Z" + "I" * (N - 1)])
+                assert lib.lsim_sv_scale(state, nat.C128, N, 1, 1.0, 0.0, hs) == 0
+            what = f"{steps} x (8 strings, 1 string, 1 k_scale)"
+    elif leg == "density":
+        src, noise = workloads.config_c3(), workloads.noise_c3()
+        c = ls.parse(src)
+        for _ in range(steps):
+            ls.probabilities(c, noise)
+        dm = Statevector(26)
+        assert lib.lsim_sv_scale(dm.buffer.data_ptr(), nat.C128, 26, 1, 1.0, 0.0, hs) == 0
+        what = f"{steps} density-matrix probability runs + 1 k_scale"
+    elif leg == "trajectories":
+        from lucivant_sim.trajectories import run_batched
+
+        c = ls.parse(workloads.teleport_chain(20, 3))
+        for _ in range(steps):
+            run_batched(c, workloads.noise_c4(), 1024, np.random.default_rng(7))
+        what = f"{steps} x 1024 trajectory shots"
+    else:
+        from lucivant_sim.batch import counts_many
+
+        circuits = [ls.parse(workloads.random_layers(12, 20, 5000 + i)) for i in range(1024)]
+        for _ in range(steps):
+            counts_many(circuits, 1000, list(range(1024)))
+        what = f"{steps} x 1024 programs"
+    torch.cuda.synchronize()
+    print(f"profiled {leg}: {what}", flush=True)
 
 
 def read_traffic() -> float | None:
@@ -426,9 +693,13 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="skip the batched, serving and trajectory legs")
     ap.add_argument(
         "--profile",
-        action="store_true",
-        help="only run the passes (and one k_scale pass, a read+write of the 1 GiB state used to "
-        "calibrate HBM counters) for rocprofv3; prints no JSON",
+        nargs="?",
+        const="dense",
+        default=None,
+        choices=PROFILE_LEGS,
+        help="only run one leg's kernels for rocprofv3 (dense: the headline passes, each step "
+        "followed by one k_scale pass, a read+write of the 1 GiB state used to calibrate HBM "
+        "counters); prints no JSON",
     )
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -456,6 +727,9 @@ def main() -> int:
 
     src, _labels = workloads.config_c2()
     circuit = ls.parse(src)
+    if args.profile not in (None, "dense"):
+        profile_leg(args.profile, circuit, _labels, max(args.steps, 1))
+        return 0
     tuned = None
     if args.tiling is not None:
         nat.load().lsim_set_dense_tiling(args.tiling)
@@ -556,6 +830,7 @@ def main() -> int:
     norm = float(torch.linalg.vector_norm(sv.buffer).item())
     assert abs(norm - 1) < 1e-9, norm
     del mats
+    pauli = None if args.no_extras else bench_pauli(sv.buffer, _labels)
 
     out = {
         "metric": METRIC,
@@ -602,6 +877,8 @@ def main() -> int:
     del sv
     torch.cuda.empty_cache()
     if not args.no_extras:
+        out["pauli_26q"] = pauli
+        out["tile_plan"] = bench_tile_plan(circuit, min(args.steps, 10), ms_per_step)
         out["batched"] = bench_batched()
         out["batched_shots"] = bench_batched_shots()
         out["serving"] = [bench_serving(q) for q in (5.0, 20.0, 1000.0)]

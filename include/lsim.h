@@ -97,6 +97,33 @@ int lsim_set_dense_mfma(int widths);
  * argument only reads it. Results are identical either way. No reference counterpart. */
 int lsim_set_dense_streaming(int enabled);
 
+/* ------------------------------------------------------------------------------------------
+ * Tile passes: a run of small operators in one read and one write of each state.
+ *
+ * The amplitudes that agree on every bit except the LSIM_TILE_BITS tile bits `bits` (HOST ints,
+ * ascending; tile bit i is state bit bits[i]) form a tile. Each tile is staged in LDS, the `nops`
+ * operators of `ops` (HOST array) are applied to it in order, and it is written back. Operator j
+ * acts on k = 1 or 2 tile bits t[0], t[1] (t[0] the most significant matrix bit) with the
+ * complex128 matrix at element offset `mat` of the DEVICE array `mats` (also for LSIM_C64
+ * states): 2^k x 2^k row-major for LSIM_TILE_DENSE, the 2^k diagonal entries for LSIM_TILE_DIAG.
+ * A dense operator on two tile bits lists the higher one first (t[0] > t[1]).
+ * Replaces, for a run of gates whose qubits are all tile bits, one NumPy sweep per gate of
+ * _apply / _apply_slices (engines.py:38-119, driven by simulator.py:229-251).
+ */
+#define LSIM_TILE_BITS 12
+#define LSIM_MAX_TILE_OPS 64
+#define LSIM_TILE_DENSE 0
+#define LSIM_TILE_DIAG 1
+
+typedef struct lsim_tile_op {
+  int kind, k;
+  int t[2];
+  long long mat;
+} lsim_tile_op;
+
+int lsim_sv_apply_tile(void* states, int dtype, int nbits, long long batch, const int* bits,
+                       int nops, const lsim_tile_op* ops, const void* mats, void* stream);
+
 /* psi <- diag(d) psi on k <= 5 target bits: the diagonal case of _apply_slices
  * (engines.py:91-104; rz, s, t, cz, cp). `diags`: M diagonals of 2^k entries; `sel` as above. */
 int lsim_sv_apply_diag(void* states, int dtype, int nbits, long long batch, int k,

@@ -95,6 +95,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Whether a launch over `bytes` of states uses streaming (non-temporal) accesses, by the
+// process-wide switch lsim_set_dense_streaming.
+bool dense_streams(uint64_t bytes);
+
 inline size_t dtype_bytes(int dtype) { return dtype == LSIM_C64 ? 8 : 16; }
 
 }  // namespace lsim

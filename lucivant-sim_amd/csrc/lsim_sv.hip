@@ -41,6 +41,8 @@ static bool streams(uint64_t bytes) {
   return mode == 1 && (bytes <= kL2Bytes || bytes > kLastLevelCacheBytes);
 }
 
+bool dense_streams(uint64_t bytes) { return streams(bytes); }
+
 // loads and stores with the streaming (non-temporal) hint, or plain ones
 template <bool NT, typename T>
 __device__ __forceinline__ T ld(const T* p) {

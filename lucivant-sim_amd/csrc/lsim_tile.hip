@@ -1,0 +1,400 @@
+// Tile passes: many small operators applied in one read and one write of the state.
+//
+// A fused dense pass (lsim_sv.hip) applies one operator of up to 5 qubits per sweep of HBM. A
+// tile pass picks kTileBits state bits (the "tile bits", always including the lowest ones, so
+// tiles are read in runs of contiguous amplitudes). A tile is the 2^kTileBits amplitudes that
+// agree on all other bits; every tile is read once, a list of 1- and 2-qubit operators on tile
+// bits is applied to it, and it is written back. Any gates whose qubits are all tile bits can run
+// in one pass, so a circuit needs far fewer sweeps of HBM (the host planner is plan.plan_tiles).
+//
+// Register blocking. One workgroup of 256 threads owns one tile at a time; each thread holds 16
+// of its amplitudes in registers. In a layout R (4 of the 12 tile positions) thread t holds the
+// entries whose other 8 positions spell t (ascending) and whose R positions spell k = 0..15, in
+// register k. A dense operator whose positions lie in R mixes registers of one thread only: no
+// LDS, no barrier. A diagonal operator runs in any layout. Only when the next dense operator
+// needs positions outside R does the tile go through LDS into the next layout. The launcher
+// splits the operator list into such segments, each with the 4 positions its operators use
+// first. The layouts the tile is read and written in keep positions 0..3 (the lowest state bits)
+// on the thread bits, so 16 consecutive lanes move 16 consecutive amplitudes.
+//
+// Operators come from a host-built table in the kernel arguments: wavefront-uniform, so each
+// matrix is read with scalar loads. LDS holds one tile (64 KiB for complex128): two workgroups
+// share a CU, and one reads or writes HBM while the other computes.
+#include <string.h>
+
+#include "lsim_internal.h"
+
+namespace lsim {
+
+constexpr int kTileBits = 12;
+constexpr int kTileSize = 1 << kTileBits;
+constexpr int kRegs = kTileSize / kBlock;  // 16 amplitudes per thread: 4 register positions
+constexpr int kMaxTileOps = LSIM_MAX_TILE_OPS;
+constexpr int kMaxSegs = kMaxTileOps + 2;
+
+struct TileProgram {
+  uint8_t bits[kTileBits];        // state bit of tile position i, ascending
+  uint16_t layout[kMaxSegs];      // segment s: mask of its 4 register positions
+  uint8_t first[kMaxSegs + 1];    // segment s runs operators first[s] .. first[s+1]-1
+  uint8_t kind[kMaxTileOps];      // LSIM_TILE_DENSE or LSIM_TILE_DIAG
+  uint8_t k[kMaxTileOps];         // 1 or 2
+  uint8_t a[kMaxTileOps];         // dense: register slot of the most significant matrix bit;
+  uint8_t b[kMaxTileOps];         //   diagonal: tile positions; b unused for k = 1
+  uint32_t mat[kMaxTileOps];      // element offset of the operator in the complex128 pool
+  int nsegs;
+};
+
+// A 16-byte load or store, with or without the streaming (non-temporal) hint.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ double2 load_amp(const double2* p) {
+  d2v v = NT ? __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p)) : *reinterpret_cast<const d2v*>(p);
+  return make_double2(v.x, v.y);
+}
+template <bool NT>
+__device__ __forceinline__ float2 load_amp(const float2* p) {
+  f2v v = NT ? __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p)) : *reinterpret_cast<const f2v*>(p);
+  return make_float2(v.x, v.y);
+}
+template <bool NT>
+__device__ __forceinline__ void store_amp(double2* p, double2 a) {
+  d2v v = {a.x, a.y};
+  if (NT)
+    __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(p));
+  else
+    *reinterpret_cast<d2v*>(p) = v;
+}
+template <bool NT>
+__device__ __forceinline__ void store_amp(float2* p, float2 a) {
+  f2v v = {a.x, a.y};
+  if (NT)
+    __builtin_nontemporal_store(v, reinterpret_cast<f2v*>(p));
+  else
+    *reinterpret_cast<f2v*>(p) = v;
+}
+
+// acc + u * a in four fused multiply-adds
+template <typename R, typename V>
+__device__ __forceinline__ V mac(V acc, double2 u, V a) {
+  const R ux = (R)u.x, uy = (R)u.y;
+  acc.x = fma(ux, a.x, acc.x);
+  acc.x = fma(-uy, a.y, acc.x);
+  acc.y = fma(ux, a.y, acc.y);
+  acc.y = fma(uy, a.x, acc.y);
+  return acc;
+}
+
+// Tile index of register k of thread t in layout R: t's bits on the other positions (ascending),
+// k's on R's.
+__device__ __forceinline__ uint32_t entry(uint32_t t, uint32_t k, uint32_t R) {
+  uint32_t e = 0;
+#pragma unroll
+  for (int p = 0; p < kTileBits; ++p) {
+    if ((R >> p) & 1u) {
+      e |= (k & 1u) << p;
+      k >>= 1;
+    } else {
+      e |= (t & 1u) << p;
+      t >>= 1;
+    }
+  }
+  return e;
+}
+
+// State offset of tile index e: bit i of e at state bit bits[i].
+__device__ __forceinline__ uint32_t deposit(uint32_t e, const uint8_t* bits) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int i = 0; i < kTileBits; ++i) o |= ((e >> i) & 1u) << bits[i];
+  return o;
+}
+
+// A 2 x 2 operator on register slot J: registers k and k | 2^J for every k with bit J clear.
+template <int J, typename R, typename V>
+__device__ __forceinline__ void regs1(V (&a)[kRegs], const double2* __restrict__ M) {
+  const double2 m00 = M[0], m01 = M[1], m10 = M[2], m11 = M[3];
+#pragma unroll
+  for (int k = 0; k < kRegs; ++k) {
+    if (k & (1 << J)) continue;
+    const V x0 = a[k], x1 = a[k | (1 << J)];
+    V y0 = {0, 0}, y1 = {0, 0};
+    a[k] = mac<R>(mac<R>(y0, m00, x0), m01, x1);
+    a[k | (1 << J)] = mac<R>(mac<R>(y1, m10, x0), m11, x1);
+  }
+}
+
+// A 4 x 4 operator on register slots H (the most significant matrix bit) and L, H > L: quad by
+// quad, in place; the matrix is read from the scalar cache for each quad.
+template <int H, int L, typename R, typename V>
+__device__ __forceinline__ void regs2(V (&a)[kRegs], const double2* __restrict__ M) {
+#pragma unroll
+  for (int k = 0; k < kRegs; ++k) {
+    if (k & ((1 << H) | (1 << L))) continue;
+    const int i0 = k, i1 = k | (1 << L), i2 = k | (1 << H), i3 = k | (1 << H) | (1 << L);
+    const V x0 = a[i0], x1 = a[i1], x2 = a[i2], x3 = a[i3];
+    V y[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      V acc = {0, 0};
+      acc = mac<R>(acc, M[r * 4], x0);
+      acc = mac<R>(acc, M[r * 4 + 1], x1);
+      acc = mac<R>(acc, M[r * 4 + 2], x2);
+      acc = mac<R>(acc, M[r * 4 + 3], x3);
+      y[r] = acc;
+    }
+    a[i0] = y[0];
+    a[i1] = y[1];
+    a[i2] = y[2];
+    a[i3] = y[3];
+  }
+}
+
+template <typename R, typename V>
+__device__ __forceinline__ void dense_op(V (&a)[kRegs], int k, int h, int l, const double2* M) {
+  if (k == 1) {
+    switch (h) {
+      case 0: regs1<0, R>(a, M); break;
+      case 1: regs1<1, R>(a, M); break;
+      case 2: regs1<2, R>(a, M); break;
+      default: regs1<3, R>(a, M); break;
+    }
+    return;
+  }
+  // the launcher lists the higher slot first (h > l)
+  switch (h * 4 + l) {
+    case 4: regs2<1, 0, R>(a, M); break;
+    case 8: regs2<2, 0, R>(a, M); break;
+    case 9: regs2<2, 1, R>(a, M); break;
+    case 12: regs2<3, 0, R>(a, M); break;
+    case 13: regs2<3, 1, R>(a, M); break;
+    default: regs2<3, 2, R>(a, M); break;
+  }
+}
+
+// The register part of a tile index in a layout: register k's bits at the layout's 4 positions,
+// given as the one-bit masks P[0] < P[1] < P[2] < P[3] (wavefront-uniform).
+__device__ __forceinline__ uint32_t kpart(int k, const uint32_t (&P)[4]) {
+  return ((k & 1) ? P[0] : 0u) | ((k & 2) ? P[1] : 0u) | ((k & 4) ? P[2] : 0u) | ((k & 8) ? P[3] : 0u);
+}
+
+__device__ __forceinline__ void layout_bits(uint32_t layout, uint32_t (&P)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    P[i] = layout & (~layout + 1u);  // lowest set bit
+    layout ^= P[i];
+  }
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <typename R, bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_tile(typename Cx<R>::V* __restrict__ states, uint32_t nbits, TileProgram prog,
+           const double2* __restrict__ mats, uint32_t batch_base) {
+  using V = typename Cx<R>::V;
+  __shared__ V tile[kTileSize];
+  V* __restrict__ st = states + ((size_t)(batch_base + blockIdx.y) << nbits);
+  static_assert(kMaxBits <= 32, "tile offsets are 32-bit");
+  const uint64_t tiles = 1ull << (nbits - kTileBits);
+  const uint32_t t = threadIdx.x;
+  const uint32_t load_layout = prog.layout[0];
+  // state offsets (below 2^kMaxBits) of this thread's entries in the layout tiles are read in
+  uint32_t lo_off[kRegs];
+#pragma unroll
+  for (int k = 0; k < kRegs; ++k) lo_off[k] = deposit(entry(t, k, load_layout), prog.bits);
+  for (uint64_t q = blockIdx.x; q < tiles; q += gridDim.x) {
+    uint64_t base = q;  // the tile's other bits: zeros inserted at the tile bits
+#pragma unroll
+    for (int i = 0; i < kTileBits; ++i) {
+      const uint32_t p = prog.bits[i];
+      base = ((base >> p) << (p + 1)) | (base & ((1ull << p) - 1));
+    }
+    V a[kRegs];
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) a[k] = load_amp<NT>(st + (base | lo_off[k]));
+    uint32_t layout = load_layout;
+    // entry(t, k, layout) = tpart | kpart(k): the thread's part, and the register part (the same
+    // in every thread, computed on the scalar unit)
+    uint32_t tpart = entry(t, 0, layout);
+    uint32_t P[4];
+    layout_bits(layout, P);
+    const int nsegs = (int)uniform((uint32_t)prog.nsegs);
+    for (int s = 0; s < nsegs; ++s) {
+      const uint32_t next = uniform(prog.layout[s]);
+      if (s > 0 && next != layout) {
+        // into the next layout through LDS
+        __syncthreads();  // the previous exchange has been read
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) tile[tpart | kpart(k, P)] = a[k];
+        layout = next;
+        tpart = entry(t, 0, layout);
+        layout_bits(layout, P);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) a[k] = tile[tpart | kpart(k, P)];
+      }
+      const int j1 = (int)uniform(prog.first[s + 1]);
+      for (int j = (int)uniform(prog.first[s]); j < j1; ++j) {
+        const double2* M = mats + uniform(prog.mat[j]);
+        const uint32_t kind = uniform(prog.kind[j]), kk = uniform(prog.k[j]);
+        const uint32_t pa = uniform(prog.a[j]), pb = uniform(prog.b[j]);
+        if (kind == LSIM_TILE_DENSE) {
+          dense_op<R>(a, (int)kk, (int)pa, (int)pb, M);
+          continue;
+        }
+        // diagonal: the entry's own bits pick the factor
+        const bool two = kk == 2;
+        const double2 d0 = M[0], d1 = M[1];
+        const double2 d2 = two ? M[2] : d0, d3 = two ? M[3] : d1;
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) {
+          const uint32_t e = tpart | kpart(k, P);
+          // factor index (hb, lb): selects written out, so no per-lane indexed array
+          const bool hb = two && ((e >> pa) & 1u);
+          const bool lb = two ? ((e >> pb) & 1u) : ((e >> pa) & 1u);
+          double2 d;
+          d.x = hb ? (lb ? d3.x : d2.x) : (lb ? d1.x : d0.x);
+          d.y = hb ? (lb ? d3.y : d2.y) : (lb ? d1.y : d0.y);
+          V y = {0, 0};
+          a[k] = mac<R>(y, d, a[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) store_amp<NT>(st + (base | deposit(tpart | kpart(k, P), prog.bits)), a[k]);
+  }
+}
+
+// Splits the operators into segments of one register layout each. A layout is 4 tile positions:
+// those of the dense operators that come next, in order, as many as fit; the first and last
+// layouts avoid positions 0..kLowPositions-1 (reading and writing HBM in contiguous runs).
+constexpr int kLowPositions = 4;
+
+// `avoid`: positions a layout that reads or writes HBM leaves on the thread bits -- the tile's
+// run of lowest state bits (at least kLowPositions of them, at most 8), so consecutive lanes
+// move consecutive amplitudes.
+static uint32_t fill_layout(const lsim_tile_op* ops, int from, int nops, uint32_t need, uint32_t avoid) {
+  uint32_t R = need;
+  for (int j = from; j < nops && __builtin_popcount(R) < 4; ++j) {
+    if (ops[j].kind != LSIM_TILE_DENSE) continue;
+    uint32_t want = 1u << ops[j].t[0];
+    if (ops[j].k == 2) want |= 1u << ops[j].t[1];
+    if (want & avoid) break;
+    if (__builtin_popcount(R | want) > 4) break;
+    R |= want;
+  }
+  for (int p = kTileBits - 1; p >= 0 && __builtin_popcount(R) < 4; --p)
+    if (!((R >> p) & 1u) && !((avoid >> p) & 1u)) R |= 1u << p;
+  return R;
+}
+
+static int make_program(const int* bits, int nops, const lsim_tile_op* ops, TileProgram* prog) {
+  memset(prog, 0, sizeof *prog);
+  for (int i = 0; i < kTileBits; ++i) prog->bits[i] = (uint8_t)bits[i];
+  int run = 0;
+  while (run < 8 && bits[run] == run) ++run;
+  const uint32_t hbm = (1u << run) - 1u;  // positions kept on the thread bits for HBM access
+  int s = 0;
+  uint32_t R = fill_layout(ops, 0, nops, 0, hbm);
+  prog->layout[0] = (uint16_t)R;
+  prog->first[0] = 0;
+  for (int j = 0; j < nops; ++j) {
+    const lsim_tile_op& o = ops[j];
+    if (o.kind == LSIM_TILE_DENSE) {
+      uint32_t want = 1u << o.t[0];
+      if (o.k == 2) want |= 1u << o.t[1];
+      if ((R & want) != want) {
+        R = fill_layout(ops, j, nops, want, 0);
+        ++s;
+        prog->layout[s] = (uint16_t)R;
+        prog->first[s] = (uint8_t)j;
+      }
+      // register slots: the rank of each position among the layout's
+      auto slot = [R](int p) { return __builtin_popcount(R & ((1u << p) - 1u)); };
+      prog->a[j] = (uint8_t)slot(o.t[0]);
+      prog->b[j] = (uint8_t)(o.k == 2 ? slot(o.t[1]) : 0);
+      if (o.k == 2 && prog->a[j] < prog->b[j]) return -2;  // the caller lists the higher bit first
+    } else {
+      prog->a[j] = (uint8_t)o.t[0];
+      prog->b[j] = (uint8_t)(o.k == 2 ? o.t[1] : 0);
+    }
+    prog->kind[j] = (uint8_t)o.kind;
+    prog->k[j] = (uint8_t)o.k;
+    prog->mat[j] = (uint32_t)o.mat;
+  }
+  if (R & hbm) {  // written back in a layout off the lowest positions
+    ++s;
+    prog->layout[s] = (uint16_t)fill_layout(ops, nops, nops, 0, hbm);
+    prog->first[s] = (uint8_t)nops;
+  }
+  prog->nsegs = s + 1;
+  prog->first[s + 1] = (uint8_t)nops;
+  for (int i = 0; i < prog->nsegs; ++i)
+    if (__builtin_popcount(prog->layout[i]) != 4 || (prog->layout[i] >> kTileBits)) return -1;
+  if ((prog->layout[0] & hbm) || (prog->layout[prog->nsegs - 1] & hbm)) return -1;
+  return 0;
+}
+
+}  // namespace lsim
+
+using namespace lsim;
+
+extern "C" {
+
+int lsim_sv_apply_tile(void* states, int dtype, int nbits, long long batch, const int* bits,
+                       int nops, const lsim_tile_op* ops, const void* mats, void* stream) {
+  LSIM_REQUIRE(states != nullptr && mats != nullptr, "the state or matrix pointer is NULL");
+  LSIM_REQUIRE(dtype == LSIM_C128 || dtype == LSIM_C64, "dtype must be LSIM_C128 or LSIM_C64");
+  LSIM_REQUIRE(nbits >= kTileBits && nbits <= kMaxBits, "tile passes need %d to %d bits, not %d",
+               kTileBits, kMaxBits, nbits);
+  LSIM_REQUIRE(batch >= 1 && batch <= (1ll << 31), "batch must be between 1 and 2^31");
+  LSIM_REQUIRE(bits != nullptr, "the tile bits are NULL");
+  LSIM_REQUIRE(nops >= 0 && nops <= kMaxTileOps, "a tile pass takes 0 to %d operators, not %d",
+               kMaxTileOps, nops);
+  LSIM_REQUIRE(nops == 0 || ops != nullptr, "the operator table is NULL");
+  for (int i = 0; i < kTileBits; ++i) {
+    LSIM_REQUIRE(bits[i] >= 0 && bits[i] < nbits, "tile bit %d is outside 0..%d", bits[i], nbits - 1);
+    LSIM_REQUIRE(i == 0 || bits[i] > bits[i - 1], "tile bits must ascend");
+  }
+  for (int i = 0; i < kLowPositions; ++i)
+    LSIM_REQUIRE(bits[i] == i, "the tile bits must include state bits 0..%d", kLowPositions - 1);
+  for (int j = 0; j < nops; ++j) {
+    const lsim_tile_op& o = ops[j];
+    LSIM_REQUIRE(o.kind == LSIM_TILE_DENSE || o.kind == LSIM_TILE_DIAG, "operator %d: unknown kind %d", j, o.kind);
+    LSIM_REQUIRE(o.k == 1 || o.k == 2, "operator %d acts on 1 or 2 tile bits, not %d", j, o.k);
+    LSIM_REQUIRE(o.t[0] >= 0 && o.t[0] < kTileBits, "operator %d: tile bit %d is outside 0..%d", j, o.t[0], kTileBits - 1);
+    if (o.k == 2)
+      LSIM_REQUIRE(o.t[1] >= 0 && o.t[1] < kTileBits && o.t[1] != o.t[0], "operator %d: bad second tile bit %d", j, o.t[1]);
+    LSIM_REQUIRE(o.mat >= 0 && o.mat <= 0x7fffffffll, "operator %d: matrix offset %lld", j, o.mat);
+  }
+  for (int j = 0; j < nops; ++j)
+    if (ops[j].kind == LSIM_TILE_DENSE && ops[j].k == 2)
+      LSIM_REQUIRE(ops[j].t[0] > ops[j].t[1], "operator %d: a dense 2-bit operator lists its higher tile bit first", j);
+  TileProgram prog;
+  if (make_program(bits, nops, ops, &prog) != 0)
+    return fail(LSIM_EINVAL, "tile pass: the layout plan failed its own checks");
+  hipStream_t s = as_stream(stream);
+  const uint64_t tiles = 1ull << (nbits - kTileBits);
+  const uint32_t grid = (uint32_t)(tiles < 65536 ? tiles : 65536);
+  const bool nt = dense_streams((uint64_t)batch * (dtype_bytes(dtype) << nbits));
+  for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
+    const dim3 g(grid, gy);
+    if (dtype == LSIM_C128) {
+      if (nt)
+        k_tile<double, true><<<g, kBlock, 0, s>>>((double2*)states, nbits, prog, (const double2*)mats, (uint32_t)b0);
+      else
+        k_tile<double, false><<<g, kBlock, 0, s>>>((double2*)states, nbits, prog, (const double2*)mats, (uint32_t)b0);
+    } else {
+      if (nt)
+        k_tile<float, true><<<g, kBlock, 0, s>>>((float2*)states, nbits, prog, (const double2*)mats, (uint32_t)b0);
+      else
+        k_tile<float, false><<<g, kBlock, 0, s>>>((float2*)states, nbits, prog, (const double2*)mats, (uint32_t)b0);
+    }
+  }
+  return check_launch("tile pass");
+}
+
+}  // extern "C"

@@ -42,6 +42,7 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "lsim_set_dense_mfma": ([_c_int], _c_int),
     "lsim_set_dense_streaming": ([_c_int], _c_int),
     "lsim_sv_apply_diag": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_void, _c_void, _c_void], _c_int),
+    "lsim_sv_apply_tile": ([_c_void, _c_int, _c_int, _c_ll, _c_int_p, _c_int, _c_void, _c_void, _c_void], _c_int),
     "lsim_sv_scale": ([_c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),
     "lsim_sv_copy_states": ([_c_void, _c_int, _c_int, _c_ll, _c_ll, _c_void, _c_void, _c_void], _c_int),
     "lsim_sv_axpy": ([_c_void, _c_void, _c_int, _c_int, _c_ll, _c_double, _c_double, _c_void], _c_int),
@@ -68,6 +69,15 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
         _c_int,
     ),
 }
+
+
+class TileOp(ctypes.Structure):
+    """``lsim_tile_op`` of include/lsim.h."""
+
+    _fields_ = [("kind", _c_int), ("k", _c_int), ("t", _c_int * 2), ("mat", _c_ll)]
+
+
+TILE_DENSE, TILE_DIAG = 0, 1
 
 
 class NativeError(RuntimeError):

@@ -1,17 +1,20 @@
 """Autotuning of fusion width and kernel variant, measured on the device, cached on disk.
 
-Four choices set the cost of a statevector simulation on the MI355X:
+Five choices set the cost of a statevector simulation on the MI355X:
 
 - the fusion width (qubits per dense pass): wider passes mean fewer HBM sweeps but more FP64
   work per amplitude (8 * 2^k flop per 32 B; k = 5 sits at the FP64 ridge of ~9.8 flop/B);
 - the dense-kernel variant for placements with low target bits: the LDS-tiled kernel or the
   direct one;
 - which operator widths (3, 4, 5 qubits) run on the FP64 matrix cores, and whether the dense
-  kernels stream (non-temporal accesses) when every target is at bit 3 or above.
+  kernels stream (non-temporal accesses) when every target is at bit 3 or above;
+- dense passes or tile passes (runs of 1-2 qubit operators applied to 4096-amplitude tiles held
+  in registers, one HBM sweep per run), and the tile planner's cost budget.
 
 :func:`tune` plans a circuit at each candidate width, runs every candidate (with the tiled kernel
 on and off) on a scratch state of the same size, and times it with HIP events, then times the
-winner's plan for each matrix-core width mask with streaming on and off; the winner is stored
+winner's plan for each matrix-core width mask with streaming on and off, then tile plans for
+each (budget, low) candidate; the winner is stored
 per (gfx architecture, qubit count) in ``~/.cache/lucivant_sim/autotune.json`` (or
 ``$LSIM_AUTOTUNE``) and read by :func:`apply`, which ``simulator`` calls on first use of a size;
 every simulation then sets its own size's switches (:func:`set_switches`) before it launches. The LDS
@@ -31,8 +34,8 @@ import torch
 
 from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Circuit
-from lucivant_sim.engines import Statevector, run_passes
-from lucivant_sim.plan import plan_statevector
+from lucivant_sim.engines import Statevector, run_plan
+from lucivant_sim.plan import TILE_BITS, plan_statevector, plan_tiles
 
 CACHE = Path(os.environ.get("LSIM_AUTOTUNE", str(Path.home() / ".cache" / "lucivant_sim" / "autotune.json")))
 
@@ -57,11 +60,11 @@ def save_cache(data: dict[str, Any]) -> None:
 
 def _time_passes(sv: Statevector, passes: list, reps: int) -> float:
     stream = torch.cuda.current_stream()
-    run_passes(sv.buffer, sv.dtype, sv.nbits, 1, passes)  # warm-up
+    run_plan(sv.buffer, sv.dtype, sv.nbits, 1, passes)  # warm-up
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(reps):
-        run_passes(sv.buffer, sv.dtype, sv.nbits, 1, passes)
+        run_plan(sv.buffer, sv.dtype, sv.nbits, 1, passes)
     end.record(stream)
     torch.cuda.synchronize()
     return start.elapsed_time(end) * 1e-3 / reps
@@ -72,11 +75,21 @@ CANDIDATES = ((3, 0), (4, 0), (5, 0), (4, 3), (5, 3), (5, 4))
 LOW_BITS = 3
 #: matrix-core width masks (bit k: k-qubit operators on the matrix cores)
 MFMA_MASKS = (0, 1 << 5, (1 << 5) | (1 << 4), (1 << 5) | (1 << 4) | (1 << 3))
+#: tile-pass (budget, low) candidates, timed after the dense winner for states of 12+ qubits
+TILE_CANDIDATES = ((32.0, 4), (48.0, 4), (48.0, 5))
 
 
-def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES, reps: int = 2, store: bool = True) -> dict[str, Any]:
-    """Time ``circuit`` for every (width, low_width) candidate, tiled kernel on and off; returns
-    (and by default stores) the timings and the winner for this circuit's width."""
+def tune(
+    circuit: Circuit,
+    candidates: tuple[tuple[int, int], ...] = CANDIDATES,
+    reps: int = 2,
+    store: bool = True,
+    tile_candidates: tuple[tuple[float, int], ...] = TILE_CANDIDATES,
+) -> dict[str, Any]:
+    """Time ``circuit`` for every (width, low_width) candidate, tiled kernel on and off, then
+    the winner's switch variants, then tile passes for each (budget, low) candidate; returns
+    (and by default stores) the timings and the winner for this circuit's width (``tile`` is
+    the winning (budget, low) when tile passes beat every dense plan, else None)."""
     lib = nat.load()
     sv = Statevector(circuit.num_qubits)
     timings: dict[str, float] = {}
@@ -113,6 +126,32 @@ def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES,
     m_s, nt_s = choice.split("_")
     mfma, streaming = int(m_s[4:]), int(nt_s[2:])
     timings.update(variant_timings)
+    # tile passes (runs of 1-2 qubit operators per sweep), under the winner's switches
+    dense_seconds = min(timings[best], variant_timings[choice])
+    tile_timings: dict[str, float] = {}
+    tile_passes: dict[str, int] = {}
+    if circuit.num_qubits >= TILE_BITS and tile_candidates:
+        lib.lsim_set_dense_mfma(mfma)
+        lib.lsim_set_dense_streaming(streaming)
+        old_tiling = lib.lsim_set_dense_tiling(tiled)
+        try:
+            for budget, low in tile_candidates:
+                passes = plan_tiles(circuit.operations, circuit.num_qubits, budget=budget, low=low)
+                key = f"tile_b{budget:g}_l{low}"
+                tile_timings[key] = _time_passes(sv, passes, reps)
+                tile_passes[key] = len(passes)
+        finally:
+            lib.lsim_set_dense_mfma(old_mfma)
+            lib.lsim_set_dense_streaming(old_streaming)
+            lib.lsim_set_dense_tiling(old_tiling)
+        timings.update(tile_timings)
+    tile = None
+    seconds = dense_seconds
+    if tile_timings:
+        tbest = min(tile_timings, key=tile_timings.__getitem__)
+        if tile_timings[tbest] < dense_seconds:
+            b_s, l_s = tbest.split("_")[1:]
+            tile, seconds = [float(b_s[1:]), int(l_s[1:])], tile_timings[tbest]
     rec = {
         "width": width,
         "low_width": low_width,
@@ -120,9 +159,10 @@ def tune(circuit: Circuit, candidates: tuple[tuple[int, int], ...] = CANDIDATES,
         "tiled": tiled,
         "mfma": mfma,
         "streaming": streaming,
-        "seconds": min(timings[best], variant_timings[choice]),
+        "tile": tile,
+        "seconds": seconds,
         "timings": timings,
-        "passes": {f"w{w}_low{lw}": len(p) for (w, lw), p in plans.items()},
+        "passes": {**{f"w{w}_low{lw}": len(p) for (w, lw), p in plans.items()}, **tile_passes},
     }
     if store:
         data = load_cache()
@@ -147,18 +187,21 @@ def best_for(num_qubits: int) -> dict[str, Any] | None:
 DEFAULT_SWITCHES = (1, (1 << 3) | (1 << 4) | (1 << 5), 1)
 
 
-def apply(num_qubits: int) -> tuple[int, int, int, tuple[int, int, int]]:
-    """(width, low_bits, low_width, switches) to plan and launch ``num_qubits``-qubit circuits
-    with; ``switches`` = (tiled, matrix-core width mask, streaming mode), set per launch by
-    :func:`set_switches`. Nothing is changed here: the switches are process-wide, so every
-    simulation sets those of its own size before it launches."""
+def apply(num_qubits: int) -> tuple[int, int, int, tuple[int, int, int], tuple[float, int] | None]:
+    """(width, low_bits, low_width, switches, tile) to plan and launch ``num_qubits``-qubit
+    circuits with; ``switches`` = (tiled, matrix-core width mask, streaming mode), set per
+    launch by :func:`set_switches`; ``tile`` = (budget, low) when tile passes won for this size.
+    Nothing is changed here: the switches are process-wide, so every simulation sets those of
+    its own size before it launches."""
     from lucivant_sim import simulator
 
     rec = best_for(num_qubits)
     if rec is None:
-        return simulator.FUSION_WIDTH, 0, 0, DEFAULT_SWITCHES
+        return simulator.FUSION_WIDTH, 0, 0, DEFAULT_SWITCHES, None
     switches = (int(rec.get("tiled", 1)), int(rec.get("mfma", DEFAULT_SWITCHES[1])), int(rec.get("streaming", 1)))
-    return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0)), switches
+    tile = rec.get("tile")
+    tile = (float(tile[0]), int(tile[1])) if tile and num_qubits >= TILE_BITS else None
+    return int(rec["width"]), int(rec.get("low_bits", 0)), int(rec.get("low_width", 0)), switches, tile
 
 
 def set_switches(switches: tuple[int, int, int]) -> None:

@@ -15,6 +15,7 @@ launched back to back with no host synchronisation.
 
 from __future__ import annotations
 
+import ctypes
 import math
 from typing import Sequence
 
@@ -25,7 +26,7 @@ from numpy.typing import NDArray
 from lucivant_sim import _native as nat
 from lucivant_sim.gates import Matrix
 from lucivant_sim.noise import Channel
-from lucivant_sim.plan import RESET_CHANNEL, Pass, descending, is_diagonal, superop_of_gate
+from lucivant_sim.plan import RESET_CHANNEL, Pass, TilePass, descending, is_diagonal, superop_of_gate
 
 #: Memory guards of the reference (engines.py:29-31): 2^26 amplitudes and 4^13 entries, 1 GiB each.
 MAX_STATEVECTOR_QUBITS = 26
@@ -111,6 +112,81 @@ def run_passes(buf: torch.Tensor, dtype: int, nbits: int, batch: int, passes: Se
     dev.record_stream(torch.cuda.current_stream())
 
 
+def plan_launches(buf: torch.Tensor, dtype: int, nbits: int, batch: int, plan: Sequence[Pass | TilePass], stream: int) -> tuple[list, list[tuple]]:
+    """Every launch of ``plan`` (fused dense and diagonal passes, global phases, tile passes) on
+    ``batch`` states of ``nbits`` bits in ``buf``, as (function, arguments) pairs that only
+    enqueue work on ``stream``; and the objects the arguments point into, which must stay alive
+    until the launches have run. All matrices are uploaded in one copy per kind (dense passes in
+    the state's dtype, tile operators in complex128)."""
+    lib = nat.load()
+    state = ptr(buf)
+    keep: list = []
+    calls: list[tuple] = []
+    flat = [p for p in plan if not isinstance(p, TilePass)]
+    tiles = [p for p in plan if isinstance(p, TilePass)]
+    addrs: list[int] = []
+    if flat:
+        dev, addrs = upload([p.matrix for p in flat], dtype)
+        keep.append(dev)
+    tile_tables = []
+    if tiles:
+        host = np.concatenate([op.matrix.reshape(-1) for tp in tiles for op in tp.ops] or [np.zeros(1, dtype=np.complex128)])
+        tdev = torch.from_numpy(np.ascontiguousarray(host, dtype=np.complex128)).to(require_device())
+        keep.append(tdev)
+        at = 0
+        for tp in tiles:
+            table = (nat.TileOp * max(len(tp.ops), 1))()
+            for j, op in enumerate(tp.ops):
+                table[j].kind = nat.TILE_DIAG if op.kind == "diag" else nat.TILE_DENSE
+                table[j].k = op.k
+                table[j].t[0] = op.bits[0]
+                table[j].t[1] = op.bits[1] if op.k == 2 else 0
+                table[j].mat = at
+                at += op.matrix.size
+            bits = nat.int_array(tp.bits)
+            keep += [table, bits]
+            tile_tables.append((bits, len(tp.ops), ctypes.addressof(table), ptr(tdev)))
+    fi = ti = 0
+    for p in plan:
+        if isinstance(p, TilePass):
+            bits, nops, table, mats = tile_tables[ti]
+            ti += 1
+            calls.append((lib.lsim_sv_apply_tile, (state, dtype, nbits, batch, bits, nops, table, mats, stream)))
+            continue
+        addr = addrs[fi]
+        fi += 1
+        if p.kind == "scale":
+            z = complex(p.matrix[0])
+            calls.append((lib.lsim_sv_scale, (state, dtype, nbits, batch, z.real, z.imag, stream)))
+        else:
+            fn = lib.lsim_sv_apply_dense if p.kind == "dense" else lib.lsim_sv_apply_diag
+            targets = nat.int_array(p.bits)
+            keep.append(targets)
+            calls.append((fn, (state, dtype, nbits, batch, p.k, targets, addr, None, stream)))
+    return keep, calls
+
+
+def launch_all(calls: list[tuple]) -> None:
+    """Issue launches made by :func:`plan_launches`; raises NativeError on the first failure."""
+    lib = nat.load()
+    for fn, args in calls:
+        if fn(*args):
+            raise nat.NativeError(f"{fn.__name__}: {lib.lsim_last_error().decode(errors='replace')}")
+
+
+def run_plan(buf: torch.Tensor, dtype: int, nbits: int, batch: int, plan: Sequence[Pass | TilePass]) -> None:
+    """Launch a plan that mixes fused dense passes and tile passes (:func:`plan.plan_tiles`), in
+    order, on ``batch`` states of ``nbits`` bits in ``buf``."""
+    if not any(isinstance(p, TilePass) for p in plan):
+        run_passes(buf, dtype, nbits, batch, plan)  # type: ignore[arg-type]
+        return
+    keep, calls = plan_launches(buf, dtype, nbits, batch, plan, stream_handle())
+    launch_all(calls)
+    for t in keep:
+        if isinstance(t, torch.Tensor):
+            t.record_stream(torch.cuda.current_stream())
+
+
 def _single(matrix: Matrix, bits: tuple[int, ...]) -> Pass:
     m, bits = descending(np.asarray(matrix, dtype=np.complex128), tuple(bits))
     if is_diagonal(m):
@@ -137,8 +213,8 @@ class _DeviceState:
     def _stream(self) -> int:
         return stream_handle()
 
-    def run(self, passes: Sequence[Pass]) -> None:
-        run_passes(self.buffer, self.dtype, self.nbits, 1, passes)
+    def run(self, passes: Sequence[Pass | TilePass]) -> None:
+        run_plan(self.buffer, self.dtype, self.nbits, 1, passes)
 
     def _reduced(self, bits: tuple[int, ...]) -> NDArray[np.complex128]:
         k = len(bits)

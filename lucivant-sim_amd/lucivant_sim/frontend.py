@@ -129,13 +129,14 @@ def compile_one(
 
     ``observable`` set: an expectation request (labels checked here; the statevector method, or
     the density matrix when ``noise`` has gate errors, as ``expectation`` chooses).
-    ``widths`` set: statevectors wider than ``lds_qubits`` also get their fused passes, planned
-    with ``widths[num_qubits]`` = (fusion width, low bits, low width). ``digest`` False skips the
+    ``widths`` set: statevectors wider than ``lds_qubits`` also get their passes, planned with
+    ``widths[num_qubits]`` = (fusion width, low bits, low width[, tile]): tile passes when
+    ``tile`` = (budget, low) is given, fused dense passes otherwise. ``digest`` False skips the
     circuit digest (callers that build no Result)."""
     from lucivant_sim._native import MAX_TERMS
     from lucivant_sim.engines import SimulationError
     from lucivant_sim.optable import Unsupported, static_table
-    from lucivant_sim.plan import plan_statevector
+    from lucivant_sim.plan import TILE_BITS, plan_statevector, plan_tiles
     from lucivant_sim.simulator import choose_method, to_circuit
 
     try:
@@ -183,8 +184,12 @@ def compile_one(
             except Unsupported:
                 pass
         if widths is not None:
-            w, low_bits, low_width = widths.get(c.num_qubits, (4, 0, 0))
-            out.passes = plan_statevector(c.operations, c.num_qubits, w, low_bits, low_width)
+            w, low_bits, low_width, *rest = widths.get(c.num_qubits, (4, 0, 0))
+            tile = rest[0] if rest else None
+            if tile is not None and c.num_qubits >= TILE_BITS:
+                out.passes = plan_tiles(c.operations, c.num_qubits, budget=tile[0], low=tile[1])
+            else:
+                out.passes = plan_statevector(c.operations, c.num_qubits, w, low_bits, low_width)
         else:
             out.circuit = c
         return out

@@ -338,3 +338,167 @@ RESET_CHANNEL = Channel(
     1,
     (np.array([[1, 0], [0, 0]], dtype=np.complex128), np.array([[0, 1], [0, 0]], dtype=np.complex128)),
 )
+
+
+# ------------------------------------------------------------------------------ tile passes
+
+#: State bits per tile of a tile pass (the kernel's LSIM_TILE_BITS).
+TILE_BITS = 12
+#: Lowest state bits every tile includes: a tile is read in runs of 2^TILE_LOW contiguous
+#: amplitudes (256 bytes in complex128).
+TILE_LOW = 4
+#: Operators one tile pass may hold (the kernel's LSIM_MAX_TILE_OPS).
+TILE_MAX_OPS = 64
+#: Default operator budget of a tile pass, in the units of :func:`_tile_cost` (about one
+#: 1-qubit dense operator each). Each unit costs one read and write of the tile in LDS; the
+#: budget keeps a pass's LDS and FP64 work near the time its sweep of HBM takes.
+TILE_BUDGET = 24.0
+
+
+@dataclass
+class TilePass:
+    """One sweep of the state in tiles: ``bits`` (ascending) are the tile's state bits (tile bit
+    i = state bit bits[i]); ``ops`` are applied to every tile in order, each a :class:`Pass`
+    whose ``bits`` are tile bits (first = most significant matrix bit) and whose kind is "dense"
+    or "diag"."""
+
+    bits: tuple[int, ...]
+    ops: list[Pass]
+    gates: int = 0
+
+    kind = "tile"
+
+
+def _tile_cost(p: Pass) -> float:
+    if p.kind == "diag":
+        return 0.5 * p.k
+    return 1.0 if p.k == 1 else 2.5
+
+
+def _fuse_in_tile(items: list[tuple[Matrix, tuple[int, ...]]], bits: tuple[int, ...]) -> list[Pass]:
+    """The gates of one tile pass as operators on at most 2 tile bits, in an order that keeps
+    every dependency (the statevector greedy and merge at width 2, then tile-local bits)."""
+    local = {q: i for i, q in enumerate(bits)}
+    blocks = _merge(_greedy(_one_qubit_runs(items), 2), 2)
+    out = []
+    for blk in blocks:
+        m, qs = descending(np.asarray(blk.matrix, dtype=np.complex128), tuple(blk.qubits))
+        tb = tuple(local[q] for q in qs)  # the higher tile bit first
+        if is_diagonal(m):
+            out.append(Pass("diag", tb, np.ascontiguousarray(np.diag(m)), blk.gates))
+        else:
+            out.append(Pass("dense", tb, np.ascontiguousarray(m), blk.gates))
+    return out
+
+
+def plan_tiles(
+    ops: tuple[Operation, ...], num_qubits: int, budget: float = TILE_BUDGET, tile_bits: int = TILE_BITS, low: int = TILE_LOW
+) -> list[TilePass | Pass]:
+    """Passes for an ideal static circuit as tile passes (statevectors of at least ``tile_bits``
+    qubits; the same gates and order as :func:`plan_statevector`, so results agree to rounding).
+
+    Each pass takes the pending gates that can run next on one set of tile bits: the ``low``
+    lowest bits, then bits added one at a time (two when a two-qubit gate needs both), each time
+    the bit that lets the most pending gates run, then improved by single swaps. A gate can run
+    when its qubits are tile bits and every earlier pending gate on its qubits can run. The pass
+    is cut where its fused operators would cost more than ``budget``. Gates on more than two
+    qubits run as their own dense pass."""
+    n = num_qubits
+    if n < tile_bits:
+        raise ValueError(f"tile passes need at least {tile_bits} qubits")
+    items: list[tuple[Matrix, tuple[int, ...]]] = []
+    phase = 1.0 + 0.0j
+    for op in ops:
+        if isinstance(op, Gate):
+            if len(op.qubits) > KERNEL_MAX_BITS:
+                raise ValueError(f"{op.name} acts on {len(op.qubits)} qubits; gates of up to {KERNEL_MAX_BITS} are supported")
+            items.append((np.asarray(op.matrix, dtype=np.complex128), tuple(op.qubits)))
+        elif isinstance(op, GlobalPhase):
+            phase *= np.exp(1j * op.angle)
+        elif isinstance(op, (Measure, Reset, Barrier)):
+            continue
+        else:
+            raise ValueError(f"a static statevector can't run {type(op).__name__}")
+    masks = [sum(1 << q for q in qs) for _, qs in items]
+    full = (1 << n) - 1
+    low_mask = (1 << low) - 1
+    window = 64 * n  # pending gates looked at when choosing bits (those beyond are blocked)
+
+    def runnable(s: int, pending: list[int]) -> int:
+        blocked = count = 0
+        for gi in pending[:window]:
+            m = masks[gi]
+            if m & blocked or m & ~s:
+                blocked |= m
+                if blocked == full:
+                    break
+            else:
+                count += 1
+        return count
+
+    def choose(pending: list[int]) -> int:
+        s = low_mask | masks[pending[0]]
+        while bin(s).count("1") < tile_bits:
+            outside = [q for q in range(n) if not (s >> q) & 1]
+            base = runnable(s, pending)
+            best = max(outside, key=lambda q: runnable(s | (1 << q), pending))
+            if runnable(s | (1 << best), pending) == base and bin(s).count("1") <= tile_bits - 2:
+                pair = max(((a, b) for i, a in enumerate(outside) for b in outside[i + 1 :]), key=lambda ab: runnable(s | (1 << ab[0]) | (1 << ab[1]), pending))
+                s |= (1 << pair[0]) | (1 << pair[1])
+            else:
+                s |= 1 << best
+        cur = runnable(s, pending)
+        for _ in range(3):
+            better = False
+            for a in [q for q in range(low, n) if (s >> q) & 1 and not (masks[pending[0]] >> q) & 1]:
+                for b in [q for q in range(n) if not (s >> q) & 1]:
+                    t = (s & ~(1 << a)) | (1 << b)
+                    v = runnable(t, pending)
+                    if v > cur:
+                        s, cur, better = t, v, True
+                        break
+            if not better:
+                break
+        return s
+
+    out: list[TilePass | Pass] = []
+    pending = list(range(len(items)))
+    while pending:
+        first = pending[0]
+        if len(items[first][1]) > 2:
+            m, qs = items[first]
+            out.append(_as_pass(m, qs, 1))
+            pending = pending[1:]
+            continue
+        s = choose(pending)
+        bits = tuple(q for q in range(n) if (s >> q) & 1)
+        blocked = 0
+        taken: list[int] = []
+        for gi in pending:
+            m = masks[gi]
+            if m & blocked or m & ~s or len(items[gi][1]) > 2:
+                blocked |= m
+            else:
+                taken.append(gi)
+        # the longest prefix of the runnable gates whose operators fit the budget and the table
+        lo_n, hi_n = 1, len(taken)
+        best_ops = _fuse_in_tile([items[g] for g in taken[:1]], bits)
+        while lo_n < hi_n:
+            mid = (lo_n + hi_n + 1) // 2
+            cand = _fuse_in_tile([items[g] for g in taken[:mid]], bits)
+            if sum(map(_tile_cost, cand)) <= budget and len(cand) <= TILE_MAX_OPS:
+                lo_n, best_ops = mid, cand
+            else:
+                hi_n = mid - 1
+        used = set(taken[:lo_n])
+        out.append(TilePass(bits, best_ops, lo_n))
+        pending = [g for g in pending if g not in used]
+    if phase != 1:
+        for p in out:
+            if p.ops if isinstance(p, TilePass) else True:
+                target = p.ops[0] if isinstance(p, TilePass) else p
+                target.matrix = target.matrix * phase
+                break
+        else:
+            out.append(Pass("scale", (), np.array([phase], dtype=np.complex128), 0))
+    return out

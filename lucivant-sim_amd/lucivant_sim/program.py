@@ -288,10 +288,12 @@ def run_table(
     want_state: bool = False,
     pauli: list[list[tuple[int, int]]] | None = None,
     want_decisions: bool = False,
+    timer: list | None = None,
 ) -> BatchOutputs:
     """:func:`run_programs` over programs already laid end to end: program j is rows
     ``starts[j] : starts[j] + counts[j]`` of ``table``, whose matrix offsets index ``pool``
-    (``optable.static_tables`` builds this form directly)."""
+    (``optable.static_tables`` builds this form directly). ``timer`` (a list) receives a pair of
+    timing events recorded on the launch stream right around the kernel launch."""
     programs = len(starts)
     if programs == 0:
         return BatchOutputs()
@@ -350,6 +352,9 @@ def run_table(
     dec = None
     if want_decisions and d_draws is not None:
         dec = torch.full((instances * ndraws * 2,), float("nan"), dtype=torch.float64, device=device)
+    if timer is not None:
+        timer.append(torch.cuda.Event(enable_timing=True))
+        timer[-1].record(torch.cuda.current_stream())
     nat.call(
         "lsim_program_run",
         dtype,
@@ -372,6 +377,9 @@ def run_table(
         ptr(dec) if dec is not None else None,
         stream_handle(),
     )
+    if timer is not None:
+        timer.append(torch.cuda.Event(enable_timing=True))
+        timer[-1].record(torch.cuda.current_stream())
     if clb is not None:
         out.clbits = clb.cpu().numpy()
     if prb is not None:

@@ -38,10 +38,10 @@ import torch
 from lucivant_sim import _native as nat
 from lucivant_sim import frontend
 from lucivant_sim import program as pg
-from lucivant_sim.engines import MAX_DENSITY_QUBITS, MAX_STATEVECTOR_QUBITS, SimulationError, Statevector, launch_expectations, pauli_masks, ptr, require_device, upload
+from lucivant_sim.engines import MAX_DENSITY_QUBITS, MAX_STATEVECTOR_QUBITS, SimulationError, Statevector, launch_expectations, pauli_masks, plan_launches, ptr, require_device
 from lucivant_sim.frontend import Compiled, compile_one
 from lucivant_sim.noise import NoiseModel
-from lucivant_sim.simulator import FUSION_WIDTH, _sample, _sample_many, _tuned, _use_switches
+from lucivant_sim.simulator import FUSION_WIDTH, _sample, _sample_many, _tile_choice, _tuned, _use_switches
 
 _ids = itertools.count()
 
@@ -131,7 +131,7 @@ class Server:
         self.completed: list[Request] = []
         self.max_lds = pg.max_qubits(dtype)
         # fusion choice per width for the passes of wide requests (planned where they are read)
-        self.widths = {n: ((self.width, 0, 0) if fusion_width else _tuned(n)) for n in range(self.max_lds + 1, 34)}
+        self.widths = {n: ((self.width, 0, 0, None) if fusion_width else (*_tuned(n), _tile_choice(n))) for n in range(self.max_lds + 1, 34)}
         self._stop = threading.Event()
         self._thread: threading.Thread | None = None
         # host sampling of wide count requests (a cumulative sum over 2^n probabilities) runs
@@ -363,21 +363,14 @@ class Server:
                     job.state = Statevector(job.c.num_qubits, dtype=self.dtype)
                     job.passes = job.c.passes
                     assert job.passes is not None
-                    job.mats, addrs = upload([p.matrix for p in job.passes], self.dtype)
+                    # every launch's function and arguments, built once (matrices uploaded)
+                    job.mats, job.calls = plan_launches(
+                        job.state.buffer, self.dtype, job.c.num_qubits, 1, job.passes, job.stream.cuda_stream
+                    )
             except BaseException as exc:  # noqa: BLE001 - reported to the requester
                 self._finish_error(job.req, exc)
                 self._release(job)
                 return
-            # every launch's function and arguments, built once
-            state, n, hs = ptr(job.state.buffer), job.c.num_qubits, job.stream.cuda_stream
-            job.calls = []
-            for p, addr in zip(job.passes, addrs):
-                if p.kind == "scale":
-                    z = complex(p.matrix[0])
-                    job.calls.append((lib.lsim_sv_scale, (state, self.dtype, n, 1, z.real, z.imag, hs)))
-                else:
-                    fn = lib.lsim_sv_apply_dense if p.kind == "dense" else lib.lsim_sv_apply_diag
-                    job.calls.append((fn, (state, self.dtype, n, 1, p.k, nat.int_array(p.bits), addr, None, hs)))
         assert job.calls is not None
         job.req.stamps.setdefault("launched", time.perf_counter())
         _use_switches(job.c.num_qubits)

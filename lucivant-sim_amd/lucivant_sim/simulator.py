@@ -33,7 +33,7 @@ from lucivant_sim.circuit import Circuit, IfElse
 from lucivant_sim.clifford import clifford_problem
 from lucivant_sim.engines import MAX_DENSITY_QUBITS, DensityMatrix, SimulationError, Statevector, marginal
 from lucivant_sim.noise import NoiseModel
-from lucivant_sim.plan import Pass, plan_density, plan_statevector
+from lucivant_sim.plan import TILE_BITS, Pass, plan_density, plan_statevector, plan_tiles
 from lucivant_sim.qasm import load, parse
 
 Method = Literal[
@@ -212,16 +212,23 @@ def choose_method(circuit: Circuit, noise: NoiseModel | None, method: str = "aut
     return method
 
 
-def _statevector_plan(circuit: Circuit, width: int, low_bits: int, low_width: int) -> list[Pass]:
-    """The fused passes of ``circuit``, planned once per recently simulated circuit (an entry
-    holds the circuit itself, so an identity match cannot confuse two circuits)."""
-    key = (id(circuit), width, low_bits, low_width)
+def _statevector_plan(circuit: Circuit, width: int, low_bits: int, low_width: int, tile: tuple[float, int] | None = None) -> list:
+    """The passes of ``circuit`` (tile passes when ``tile`` = (budget, low) is given and the
+    state is wide enough, fused dense passes otherwise), planned once per recently simulated
+    circuit (an entry holds the circuit itself, so an identity match cannot confuse two
+    circuits)."""
+    if tile is not None and circuit.num_qubits < TILE_BITS:
+        tile = None
+    key = (id(circuit), width, low_bits, low_width, tile)
     with _RECENT_LOCK:
         hit = _PLANS.get(key)
         if hit is not None and hit[0] is circuit:
             _PLANS.move_to_end(key)
             return hit[1]
-    passes = plan_statevector(circuit.operations, circuit.num_qubits, width, low_bits, low_width)
+    if tile is not None:
+        passes = plan_tiles(circuit.operations, circuit.num_qubits, budget=tile[0], low=tile[1])
+    else:
+        passes = plan_statevector(circuit.operations, circuit.num_qubits, width, low_bits, low_width)
     with _RECENT_LOCK:
         _PLANS[key] = (circuit, passes)
         while len(_PLANS) > _RECENT:
@@ -234,8 +241,11 @@ def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseMo
     measurements deferred. Runs as fused device passes."""
     _use_switches(state.nbits)
     if isinstance(state, Statevector):
-        w, low_bits, low_width = (width, 0, 0) if width else _tuned(circuit.num_qubits)
-        state.run(_statevector_plan(circuit, w, low_bits, low_width))
+        if width:
+            state.run(_statevector_plan(circuit, width, 0, 0))
+        else:
+            w, low_bits, low_width = _tuned(circuit.num_qubits)
+            state.run(_statevector_plan(circuit, w, low_bits, low_width, _tile_choice(circuit.num_qubits)))
         return
     group: list[Pass] = []
     for item in plan_density(circuit.operations, circuit.num_qubits, noise):
@@ -249,12 +259,13 @@ def _evolve(state: Statevector | DensityMatrix, circuit: Circuit, noise: NoiseMo
     state.run(group)
 
 
-_TUNED: dict[int, tuple[int, int, int, tuple[int, int, int]]] = {}
+_TUNED: dict[int, tuple] = {}
 _TUNED_LOCK = threading.Lock()
 
 
-def _tuned_entry(num_qubits: int) -> tuple[int, int, int, tuple[int, int, int]]:
-    """The autotuner's stored choice for this size (see autotune.py), read once per size."""
+def _tuned_entry(num_qubits: int) -> tuple:
+    """The autotuner's stored choice for this size (see autotune.py), read once per size:
+    (width, low_bits, low_width, switches, tile) with tile = (budget, low) or None."""
     with _TUNED_LOCK:
         hit = _TUNED.get(num_qubits)
     if hit is None:
@@ -263,7 +274,7 @@ def _tuned_entry(num_qubits: int) -> tuple[int, int, int, tuple[int, int, int]]:
         try:
             hit = autotune.apply(num_qubits)
         except Exception:  # noqa: BLE001 - no device facts or an unreadable cache: the default
-            hit = (0, 0, 0, autotune.DEFAULT_SWITCHES)
+            hit = (0, 0, 0, autotune.DEFAULT_SWITCHES, None)
         with _TUNED_LOCK:
             _TUNED[num_qubits] = hit
     return hit
@@ -272,8 +283,14 @@ def _tuned_entry(num_qubits: int) -> tuple[int, int, int, tuple[int, int, int]]:
 def _tuned(num_qubits: int) -> tuple[int, int, int]:
     """(width, low_bits, low_width) from the autotuner's stored choice for this size, else
     (FUSION_WIDTH, 0, 0)."""
-    w, low_bits, low_width, _ = _tuned_entry(num_qubits)
+    w, low_bits, low_width = _tuned_entry(num_qubits)[:3]
     return (w or FUSION_WIDTH, low_bits, low_width)
+
+
+def _tile_choice(num_qubits: int) -> tuple[float, int] | None:
+    """(budget, low) when the autotuner found tile passes faster for this size, else None."""
+    tile = _tuned_entry(num_qubits)[4]
+    return None if tile is None else (float(tile[0]), int(tile[1]))
 
 
 def _use_switches(nbits: int) -> None:

@@ -421,3 +421,35 @@ def test_grouped_pauli_strings_equal_the_oracle(dtype: int, n: int) -> None:
         want = np.array([orc.expectation_sv(v, lbl, n) for lbl in labels])
         assert np.abs(got[b] - want).max() < TOL[dtype]
     assert np.all(np.abs(got[:, 0] - 1) < TOL[dtype])  # the identity
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize(("n", "budget"), [(12, 24.0), (13, 6.0), (16, 24.0), (16, 1000.0)])
+def test_tile_passes_equal_the_oracle(dtype: int, n: int, budget: float) -> None:
+    """Tile passes (lsim_sv_apply_tile: a tile's amplitudes staged in LDS, a run of 1-2 qubit
+    operators applied, written back) on a batch of two random states: every amplitude against the
+    oracle applying the circuit's gates one by one (random layers with cx/cz, a wide gate that
+    runs as its own dense pass, a global phase)."""
+    import lucivant_sim as ls
+    from lucivant_sim import workloads as wl
+    from lucivant_sim.engines import run_plan
+    from lucivant_sim.plan import TilePass, plan_tiles
+
+    print(f"tile passes: n={n} budget={budget} dtype={dtype}", flush=True)
+    src = wl.random_layers(n, 6, 17 + n, two_qubit=("cx", "cz"), measure=False)
+    src += f"ccx q[0], q[{n // 2}], q[{n - 1}];\nh q[{n - 1}];\ngphase(0.3);\ncx q[{n - 1}], q[2];\n"
+    c = ls.parse(src)
+    plan = plan_tiles(c.operations, n, budget=budget)
+    assert any(isinstance(p, TilePass) for p in plan)
+    psi = states(n, 2, 70 + n)
+    s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+    run_plan(s, dtype, n, 2, plan)
+    got = s.cpu().numpy().reshape(2, -1)
+    for b in range(2):
+        want = psi[b].astype(NPDT[dtype]).astype(np.complex128)
+        for op in c.operations:
+            if isinstance(op, ls.circuit.Gate):
+                want = orc.apply_sv(want, np.asarray(op.matrix), tuple(op.qubits), n)
+            elif isinstance(op, ls.circuit.GlobalPhase):
+                want = want * np.exp(1j * op.angle)
+        assert np.abs(got[b] - want).max() < (1e-12 if dtype == nat.C128 else 2e-5)

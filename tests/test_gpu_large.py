@@ -47,17 +47,19 @@ def test_26_qubit_circuit_round_trip_and_observables() -> None:
     assert abs(amp0 - 1) < 1e-10 and rest < 1e-10
 
 
-@pytest.mark.parametrize("width", [4, 5])
-def test_26_qubit_circuit_matches_the_oracle_fixture(width: int) -> None:
+@pytest.mark.parametrize("width", [4, 5, "tile"])
+def test_26_qubit_circuit_matches_the_oracle_fixture(width) -> None:  # type: ignore[no-untyped-def]
     """The 26-qubit configuration against the oracle's own full run (tests/golden/
     make_full_size.py oracle-c2): 4096 amplitudes at seeded indices, probability checksums and the
-    8 Pauli expectations, within 1e-10, for two fusion plans (widths 4 and 5)."""
+    8 Pauli expectations, within 1e-10, for three plans (fused widths 4 and 5, tile passes)."""
+    from lucivant_sim.plan import plan_tiles
+
     rec = json.loads((Path(__file__).parent / "golden" / "full_c2_oracle.json").read_text())
     src, labels = wl.config_c2()
     assert labels == rec["labels"]
     c = ls.parse(src)
     sv = Statevector(26)
-    sv.run(plan_statevector(c.operations, 26, width))
+    sv.run(plan_tiles(c.operations, 26) if width == "tile" else plan_statevector(c.operations, 26, width))
     idx = torch.tensor(rec["indices"], dtype=torch.int64, device=sv.buffer.device)
     got = sv.buffer[idx].cpu().numpy()
     want = np.array([complex(re, im) for re, im in rec["amplitudes"]])

@@ -202,3 +202,34 @@ def test_expectation_requests_with_many_or_no_terms() -> None:
             assert abs(r.result - ls.expectation(r.program, r.observable)) < 1e-10
         else:
             assert r.result == ls.run(r.program, r.shots, seed=r.seed).counts
+
+
+def test_tile_plans_chosen_by_the_autotuner_match_the_oracle(monkeypatch: pytest.MonkeyPatch) -> None:
+    """When the stored choice for a size is tile passes, ``run``, ``expectation`` and the server
+    all plan that size's circuits into tile passes: 18- and 20-qubit results still equal the
+    oracle's (counts exactly, expectations within 1e-10)."""
+    from lucivant_sim import autotune, simulator
+    from oracle import engine as orc
+
+    for n in (18, 20):
+        monkeypatch.setitem(simulator._TUNED, n, (4, 0, 0, autotune.DEFAULT_SWITCHES, (24.0, 4)))
+    assert simulator._tile_choice(20) == (24.0, 4)
+    counts20 = Request(wl.random_layers(20, 6, 123), shots=1000, seed=9)
+    counts20_want = orc.run_counts(ls.parse(counts20.program), 1000, 9)
+    rng = np.random.default_rng(22)
+    hea18 = Request(wl.hea_ansatz(18, 12, 7), observable={lbl: float(w) for lbl, w in zip(wl.pauli_strings(18, 16, 5), rng.uniform(-1, 1, 16))})
+    psi18 = orc.evolve_sv(ls.parse(hea18.program))
+    hea18_want = sum(w * orc.expectation_sv(psi18, lbl, 18) for lbl, w in hea18.observable.items())  # type: ignore[union-attr]
+
+    assert ls.run(counts20.program, 1000, seed=9).counts == counts20_want
+    assert abs(ls.expectation(hea18.program, hea18.observable) - hea18_want) < 1e-10  # type: ignore[arg-type]
+    server = Server(streams=2, chunk_passes=8)
+    assert server.widths[20][3] == (24.0, 4)
+    mix = [counts20, hea18, *_requests()[:2]]
+    for r in mix:
+        server.submit(r)
+    server.run_until_idle()
+    for r in mix:
+        assert r.error is None, r.error
+    assert counts20.result == counts20_want
+    assert abs(hea18.result - hea18_want) < 1e-10

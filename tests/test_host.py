@@ -620,3 +620,37 @@ def test_trajectory_batch_rescales_small_norms_per_slot(monkeypatch) -> None:  #
     noise = dynamic_noise()
     got, _ = _host_trajectory_counts(circuit, noise, 120, 8)
     assert got == orc.trajectory_counts(circuit, noise, 120, np.random.default_rng(8))
+
+
+@pytest.mark.parametrize("budget", [6.0, 24.0, 1000.0])
+def test_tile_passes_equal_sequential_gates(budget: float) -> None:
+    """Applying every tile pass's operators, on the state bits its tile bits name, gives the
+    circuit's state (oracle application, within 1e-12): random layers with cx/cz, a wide gate
+    (run as its own dense pass) and a global phase. Every pass's tile bits include the lowest
+    ones, and its operator cost respects the budget."""
+    from lucivant_sim.plan import TILE_LOW, TilePass, _tile_cost, plan_tiles
+
+    n = 14
+    src = wl.random_layers(n, 6, 91, two_qubit=("cx", "cz"), measure=False)
+    src += "ccx q[0], q[7], q[13];\nh q[13];\ngphase(0.3);\ncx q[13], q[2];\n"
+    c = ls.parse(src)
+    want = orc.evolve_sv(c)
+    plan = plan_tiles(c.operations, n, budget=budget)
+    psi = np.zeros(2**n, dtype=np.complex128)
+    psi[0] = 1
+    gates = 0
+    for p in plan:
+        if isinstance(p, TilePass):
+            assert len(p.bits) == 12 and p.bits[:TILE_LOW] == tuple(range(TILE_LOW)) and list(p.bits) == sorted(p.bits)
+            assert sum(map(_tile_cost, p.ops)) <= budget or len(p.ops) == 1
+            for op in p.ops:
+                qubits = tuple(p.bits[b] for b in op.bits)
+                m = np.diag(op.matrix) if op.kind == "diag" else op.matrix
+                psi = orc.apply_sv(psi, m, qubits, n)
+            gates += p.gates
+        else:
+            m = np.diag(p.matrix) if p.kind == "diag" else p.matrix
+            psi = orc.apply_sv(psi, m, p.bits, n) if p.kind != "scale" else psi * p.matrix[0]
+            gates += p.gates
+    assert np.abs(psi - want).max() < 1e-12
+    assert gates == sum(1 for op in c.operations if isinstance(op, ls.circuit.Gate))
