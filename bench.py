@@ -12,9 +12,11 @@ Printed as one JSON line:
 - ``roofline``: the dense fused-gate kernel, bytes per launch 2 x 16 x 2^26 over its average
   launch duration measured with HIP events on the launch stream, against the 8 TB/s HBM peak;
   ``traffic`` comes from profiles/ when a rocprofv3 counter summary is committed there;
-- ``cpu_baseline``: the oracle's NumPy tensordot (the reference's own contraction,
-  engines.py:38-42) applying the circuit's first gates at 26 qubits on one host thread, in the
-  same unit.
+- ``cpu_baseline``: the reference's own CPU route restated in the oracle (one-qubit runs fused,
+  then its size-dependent slice dispatch, engines.py:45-119) applying the circuit's first gates
+  at 26 qubits on one host thread, in the same unit;
+- ``reference_runs``: the reference's own recorded seconds on the 12-, 13- and 20-qubit
+  configurations beside this build's GPU seconds for the same runs.
 """
 
 from __future__ import annotations
@@ -38,33 +40,75 @@ BYTES_PER_PASS = 2 * 16 * (1 << N)
 
 
 def cpu_baseline(circuit, budget_s: float = 15.0) -> dict:  # type: ignore[no-untyped-def]
-    """The oracle applying gates one by one at 26 qubits (NumPy tensordot, one thread)."""
+    """The reference's own CPU route for this circuit, restated in the oracle, on one host thread:
+    runs of one-qubit gates fused (simulator.py:196-236, as its ideal statevector evolution does),
+    then each gate through its size-dependent dispatch at 26 qubits (engines.py:45-119): a batched
+    2x2 product for one-qubit gates with at least 8 entries below their axis, slice moves for
+    cx/cz, slice combinations for the lowest three qubits. Gates are applied in circuit order
+    until the budget is spent; the per-gate mean gives the GB/s figure and the estimated seconds
+    per whole simulation."""
     import numpy as np
     from threadpoolctl import threadpool_limits
 
     from lucivant_sim.circuit import Gate
     from oracle import engine as orc
 
-    gates = [op for op in circuit.operations if isinstance(op, Gate)]
+    gates = [op for op in orc.fuse_one_qubit_runs(circuit.operations) if isinstance(op, Gate)]
     rng = np.random.default_rng(0)
-    psi = (rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)).astype(np.complex128)
-    psi /= np.linalg.norm(psi)
+    t = (rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)).astype(np.complex128)
+    t /= np.linalg.norm(t)
+    t = t.reshape([2] * N)
     done, elapsed = 0, 0.0
+    routes = {"batched_2x2": 0, "slices": 0}
     with threadpool_limits(limits=1):
         while done < len(gates) and elapsed < budget_s:
             g = gates[done]
+            axes = [N - 1 - q for q in g.qubits]
+            low = len(axes) == 1 and (1 << (N - 1 - axes[0])) < orc.BATCHED_FROM_INNER
+            routes["slices" if len(axes) == 2 or low else "batched_2x2"] += 1
             t0 = time.perf_counter()
-            psi = orc.apply_sv(psi, g.matrix, g.qubits, N)
+            t = orc.apply_slices(t, g.matrix, axes)
             elapsed += time.perf_counter() - t0
             done += 1
+    per_gate = elapsed / done
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
     return {
         "value": round(done * BYTES_PER_PASS / elapsed / 1e9, 3),
         "unit": "GB/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"first {done} of {len(gates)} gates of the 26-qubit circuit, one NumPy tensordot per gate (unfused), "
-        f"{elapsed:.1f} s on one host thread; GB/s counts 2 x 16 B x 2^26 per gate",
+        "sample": f"first {done} of the {len(gates)} gates left after one-qubit-run fusion of the 26-qubit circuit, "
+        f"each through the reference's slice dispatch ({routes['batched_2x2']} batched 2x2 products, "
+        f"{routes['slices']} slice updates), {elapsed:.1f} s on one host thread; GB/s counts 2 x 16 B x 2^26 per gate",
+        "seconds_per_gate": round(per_gate, 4),
+        "est_seconds_per_simulation": round(per_gate * len(gates), 1),
+        "host_cpus": os.cpu_count(),
+        "host_cpus_in_affinity": affinity,
+        "threads": "threadpoolctl limit 1 (BLAS and OpenMP pools)",
     }
+
+
+def reference_runs(gpu: dict) -> dict:
+    """The reference's own seconds on the configurations it could run, recorded when the golden
+    fixtures were written (tests/golden/runs.json, full_c3_reference.json: lucivant-sim 0.3.0 on an
+    8-CPU container, thread pools unset), beside this build's GPU seconds for the same runs."""
+    gold = ROOT / "tests" / "golden"
+    runs = json.loads((gold / "runs.json").read_text())
+    c3 = json.loads((gold / "full_c3_reference.json").read_text())
+    env = runs["env"]
+    out = {
+        "reference_host": f"{env['cpus']} CPUs, numpy {env['numpy']}, thread pools unset",
+        "c1_12q_1000_shots": {"reference_s": round(runs["runs"]["c1_12q_seed7"]["wall_seconds"], 4), "gpu_s": gpu.get("c1")},
+        "c3_13q_density_4000_shots": {"reference_s": round(c3["wall_seconds"], 1), "gpu_s": gpu.get("c3")},
+        "c4_20q_256_shots": {"reference_s": round(runs["runs"]["c4_20q_256shots_seed7"]["wall_seconds"], 1), "gpu_s": gpu.get("c4")},
+    }
+    for row in out.values():
+        if isinstance(row, dict) and row.get("gpu_s"):
+            row["ratio"] = round(row["reference_s"] / row["gpu_s"], 1)
+    return out
 
 
 def plain_counts(circuit, shots: int, seed: int):  # type: ignore[no-untyped-def]
@@ -416,6 +460,28 @@ def bench_trajectories(shots: int = 4096) -> dict:
         "shots_with_different_bits": int(np.sum(a != b)),
         "tvd": round(0.5 * sum(abs(pa.get(k, 0) - pb.get(k, 0)) for k in keys), 6),
     }
+    return out
+
+
+def bench_reference_configs() -> dict:
+    """GPU seconds of the runs whose reference seconds are recorded in tests/golden: the 12-qubit
+    configuration (1000 shots, seed 7) and the 20-qubit trajectory configuration (256 shots,
+    seed 7), each through ls.run from OpenQASM text, after one warm-up; counts must equal the
+    reference's."""
+    import lucivant_sim as ls
+    from lucivant_sim import workloads
+
+    gold = ROOT / "tests" / "golden"
+    progs = json.loads((gold / "programs.json").read_text())
+    runs = json.loads((gold / "runs.json").read_text())["runs"]
+    out = {}
+    for key, name, noise in (("c1", "c1_12q_seed7", None), ("c4", "c4_20q_256shots_seed7", workloads.noise_c4())):
+        rec = runs[name]
+        ls.run(progs[name], rec["shots"], seed=rec["seed"], noise=noise, method=rec["method"])
+        t0 = time.perf_counter()
+        res = ls.run(progs[name], rec["shots"], seed=rec["seed"], noise=noise, method=rec["method"])
+        out[key] = round(time.perf_counter() - t0, 4)
+        assert res.counts == rec["counts"], name
     return out
 
 
@@ -886,6 +952,10 @@ def main() -> int:
         out["graph"] = bench_graph()
         out["trajectories_20q"] = bench_trajectories()
         out["density_matrix_13q"] = bench_density_matrix()
+    if not args.no_extras:
+        gpu_s = bench_reference_configs()
+        gpu_s["c3"] = out["density_matrix_13q"]["run_4000_shots_seconds"]
+        out["reference_runs"] = reference_runs(gpu_s)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(circuit)
     print(json.dumps(out), flush=True)

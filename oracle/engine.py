@@ -61,6 +61,115 @@ def full_operator(matrix: np.ndarray, qubits: tuple[int, ...], n: int) -> np.nda
     return np.where(same, np.asarray(matrix)[r[:, None], r[None, :]], 0)
 
 
+# ------------------------------------------------- the reference's dispatch on large states
+#
+# The reference applies gates through a size-dependent dispatch (engines.py:45-119) rather than
+# always through the tensordot above. Its results equal the contraction's up to rounding; it is
+# restated here because it sets the reference's CPU cost, which the CPU baseline times.
+
+#: state sizes (entries) from which slices replace the contraction: permutation-with-phase
+#: gates from 2^14, general matrices from 2^18 (engines.py:20-24)
+SLICE_FROM_MOVES = 1 << 14
+SLICE_FROM_GENERAL = 1 << 18
+#: a general one-qubit gate is one batched 2x2 product when at least this many entries lie below
+#: its axis (engines.py:25-27)
+BATCHED_FROM_INNER = 8
+
+
+def _is_permutation_with_phases(m: np.ndarray) -> bool:
+    nz = m != 0
+    return bool(np.all(nz.sum(axis=0) == 1) and np.all(nz.sum(axis=1) == 1))
+
+
+def apply_slices(tensor: np.ndarray, matrix: np.ndarray, axes: list[int]) -> np.ndarray:
+    """U on ``axes`` of the [2]*n tensor, choosing the reference's route (engines.py:45-119):
+
+    - no axes, more than two, or fewer than 2^14 entries: the contraction;
+    - a one-qubit gate that is not a permutation with phases, with at least 8 entries below its
+      axis: one batched product of U with the (outer, 2, inner) view;
+    - below 2^14 entries (permutations) or 2^18 (general): the contraction;
+    - otherwise the 2^k slices of the tensor cut by the gate's axes (first axis = matrix MSB)
+      are rewritten in place: a permutation with phases moves (and scales) slices, a general
+      matrix forms each new slice as the U-weighted sum of copies of the old ones, skipping
+      zero entries (an all-zero row clears its slice).
+
+    Returns the updated tensor (the input itself on the in-place route)."""
+    k = len(axes)
+    if k == 0 or k > 2 or tensor.size < SLICE_FROM_MOVES:
+        return contract(tensor, matrix, axes)
+    u = np.asarray(matrix, dtype=np.complex128)
+    moves = _is_permutation_with_phases(u)
+    if k == 1 and not moves:
+        below = 1 << (tensor.ndim - 1 - axes[0])
+        if below >= BATCHED_FROM_INNER:
+            blocks = np.matmul(u, tensor.reshape(-1, 2, below))
+            return blocks.reshape(tensor.shape)
+    if tensor.size < (SLICE_FROM_MOVES if moves else SLICE_FROM_GENERAL):
+        return contract(tensor, matrix, axes)
+    t = tensor if tensor.flags.c_contiguous else np.ascontiguousarray(tensor)
+    # a view with the gate's axes leading, first axis first: view[b0, b1] is a writable slice
+    view = np.moveaxis(t, axes, list(range(k)))
+    d = 1 << k
+
+    def part(r: int) -> np.ndarray:
+        return view[tuple((r >> (k - 1 - i)) & 1 for i in range(k))]
+
+    if moves:
+        source = [int(np.flatnonzero(u[r])[0]) for r in range(d)]
+        kept = {c: part(c).copy() for r, c in enumerate(source) if c != r}
+        for r, c in enumerate(source):
+            w = u[r, c]
+            if c == r:
+                if w != 1:
+                    part(r)[...] *= w
+            elif w == 1:
+                part(r)[...] = kept[c]
+            else:
+                np.multiply(kept[c], w, out=part(r))
+        return t
+    old = [part(c).copy() for c in range(d)]
+    for r in range(d):
+        dst = part(r)
+        cols = [c for c in range(d) if u[r, c] != 0]
+        if not cols:
+            dst[...] = 0
+            continue
+        np.multiply(old[cols[0]], u[r, cols[0]], out=dst)
+        for c in cols[1:]:
+            dst += u[r, c] * old[c]
+    return t
+
+
+def fuse_one_qubit_runs(ops):  # type: ignore[no-untyped-def]
+    """Consecutive one-qubit gates on a qubit become one gate, the product of their matrices
+    (later @ earlier); a wider gate, a measurement or a reset on the qubit ends its run, and an
+    if/else or barrier ends every run (simulator.py:196-226, applied to ideal statevectors)."""
+    from lucivant_sim.circuit import Barrier, Gate, IfElse, Measure, Reset
+
+    out: list = []
+    open_run: dict[int, int] = {}
+    for op in ops:
+        if isinstance(op, Gate) and len(op.qubits) == 1:
+            q = op.qubits[0]
+            if q in open_run:
+                at = open_run[q]
+                fused = np.asarray(op.matrix, dtype=np.complex128) @ np.asarray(out[at].matrix, dtype=np.complex128)
+                out[at] = Gate(name="fused", qubits=(q,), matrix=fused)
+            else:
+                open_run[q] = len(out)
+                out.append(op)
+            continue
+        if isinstance(op, (IfElse, Barrier)):
+            open_run.clear()
+        elif isinstance(op, Gate):
+            for q in op.qubits:
+                open_run.pop(q, None)
+        elif isinstance(op, (Measure, Reset)):
+            open_run.pop(op.qubit, None)
+        out.append(op)
+    return out
+
+
 # ------------------------------------------------------------------------------- statevector
 
 

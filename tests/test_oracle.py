@@ -59,6 +59,69 @@ def test_contraction_matches_the_brute_force_operator_on_every_placement(n: int,
             assert np.allclose(got, want, atol=1e-13), (name, qubits)
 
 
+_S = np.diag([1, 1j]).astype(np.complex128)
+_SWAP = np.eye(4, dtype=np.complex128)[[0, 2, 1, 3]]
+_CZ = np.diag([1, 1, 1, -1]).astype(np.complex128)
+_ISWAPISH = np.array([[1, 0, 0, 0], [0, 0, 1j, 0], [0, -1, 0, 0], [0, 0, 0, 1j]], dtype=np.complex128)
+
+
+@pytest.mark.parametrize(
+    "n, matrix, qubits",
+    [
+        (13, _unitary(1, 5), (4,)),  # below 2^14 entries: the contraction
+        (15, _unitary(1, 6), (3,)),  # 8 entries below the axis: the batched 2x2 product
+        (15, _unitary(1, 6), (14,)),
+        (15, _unitary(1, 7), (1,)),  # general, too few entries below and below 2^18: contraction
+        (18, _unitary(1, 7), (1,)),  # general slices
+        (18, _unitary(1, 8), (0,)),
+        (18, MATRICES[1]["projector"], (2,)),  # an all-zero row clears its slice
+        (14, orc.PAULI["X"], (0,)),  # moves
+        (14, orc.PAULI["Y"], (13,)),
+        (14, _S, (6,)),
+        (14, MATRICES[2]["cx"], (0, 9)),  # asymmetric: control and target must not swap
+        (14, MATRICES[2]["cx"], (9, 0)),
+        (14, _SWAP, (2, 11)),
+        (14, _CZ, (5, 1)),
+        (14, _ISWAPISH, (12, 3)),
+        (15, _unitary(2, 9), (3, 10)),  # general two-qubit below 2^18: contraction
+        (18, _unitary(2, 9), (3, 10)),  # general two-qubit slices
+        (18, MATRICES[2]["lower"], (17, 0)),
+    ],
+)
+def test_reference_dispatch_equals_the_contraction(n: int, matrix: np.ndarray, qubits: tuple[int, ...]) -> None:
+    """Every route of the reference's size-dependent dispatch gives the contraction's result,
+    which the brute-force operators pin; the contraction is applied to an untouched copy because
+    the slice routes work in place."""
+    psi = _state(n, 3 * n + qubits[0]).reshape([2] * n)
+    axes = [n - 1 - q for q in qubits]
+    want = orc.contract(psi.copy(), matrix, axes)
+    got = orc.apply_slices(psi.copy(), matrix, axes)
+    assert got.shape == psi.shape
+    assert np.allclose(got, want, rtol=0, atol=1e-14)
+
+
+def test_one_qubit_runs_fuse_into_their_product() -> None:
+    """Runs of one-qubit gates on a qubit become one gate; a two-qubit gate on the qubit ends the
+    run, an if/else or barrier ends all of them, and the fused circuit's state equals the
+    unfused one's."""
+    from lucivant_sim.circuit import Barrier, Gate
+
+    src = "OPENQASM 3.0;\ninclude \"stdgates.inc\";\nqubit[3] q;\nh q[0];\nt q[0];\nrx(0.3) q[1];\ncx q[0], q[1];\ns q[0];\nsx q[0];\nry(0.7) q[2];\nrz(0.2) q[2];\n"
+    circuit = ls.parse(src)
+    fused = orc.fuse_one_qubit_runs(circuit.operations)
+    assert [len(op.qubits) for op in fused if isinstance(op, Gate)] == [1, 1, 2, 1, 1]
+    assert [op.name for op in fused if isinstance(op, Gate)] == ["fused", "rx", "cx", "fused", "fused"]
+    n = 3
+    psi = np.zeros(8, dtype=np.complex128)
+    psi[0] = 1
+    for op in fused:
+        psi = orc.apply_sv(psi, op.matrix, op.qubits, n)
+    assert np.allclose(psi, orc.evolve_sv(circuit), atol=1e-14)
+    h = circuit.operations[0]
+    split = orc.fuse_one_qubit_runs((h, Barrier(qubits=(1,)), h))
+    assert len([op for op in split if isinstance(op, Gate)]) == 2
+
+
 def test_full_operator_is_little_endian_with_the_first_qubit_most_significant() -> None:
     # cx with control qubit 0 and target qubit 1 maps |01> (index 1: qubit 0 set) to |11> (index 3)
     cx = MATRICES[2]["cx"]
