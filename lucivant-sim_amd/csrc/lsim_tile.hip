@@ -31,15 +31,21 @@ constexpr int kTileSize = 1 << kTileBits;
 constexpr int kRegs = kTileSize / kBlock;  // 16 amplitudes per thread: 4 register positions
 constexpr int kMaxTileOps = LSIM_MAX_TILE_OPS;
 constexpr int kMaxSegs = kMaxTileOps + 2;
+// Workgroups of a tile launch (per state): four per CU of the 256 on MI355X (two resident at a
+// time), each working through its share of the tiles with the next one in flight.
+constexpr uint64_t kTileGrid = 1024;
 
+// Every field is 32-bit: the kernel indexes the arrays with wavefront-uniform indices, and
+// 32-bit kernel-argument words are read with scalar loads (narrower ones would take vector
+// loads, whose waits would also wait for the prefetched amplitudes).
 struct TileProgram {
-  uint8_t bits[kTileBits];        // state bit of tile position i, ascending
-  uint16_t layout[kMaxSegs];      // segment s: mask of its 4 register positions
-  uint8_t first[kMaxSegs + 1];    // segment s runs operators first[s] .. first[s+1]-1
-  uint8_t kind[kMaxTileOps];      // LSIM_TILE_DENSE or LSIM_TILE_DIAG
-  uint8_t k[kMaxTileOps];         // 1 or 2
-  uint8_t a[kMaxTileOps];         // dense: register slot of the most significant matrix bit;
-  uint8_t b[kMaxTileOps];         //   diagonal: tile positions; b unused for k = 1
+  uint32_t bits[kTileBits];       // state bit of tile position i, ascending
+  uint32_t layout[kMaxSegs];      // segment s: mask of its 4 register positions
+  uint32_t first[kMaxSegs + 1];   // segment s runs operators first[s] .. first[s+1]-1
+  uint32_t kind[kMaxTileOps];     // LSIM_TILE_DENSE or LSIM_TILE_DIAG
+  uint32_t k[kMaxTileOps];        // 1 or 2
+  uint32_t a[kMaxTileOps];        // dense: register slot of the most significant matrix bit;
+  uint32_t b[kMaxTileOps];        //   diagonal: tile positions; b unused for k = 1
   uint32_t mat[kMaxTileOps];      // element offset of the operator in the complex128 pool
   int nsegs;
 };
@@ -104,7 +110,7 @@ __device__ __forceinline__ uint32_t entry(uint32_t t, uint32_t k, uint32_t R) {
 }
 
 // State offset of tile index e: bit i of e at state bit bits[i].
-__device__ __forceinline__ uint32_t deposit(uint32_t e, const uint8_t* bits) {
+__device__ __forceinline__ uint32_t deposit(uint32_t e, const uint32_t* bits) {
   uint32_t o = 0;
 #pragma unroll
   for (int i = 0; i < kTileBits; ++i) o |= ((e >> i) & 1u) << bits[i];
@@ -189,6 +195,80 @@ __device__ __forceinline__ void layout_bits(uint32_t layout, uint32_t (&P)[4]) {
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// State offset of the register part of a tile index: the deposit of kpart(k, P) (wavefront-
+// uniform). deposit() maps disjoint bit sets to disjoint bit sets, so an entry's offset is the
+// thread part's offset OR this.
+__device__ __forceinline__ uint32_t kofs(int k, const uint32_t (&P)[4], const uint32_t* bits) {
+  return deposit(kpart(k, P), bits);
+}
+
+// The tile's state offset: the tile number q with zeros inserted at the tile bits.
+__device__ __forceinline__ uint64_t tile_base(uint64_t q, const uint32_t* bits) {
+#pragma unroll
+  for (int i = 0; i < kTileBits; ++i) {
+    const uint32_t p = bits[i];
+    q = ((q >> p) << (p + 1)) | (q & ((1ull << p) - 1));
+  }
+  return q;
+}
+
+// Every segment of the program on one tile held in registers (a, in the load layout on entry),
+// regrouping through LDS between segments; on return (tpart, P) describe the final layout.
+template <typename R, typename V>
+__device__ __forceinline__ void run_program(V (&a)[kRegs], V* tile, const TileProgram& prog,
+                                            const double2* __restrict__ mats, uint32_t t,
+                                            uint32_t& tpart, uint32_t (&P)[4]) {
+  uint32_t layout = uniform(prog.layout[0]);
+  tpart = entry(t, 0, layout);
+  layout_bits(layout, P);
+  const int nsegs = (int)uniform((uint32_t)prog.nsegs);
+  for (int s = 0; s < nsegs; ++s) {
+    if (s > 0) {
+      // into segment s's layout (the planner starts a segment only where the layout changes)
+      // through LDS
+      __syncthreads();  // the previous exchange (or the previous tile's) has been read
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) tile[tpart | kpart(k, P)] = a[k];
+      layout = uniform(prog.layout[s]);
+      tpart = entry(t, 0, layout);
+      layout_bits(layout, P);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) a[k] = tile[tpart | kpart(k, P)];
+    }
+    const int j1 = (int)uniform(prog.first[s + 1]);
+    for (int j = (int)uniform(prog.first[s]); j < j1; ++j) {
+      const double2* M = mats + uniform(prog.mat[j]);
+      const uint32_t kind = uniform(prog.kind[j]), kk = uniform(prog.k[j]);
+      const uint32_t pa = uniform(prog.a[j]), pb = uniform(prog.b[j]);
+      if (kind == LSIM_TILE_DENSE) {
+        dense_op<R>(a, (int)kk, (int)pa, (int)pb, M);
+        continue;
+      }
+      // diagonal: the entry's own bits pick the factor
+      const bool two = kk == 2;
+      const double2 d0 = M[0], d1 = M[1];
+      const double2 d2 = two ? M[2] : d0, d3 = two ? M[3] : d1;
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) {
+        const uint32_t e = tpart | kpart(k, P);
+        // factor index (hb, lb): selects written out, so no per-lane indexed array
+        const bool hb = two && ((e >> pa) & 1u);
+        const bool lb = two ? ((e >> pb) & 1u) : ((e >> pa) & 1u);
+        double2 d;
+        d.x = hb ? (lb ? d3.x : d2.x) : (lb ? d1.x : d0.x);
+        d.y = hb ? (lb ? d3.y : d2.y) : (lb ? d1.y : d0.y);
+        V y = {0, 0};
+        a[k] = mac<R>(y, d, a[k]);
+      }
+    }
+  }
+}
+
+// Persistent workgroups: workgroup w takes tiles w, w + G, w + 2G, ... (G = gridDim.x). While one
+// tile is computed in registers, the next one's amplitudes are already being loaded into a second
+// register set, so HBM traffic overlaps the FP64 and LDS work instead of alternating with it.
+// (The barriers of the LDS exchanges wait only for LDS: the prefetch stays in flight.)
 template <typename R, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_tile(typename Cx<R>::V* __restrict__ states, uint32_t nbits, TileProgram prog,
@@ -199,71 +279,56 @@ __global__ void __launch_bounds__(kBlock)
   static_assert(kMaxBits <= 32, "tile offsets are 32-bit");
   const uint64_t tiles = 1ull << (nbits - kTileBits);
   const uint32_t t = threadIdx.x;
-  const uint32_t load_layout = prog.layout[0];
-  // state offsets (below 2^kMaxBits) of this thread's entries in the layout tiles are read in
-  uint32_t lo_off[kRegs];
+  // offsets of this thread's entries in the layouts tiles are read and written in: a per-thread
+  // part and a wavefront-uniform part per register. The last segment's layout is the one every
+  // tile is written back in (an exchange runs exactly where the layout changes).
+  const uint32_t load_layout = uniform(prog.layout[0]);
+  const uint32_t store_layout = uniform(prog.layout[uniform((uint32_t)prog.nsegs) - 1]);
+  uint32_t P0[4], PS[4];
+  layout_bits(load_layout, P0);
+  layout_bits(store_layout, PS);
+  const uint32_t toff = deposit(entry(t, 0, load_layout), prog.bits);
+  const uint32_t tsoff = deposit(entry(t, 0, store_layout), prog.bits);
+  uint32_t lofs[kRegs], sofs[kRegs];
 #pragma unroll
-  for (int k = 0; k < kRegs; ++k) lo_off[k] = deposit(entry(t, k, load_layout), prog.bits);
-  for (uint64_t q = blockIdx.x; q < tiles; q += gridDim.x) {
-    uint64_t base = q;  // the tile's other bits: zeros inserted at the tile bits
+  for (int k = 0; k < kRegs; ++k) {
+    lofs[k] = uniform(kofs(k, P0, prog.bits));
+    sofs[k] = uniform(kofs(k, PS, prog.bits));
+  }
+  uint64_t q = blockIdx.x;
+  if (q >= tiles) return;
+  V a[kRegs], b[kRegs];
+  auto load = [&](V (&dst)[kRegs], uint64_t tq) {
+    const V* src = st + (tile_base(tq, prog.bits) | toff);
 #pragma unroll
-    for (int i = 0; i < kTileBits; ++i) {
-      const uint32_t p = prog.bits[i];
-      base = ((base >> p) << (p + 1)) | (base & ((1ull << p) - 1));
-    }
-    V a[kRegs];
+    for (int k = 0; k < kRegs; ++k) dst[k] = load_amp<NT>(src + lofs[k]);
+  };
+  auto finish = [&](V (&v)[kRegs], uint64_t tq) {
+    uint32_t tpart, P[4];
+    run_program<R>(v, tile, prog, mats, t, tpart, P);
+    V* dst = st + (tile_base(tq, prog.bits) | tsoff);
 #pragma unroll
-    for (int k = 0; k < kRegs; ++k) a[k] = load_amp<NT>(st + (base | lo_off[k]));
-    uint32_t layout = load_layout;
-    // entry(t, k, layout) = tpart | kpart(k): the thread's part, and the register part (the same
-    // in every thread, computed on the scalar unit)
-    uint32_t tpart = entry(t, 0, layout);
-    uint32_t P[4];
-    layout_bits(layout, P);
-    const int nsegs = (int)uniform((uint32_t)prog.nsegs);
-    for (int s = 0; s < nsegs; ++s) {
-      const uint32_t next = uniform(prog.layout[s]);
-      if (s > 0 && next != layout) {
-        // into the next layout through LDS
-        __syncthreads();  // the previous exchange has been read
-#pragma unroll
-        for (int k = 0; k < kRegs; ++k) tile[tpart | kpart(k, P)] = a[k];
-        layout = next;
-        tpart = entry(t, 0, layout);
-        layout_bits(layout, P);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kRegs; ++k) a[k] = tile[tpart | kpart(k, P)];
-      }
-      const int j1 = (int)uniform(prog.first[s + 1]);
-      for (int j = (int)uniform(prog.first[s]); j < j1; ++j) {
-        const double2* M = mats + uniform(prog.mat[j]);
-        const uint32_t kind = uniform(prog.kind[j]), kk = uniform(prog.k[j]);
-        const uint32_t pa = uniform(prog.a[j]), pb = uniform(prog.b[j]);
-        if (kind == LSIM_TILE_DENSE) {
-          dense_op<R>(a, (int)kk, (int)pa, (int)pb, M);
-          continue;
-        }
-        // diagonal: the entry's own bits pick the factor
-        const bool two = kk == 2;
-        const double2 d0 = M[0], d1 = M[1];
-        const double2 d2 = two ? M[2] : d0, d3 = two ? M[3] : d1;
-#pragma unroll
-        for (int k = 0; k < kRegs; ++k) {
-          const uint32_t e = tpart | kpart(k, P);
-          // factor index (hb, lb): selects written out, so no per-lane indexed array
-          const bool hb = two && ((e >> pa) & 1u);
-          const bool lb = two ? ((e >> pb) & 1u) : ((e >> pa) & 1u);
-          double2 d;
-          d.x = hb ? (lb ? d3.x : d2.x) : (lb ? d1.x : d0.x);
-          d.y = hb ? (lb ? d3.y : d2.y) : (lb ? d1.y : d0.y);
-          V y = {0, 0};
-          a[k] = mac<R>(y, d, a[k]);
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kRegs; ++k) store_amp<NT>(st + (base | deposit(tpart | kpart(k, P), prog.bits)), a[k]);
+    for (int k = 0; k < kRegs; ++k) store_amp<NT>(dst + sofs[k], v[k]);
+  };
+  // Vector memory operations complete in order of issue. After a tile's stores are issued, the
+  // wait until at most kRegs operations are outstanding (those stores) means the next tile's
+  // loads, issued before them, have landed; the copy into `a` keeps one register set live across
+  // the loop, so the compiler's own waits know `a` is complete and the LDS exchanges and the
+  // arithmetic wait for nothing in flight. (s_waitcnt immediates: vmcnt in bits 3:0 and 15:14;
+  // expcnt 7 and lgkmcnt 15 wait for nothing.)
+  constexpr int kWaitLoads = ((kRegs & 15) | ((kRegs >> 4) << 14)) | (7 << 4) | (15 << 8);
+  constexpr int kWaitAll = (7 << 4) | (15 << 8);
+  load(a, q);
+  __builtin_amdgcn_s_waitcnt(kWaitAll);
+  for (;;) {
+    const uint64_t q1 = q + gridDim.x;
+    const bool more = q1 < tiles;
+    if (more) load(b, q1);
+    finish(a, q);
+    if (!more) break;
+    __builtin_amdgcn_s_waitcnt(kWaitLoads);
+    for (int k = 0; k < kRegs; ++k) a[k] = b[k];
+    q = q1;
   }
 }
 
@@ -288,6 +353,77 @@ static uint32_t fill_layout(const lsim_tile_op* ops, int from, int nops, uint32_
   for (int p = kTileBits - 1; p >= 0 && __builtin_popcount(R) < 4; --p)
     if (!((R >> p) & 1u) && !((avoid >> p) & 1u)) R |= 1u << p;
   return R;
+}
+
+static uint32_t op_mask(const lsim_tile_op& o) {
+  return (1u << o.t[0]) | (o.k == 2 ? 1u << o.t[1] : 0u);
+}
+
+// Reorders the operators into runs that each fit one register layout, so that fewer LDS
+// exchanges are needed. Repeatedly, over the sets of 4 tile positions (in the first round, those
+// off `hbm` when one lets any operator run), take the set that lets the most remaining operators
+// run, preferring sets off `hbm` on ties, and emit those operators in their original order. An
+// operator runs when no earlier remaining operator that does not run shares a position with it,
+// and, if it is dense, when its positions lie in the set (diagonal ones run in any layout).
+// Operators on disjoint positions commute, so the product is unchanged up to rounding.
+static int runnable(const uint32_t* masks, const bool* dense, const int* left, int nleft, uint32_t R,
+                    int* taken) {
+  uint32_t blocked = 0;
+  int count = 0;
+  for (int i = 0; i < nleft; ++i) {
+    const int j = left[i];
+    const uint32_t m = masks[j];
+    if ((m & blocked) || (dense[j] && (m & ~R))) {
+      blocked |= m;
+    } else {
+      if (taken) taken[count] = j;
+      ++count;
+    }
+  }
+  return count;
+}
+
+static void schedule_ops(int nops, const lsim_tile_op* ops, uint32_t hbm, lsim_tile_op* out) {
+  uint32_t masks[kMaxTileOps];
+  bool dense[kMaxTileOps];
+  int left[kMaxTileOps], taken[kMaxTileOps], nleft = nops, nout = 0;
+  for (int j = 0; j < nops; ++j) {
+    masks[j] = (1u << ops[j].t[0]) | (ops[j].k == 2 ? 1u << ops[j].t[1] : 0u);
+    dense[j] = ops[j].kind == LSIM_TILE_DENSE;
+    left[j] = j;
+  }
+  bool first = true;
+  while (nleft > 0) {
+    uint32_t best = 0;
+    int best_count = 0;
+    bool best_off = false;
+    for (int pass = 0; pass < 2 && best_count == 0; ++pass) {
+      for (uint32_t R = 0; R < (1u << kTileBits); ++R) {
+        if (__builtin_popcount(R) != 4) continue;
+        if (pass == 0 && first && (R & hbm)) continue;
+        const int count = runnable(masks, dense, left, nleft, R, nullptr);
+        const bool off = !(R & hbm);
+        if (count > best_count || (count == best_count && count > 0 && off && !best_off)) {
+          best = R;
+          best_count = count;
+          best_off = off;
+        }
+      }
+    }
+    int n = runnable(masks, dense, left, nleft, best, taken);
+    if (n == 0) taken[n++] = left[0];  // unreachable: a set holding the first operator's positions runs it
+    // emit the taken operators and keep the rest in order
+    int keep = 0, t = 0;
+    for (int i = 0; i < nleft; ++i) {
+      if (t < n && left[i] == taken[t]) {
+        out[nout++] = ops[taken[t++]];
+      } else {
+        left[keep++] = left[i];
+      }
+    }
+    nleft = keep;
+    first = false;
+  }
 }
 
 static int make_program(const int* bits, int nops, const lsim_tile_op* ops, TileProgram* prog) {
@@ -331,8 +467,10 @@ static int make_program(const int* bits, int nops, const lsim_tile_op* ops, Tile
   }
   prog->nsegs = s + 1;
   prog->first[s + 1] = (uint8_t)nops;
-  for (int i = 0; i < prog->nsegs; ++i)
+  for (int i = 0; i < prog->nsegs; ++i) {
     if (__builtin_popcount(prog->layout[i]) != 4 || (prog->layout[i] >> kTileBits)) return -1;
+    if (i > 0 && prog->layout[i] == prog->layout[i - 1]) return -1;  // the kernel exchanges at every segment
+  }
   if ((prog->layout[0] & hbm) || (prog->layout[prog->nsegs - 1] & hbm)) return -1;
   return 0;
 }
@@ -372,12 +510,18 @@ int lsim_sv_apply_tile(void* states, int dtype, int nbits, long long batch, cons
   for (int j = 0; j < nops; ++j)
     if (ops[j].kind == LSIM_TILE_DENSE && ops[j].k == 2)
       LSIM_REQUIRE(ops[j].t[0] > ops[j].t[1], "operator %d: a dense 2-bit operator lists its higher tile bit first", j);
+  lsim_tile_op ordered[kMaxTileOps];
+  {
+    int run = 0;
+    while (run < 8 && bits[run] == run) ++run;
+    schedule_ops(nops, ops, (1u << run) - 1u, ordered);
+  }
   TileProgram prog;
-  if (make_program(bits, nops, ops, &prog) != 0)
+  if (make_program(bits, nops, ordered, &prog) != 0)
     return fail(LSIM_EINVAL, "tile pass: the layout plan failed its own checks");
   hipStream_t s = as_stream(stream);
   const uint64_t tiles = 1ull << (nbits - kTileBits);
-  const uint32_t grid = (uint32_t)(tiles < 65536 ? tiles : 65536);
+  const uint32_t grid = (uint32_t)(tiles < kTileGrid ? tiles : kTileGrid);
   const bool nt = dense_streams((uint64_t)batch * (dtype_bytes(dtype) << nbits));
   for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
     const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);

@@ -424,19 +424,20 @@ def test_grouped_pauli_strings_equal_the_oracle(dtype: int, n: int) -> None:
 
 
 @pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
-@pytest.mark.parametrize(("n", "budget"), [(12, 24.0), (13, 6.0), (16, 24.0), (16, 1000.0)])
+@pytest.mark.parametrize(("n", "budget"), [(12, 24.0), (13, 6.0), (16, 24.0), (16, 1000.0), (22, 24.0)])
 def test_tile_passes_equal_the_oracle(dtype: int, n: int, budget: float) -> None:
     """Tile passes (lsim_sv_apply_tile: a tile's amplitudes staged in LDS, a run of 1-2 qubit
     operators applied, written back) on a batch of two random states: every amplitude against the
     oracle applying the circuit's gates one by one (random layers with cx/cz, a wide gate that
-    runs as its own dense pass, a global phase)."""
+    runs as its own dense pass, a global phase). At 22 qubits each workgroup of a launch takes
+    two of the 1024 tiles, the second one prefetched while the first is computed."""
     import lucivant_sim as ls
     from lucivant_sim import workloads as wl
     from lucivant_sim.engines import run_plan
     from lucivant_sim.plan import TilePass, plan_tiles
 
     print(f"tile passes: n={n} budget={budget} dtype={dtype}", flush=True)
-    src = wl.random_layers(n, 6, 17 + n, two_qubit=("cx", "cz"), measure=False)
+    src = wl.random_layers(n, 6 if n < 20 else 3, 17 + n, two_qubit=("cx", "cz"), measure=False)
     src += f"ccx q[0], q[{n // 2}], q[{n - 1}];\nh q[{n - 1}];\ngphase(0.3);\ncx q[{n - 1}], q[2];\n"
     c = ls.parse(src)
     plan = plan_tiles(c.operations, n, budget=budget)
@@ -453,3 +454,38 @@ def test_tile_passes_equal_the_oracle(dtype: int, n: int, budget: float) -> None
             elif isinstance(op, ls.circuit.GlobalPhase):
                 want = want * np.exp(1j * op.angle)
         assert np.abs(got[b] - want).max() < (1e-12 if dtype == nat.C128 else 2e-5)
+
+
+@pytest.mark.parametrize(
+    "n, keep",
+    [
+        (6, [1, 4]),  # pieces of 2 entries
+        (10, [0, 7]),  # qubit 0 kept: entry by entry
+        (12, [3, 11]),  # pieces of 8
+        (14, [7, 8, 13]),  # pieces of 128
+        (16, [9]),  # pieces of 512: a tree over four 128-entry blocks
+        (18, [13, 17]),  # pieces of 8192
+        (20, [16, 19]),  # runs of 2^16 cut into pieces of 8192
+        (22, [0, 1, 2, 3, 4, 21]),
+        (22, list(range(8, 22))),
+    ],
+)
+def test_device_marginal_equals_numpy_bit_for_bit(n: int, keep: list[int]) -> None:
+    """lsim_marginal over two states of float64 probabilities against the reference's marginal
+    (NumPy's tensor.sum over the dropped axes, engines.py:267-275): identical float64 values,
+    for chunk sizes from 1 to 8192 entries and runs longer than NumPy's buffer."""
+    from lucivant_sim.engines import marginal
+
+    print(f"marginal: n={n} keep={keep}", flush=True)
+    rng = np.random.default_rng(n)
+    p = rng.random((2, 1 << n)) ** 3
+    p /= p.sum(axis=1, keepdims=True)
+    probs = dev(p.reshape(-1))
+    karr = nat.int_array(keep)
+    nbytes = nat.load().lsim_marginal_workspace_bytes(n, 2, len(keep), karr)
+    ws = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device="cuda")
+    out = torch.empty(2 << len(keep), dtype=torch.float64, device="cuda")
+    nat.call("lsim_marginal", ptr(probs), n, 2, len(keep), karr, ptr(out), ptr(ws), stream_handle())
+    got = out.cpu().numpy().reshape(2, -1)
+    for b in range(2):
+        assert np.array_equal(got[b], marginal(p[b], n, keep)), b
