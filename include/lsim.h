@@ -175,6 +175,19 @@ int lsim_sv_pauli_expect_noalloc(const void* states, int dtype, int nbits, long 
                                  int nterms, const unsigned long long* xmask,
                                  const unsigned long long* zmask, double* out, void* stream);
 
+/* Bytes of device workspace lsim_marginal needs for these sizes (0 when qubit 0 is kept). */
+size_t lsim_marginal_workspace_bytes(int n, long long batch, int nkeep, const int* keep);
+
+/* out[b][j] = the probability that the `nkeep` qubits `keep` (HOST ints, ascending) read j
+ * (bit i of j = keep[i]), from float64 probabilities probs[b][2^n] on the device: the marginal of
+ * engines.py:267-275, summed over the other qubits in exactly NumPy's order for
+ * probs.reshape([2]*n).sum(axis=dropped) (pieces of at most 8192 contiguous entries, each
+ * pairwise-summed, added in increasing index order), so sampling from it matches the host
+ * marginal bit for bit and only 2^nkeep values leave the device. `workspace` (device) holds
+ * lsim_marginal_workspace_bytes(n, batch, nkeep, keep) bytes; it may be NULL when that is 0. */
+int lsim_marginal(const double* probs, int n, long long batch, int nkeep, const int* keep,
+                  double* out, double* workspace, void* stream);
+
 /* probs[b][i] = max(Re rho_b[i][i], 0) for n-qubit density matrices: DensityMatrix.probabilities
  * (engines.py:257-260). */
 int lsim_dm_probabilities(const void* rho, int dtype, int n, long long batch, double* probs,

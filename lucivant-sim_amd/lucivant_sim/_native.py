@@ -58,6 +58,8 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
         _c_int,
     ),
     "lsim_dm_probabilities": ([_c_void, _c_int, _c_int, _c_ll, _c_void, _c_void], _c_int),
+    "lsim_marginal_workspace_bytes": ([_c_int, _c_ll, _c_int, _c_void], ctypes.c_size_t),
+    "lsim_marginal": ([_c_void, _c_int, _c_ll, _c_int, _c_void, _c_void, _c_void, _c_void], _c_int),
     "lsim_program_max_qubits": ([_c_int], _c_int),
     "lsim_program_run": (
         [_c_int, _c_int, _c_ll, _c_void, _c_void, _c_void, _c_void, _c_void, _c_void, _c_void]

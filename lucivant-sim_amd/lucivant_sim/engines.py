@@ -213,6 +213,24 @@ class _DeviceState:
     def _stream(self) -> int:
         return stream_handle()
 
+    def device_probabilities(self) -> torch.Tensor:
+        raise NotImplementedError
+
+    def probabilities(self) -> NDArray[np.float64]:
+        """The measurement distribution of every basis state, as float64 on the host."""
+        return self.device_probabilities().cpu().numpy()
+
+    def marginal(self, qubits: Sequence[int]) -> NDArray[np.float64]:
+        """Probabilities of ``qubits`` alone (index bit i = the i-th smallest of them): the
+        reference's marginal (engines.py:267-275). When they are not every qubit, the others are
+        summed out on the device in NumPy's own order (bit-identical to the host marginal), and
+        only the 2^m results are copied to the host."""
+        keep = sorted(set(int(q) for q in qubits))
+        probs = self.device_probabilities()
+        if len(keep) == self.n or not device_marginal_pays(self.n, keep):
+            return marginal(probs.cpu().numpy(), self.n, keep)
+        return device_marginal(probs, self.n, keep, self._stream())
+
     def run(self, passes: Sequence[Pass | TilePass]) -> None:
         run_plan(self.buffer, self.dtype, self.nbits, 1, passes)
 
@@ -310,10 +328,10 @@ class Statevector(_DeviceState):
         if self.measure(qubit, rng):
             self.apply(np.array([[0, 1], [1, 0]], dtype=np.complex128), (qubit,))
 
-    def probabilities(self) -> NDArray[np.float64]:
+    def device_probabilities(self) -> torch.Tensor:
         out = torch.empty(1 << self.n, dtype=torch.float64, device=self.buffer.device)
         nat.call("lsim_sv_probabilities", ptr(self.buffer), self.dtype, self.n, 1, ptr(out), self._stream())
-        return out.cpu().numpy()
+        return out
 
     def vector(self) -> NDArray[np.complex128]:
         return self.buffer.cpu().numpy().astype(np.complex128, copy=True)
@@ -397,10 +415,10 @@ class DensityMatrix(_DeviceState):
     def run_kraus(self, channel: Channel, qubits: tuple[int, ...]) -> None:
         self._kraus_sum(channel, tuple(qubits))
 
-    def probabilities(self) -> NDArray[np.float64]:
+    def device_probabilities(self) -> torch.Tensor:
         out = torch.empty(1 << self.n, dtype=torch.float64, device=self.buffer.device)
         nat.call("lsim_dm_probabilities", ptr(self.buffer), self.dtype, self.n, 1, ptr(out), self._stream())
-        return out.cpu().numpy()
+        return out
 
     def matrix(self) -> NDArray[np.complex128]:
         d = 1 << self.n
@@ -443,6 +461,39 @@ def launch_expectations(fn: str, buf: torch.Tensor, dtype: int, n: int, nbits: i
         ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(nbits, 1, len(chunk)))
         nat.call(fn, ptr(buf), dtype, n, 1, len(chunk), xs, zs, ptr(res) + 8 * start, ptr(ws), stream)
     return res[: len(labels)]
+
+
+#: NumPy's iterator buffer (np.getbufsize()): the longest piece of a contiguous run of dropped
+#: entries that NumPy's reduction sums pairwise before adding it to the output
+NUMPY_CHUNK_BITS = 13
+#: log2 of the longest chain of ordered additions per marginal entry left to the device (each
+#: entry adds its pieces one after another, as NumPy does; longer chains stay on the host)
+DEVICE_MARGINAL_CHAIN_BITS = 20
+
+
+def device_marginal_pays(n: int, keep: Sequence[int]) -> bool:
+    """Whether :func:`device_marginal` suits this keep set: fewer than every qubit, and at most
+    2^DEVICE_MARGINAL_CHAIN_BITS ordered additions per entry (each entry adds the pieces of
+    2^min(low, 13) entries that share its kept bits, ``low`` being the number of dropped qubits
+    below the lowest kept one)."""
+    low = 0
+    while low < n and low not in keep:
+        low += 1
+    chain_bits = n - min(low, NUMPY_CHUNK_BITS) - len(keep)
+    return 0 < len(keep) < n and chain_bits <= DEVICE_MARGINAL_CHAIN_BITS
+
+
+def device_marginal(probs: torch.Tensor, n: int, keep: Sequence[int], stream: int) -> NDArray[np.float64]:
+    """The marginal of float64 device probabilities over the sorted qubits ``keep``, summed on
+    the device by lsim_marginal in NumPy's order, so it equals :func:`marginal` of the same
+    values bit for bit; returns the 2^m values on the host."""
+    keep_arr = nat.int_array(list(keep))
+    lib = nat.load()
+    nbytes = lib.lsim_marginal_workspace_bytes(n, 1, len(keep), keep_arr)
+    out = torch.empty(1 << len(keep), dtype=torch.float64, device=probs.device)
+    ws = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=probs.device)
+    nat.call("lsim_marginal", ptr(probs), n, 1, len(keep), keep_arr, ptr(out), ptr(ws), stream)
+    return out.cpu().numpy()
 
 
 def marginal(probabilities: NDArray[np.float64], num_qubits: int, qubits: list[int]) -> NDArray[np.float64]:

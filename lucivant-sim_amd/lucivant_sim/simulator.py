@@ -312,8 +312,28 @@ def _sample(
     if not pairs or shots == 0:
         return {}
     measured = sorted({q for q, _ in pairs})
+    return _sample_marginal(circuit, marginal(probs, circuit.num_qubits, measured), shots, noise, rng)
+
+
+def _sample_state(
+    circuit: Circuit, state: Statevector | DensityMatrix, shots: int, noise: NoiseModel | None, rng: np.random.Generator
+) -> dict[str, int]:
+    """:func:`_sample` of a device state's probabilities, with the marginal over the measured
+    qubits formed on the device when they are not every qubit (the same float64 values)."""
+    pairs = circuit.final_measurements()
+    if not pairs or shots == 0:
+        return {}
+    measured = sorted({q for q, _ in pairs})
+    return _sample_marginal(circuit, state.marginal(measured), shots, noise, rng)
+
+
+def _sample_marginal(
+    circuit: Circuit, dist: NDArray[np.float64], shots: int, noise: NoiseModel | None, rng: np.random.Generator
+) -> dict[str, int]:
+    """The sampling of :func:`_sample` from the measured qubits' marginal ``dist``."""
+    pairs = circuit.final_measurements()
+    measured = sorted({q for q, _ in pairs})
     position = {q: i for i, q in enumerate(measured)}
-    dist = marginal(probs, circuit.num_qubits, measured)
     cumulative = _cumsum(dist)
     draws = rng.random(shots)
     wide = circuit.num_clbits > 62
@@ -488,7 +508,7 @@ def run(
         _evolve(state, circuit, noise)
         if isinstance(state, Statevector):
             vector = state.vector()
-        counts = _sample(circuit, state.probabilities(), shots, noise, rng)
+        counts = _sample_state(circuit, state, shots, noise, rng)
     elapsed = round(time.perf_counter() - started, 6)
     return Result(
         counts=dict(sorted(counts.items())),
@@ -576,7 +596,7 @@ def outcome_probabilities(
         return {}
     state = _exact_state(circuit, noise)
     measured = sorted({q for q, _ in pairs})
-    dist = marginal(state.probabilities(), circuit.num_qubits, measured)
+    dist = state.marginal(measured)
     recorded = _recorded_distribution(dist, measured, pairs, noise, threshold)
     return {circuit.format_bits(bits): w for bits, w in recorded}
 

@@ -346,3 +346,30 @@ def test_c4_20_qubit_complex64_against_the_reference_counts(golden_runs: dict, g
     assert res.precision is not None and res.precision["checked_shots"] == 256
     assert res.precision["shots_differing_from_complex128"] == len(differing)
     assert res.precision["max_abs_dp"] == stats["max_abs_dp"]
+
+
+def test_partial_measurement_counts_use_the_device_marginal_and_equal_the_oracle() -> None:
+    """A 20-qubit circuit measuring 6 of its qubits: the marginal is summed on the device, and
+    counts and outcome probabilities equal the oracle's (host NumPy marginal of the same seeded
+    run); a noisy density-matrix run measuring 3 of 8 qubits likewise."""
+    from lucivant_sim.engines import device_marginal_pays
+
+    n = 20
+    measured = [2, 5, 11, 12, 16, 19]
+    body = wl.random_layers(n, 4, 77, measure=False).replace(f"qubit[{n}] q;", f"qubit[{n}] q;\nbit[6] c;")
+    src = body + "".join(f"c[{i}] = measure q[{q}];\n" for i, q in enumerate(measured))
+    c = ls.parse(src)
+    assert device_marginal_pays(n, measured)
+    res = ls.run(src, 3000, seed=11)
+    assert res.counts == orc.run_counts(c, 3000, 11)
+    want = orc.marginal(orc.probabilities_sv(orc.evolve_sv(c)), n, measured)
+    got = ls.outcome_probabilities(src)
+    assert abs(sum(got.values()) - 1) < 1e-12
+    for i, p in enumerate(want):
+        if p > 1e-12:
+            assert abs(got[c.format_bits(i)] - p) < 1e-12
+    noise = wl.noise_c3()
+    part = wl.config_c3(8, 3).replace("c = measure q;", "c[0] = measure q[3];\nc[1] = measure q[6];\nc[2] = measure q[7];")
+    assert device_marginal_pays(8, [3, 6, 7])
+    r = ls.run(part, 2000, seed=5, noise=noise, method="density_matrix")
+    assert r.counts == orc.run_counts(ls.parse(part), 2000, 5, noise=noise, method="density_matrix")
