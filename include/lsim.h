@@ -129,6 +129,16 @@ int lsim_sv_apply_tile(void* states, int dtype, int nbits, long long batch, cons
 int lsim_sv_apply_diag(void* states, int dtype, int nbits, long long batch, int k,
                        const int* targets, const void* diags, const int* sel, void* stream);
 
+/* A permutation with phases on k <= 5 target bits: row r of the operator has its one non-zero
+ * entry phases[r] in column perm[r] (perm: HOST ints, a permutation of 0..2^k-1; phases: DEVICE,
+ * 2^k complex values in dtype per selector row). Bit r of unit_rows says phases[r] is exactly 1
+ * (a plain move, no multiply). sel (DEVICE, optional) picks each state's phase row, or skips the
+ * state when negative. The reference's slice moves for x, y, cx, swap and their phased forms
+ * (engines.py:91-104): one read and one write of every amplitude, no matrix arithmetic. */
+int lsim_sv_apply_perm(void* states, int dtype, int nbits, long long batch, int k,
+                       const int* targets, const int* perm, unsigned int unit_rows,
+                       const void* phases, const int* sel, void* stream);
+
 /* psi <- c psi in every state: Statevector.scale (engines.py:156-157). */
 int lsim_sv_scale(void* states, int dtype, int nbits, long long batch, double re, double im,
                   void* stream);
@@ -248,6 +258,23 @@ int lsim_program_run(int dtype, int n, long long instances, const lsim_prog_op* 
                      unsigned long long* clbits, double* probs, void* state, int nterms,
                      const unsigned long long* pauli_x, const unsigned long long* pauli_z,
                      double* expect, double* decisions, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Library-owned HIP graphs (beyond the reference: its engines have no launch overhead to cut).
+ * A caller records a sequence of calls to this library on one stream and replays it with one
+ * launch: lsim_graph_capture_begin(stream); the calls (each only enqueues kernels on `stream`);
+ * lsim_graph_capture_end(stream, &g); then lsim_graph_launch(g, stream) as often as wanted.
+ * Capture is thread-local: other threads may use the GPU meanwhile. The graph keeps the
+ * pointers and arguments of the recorded calls; what they point to may change between replays.
+ * `stream` must not be the null stream. No call during capture may allocate or synchronise.
+ */
+typedef struct lsim_graph lsim_graph;
+int lsim_graph_capture_begin(void* stream);
+int lsim_graph_capture_end(void* stream, lsim_graph** out);
+/* Kernel nodes the graph holds (-1 for NULL). */
+long long lsim_graph_nodes(const lsim_graph* g);
+int lsim_graph_launch(lsim_graph* g, void* stream);
+int lsim_graph_destroy(lsim_graph* g);
 
 #ifdef __cplusplus
 }

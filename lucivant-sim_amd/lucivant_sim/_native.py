@@ -58,6 +58,12 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
         _c_int,
     ),
     "lsim_dm_probabilities": ([_c_void, _c_int, _c_int, _c_ll, _c_void, _c_void], _c_int),
+    "lsim_sv_apply_perm": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_int_p, ctypes.c_uint, _c_void, _c_void, _c_void], _c_int),
+    "lsim_graph_capture_begin": ([_c_void], _c_int),
+    "lsim_graph_capture_end": ([_c_void, ctypes.POINTER(ctypes.c_void_p)], _c_int),
+    "lsim_graph_nodes": ([_c_void], _c_ll),
+    "lsim_graph_launch": ([_c_void, _c_void], _c_int),
+    "lsim_graph_destroy": ([_c_void], _c_int),
     "lsim_marginal_workspace_bytes": ([_c_int, _c_ll, _c_int, _c_void], ctypes.c_size_t),
     "lsim_marginal": ([_c_void, _c_int, _c_ll, _c_int, _c_void, _c_void, _c_void, _c_void], _c_int),
     "lsim_program_max_qubits": ([_c_int], _c_int),

@@ -98,16 +98,14 @@ def run_passes(buf: torch.Tensor, dtype: int, nbits: int, batch: int, passes: Se
     """Launch ``passes`` in order on ``batch`` states of ``nbits`` bits held in ``buf``."""
     if not passes:
         return
-    dev, addrs = upload([p.matrix for p in passes], dtype)
+    dev, addrs = upload([p.payload() for p in passes], dtype)
     stream = stream_handle()
     state = ptr(buf)
+    lib = nat.load()
     for p, addr in zip(passes, addrs):
-        if p.kind == "scale":
-            z = complex(p.matrix[0])
-            nat.call("lsim_sv_scale", state, dtype, nbits, batch, z.real, z.imag, stream)
-            continue
-        fn = "lsim_sv_apply_dense" if p.kind == "dense" else "lsim_sv_apply_diag"
-        nat.call(fn, state, dtype, nbits, batch, p.k, nat.int_array(p.bits), addr, None, stream)
+        fn, args = pass_launch(lib, p, state, dtype, nbits, batch, addr, stream)
+        if fn(*args):
+            raise nat.NativeError(f"{fn.__name__}: {lib.lsim_last_error().decode(errors='replace')}")
     # keep the matrices alive until the launches that read them have run
     dev.record_stream(torch.cuda.current_stream())
 
@@ -126,7 +124,7 @@ def plan_launches(buf: torch.Tensor, dtype: int, nbits: int, batch: int, plan: S
     tiles = [p for p in plan if isinstance(p, TilePass)]
     addrs: list[int] = []
     if flat:
-        dev, addrs = upload([p.matrix for p in flat], dtype)
+        dev, addrs = upload([p.payload() for p in flat], dtype)
         keep.append(dev)
     tile_tables = []
     if tiles:
@@ -155,15 +153,27 @@ def plan_launches(buf: torch.Tensor, dtype: int, nbits: int, batch: int, plan: S
             continue
         addr = addrs[fi]
         fi += 1
-        if p.kind == "scale":
-            z = complex(p.matrix[0])
-            calls.append((lib.lsim_sv_scale, (state, dtype, nbits, batch, z.real, z.imag, stream)))
-        else:
-            fn = lib.lsim_sv_apply_dense if p.kind == "dense" else lib.lsim_sv_apply_diag
-            targets = nat.int_array(p.bits)
-            keep.append(targets)
-            calls.append((fn, (state, dtype, nbits, batch, p.k, targets, addr, None, stream)))
+        fn, args = pass_launch(lib, p, state, dtype, nbits, batch, addr, stream)
+        keep.append(args)
+        calls.append((fn, args))
     return keep, calls
+
+
+def pass_launch(lib, p: Pass, state: int, dtype: int, nbits: int, batch: int, addr: int, stream: int) -> tuple:  # type: ignore[no-untyped-def]
+    """The entry point and arguments of one pass over ``batch`` states (``addr``: the device
+    copy of ``p.payload()``): a scale, a diagonal, a permutation with phases, or a dense
+    operator. The host arrays among the arguments must stay alive until the call."""
+    if p.kind == "scale":
+        z = complex(p.matrix[0])
+        return lib.lsim_sv_scale, (state, dtype, nbits, batch, z.real, z.imag, stream)
+    targets = nat.int_array(p.bits)
+    if p.kind == "diag":
+        return lib.lsim_sv_apply_diag, (state, dtype, nbits, batch, p.k, targets, addr, None, stream)
+    mv = p.moves()
+    if mv is not None:
+        src, _entries, unit = mv
+        return lib.lsim_sv_apply_perm, (state, dtype, nbits, batch, p.k, targets, nat.int_array(src), unit, addr, None, stream)
+    return lib.lsim_sv_apply_dense, (state, dtype, nbits, batch, p.k, targets, addr, None, stream)
 
 
 def launch_all(calls: list[tuple]) -> None:

@@ -1,10 +1,11 @@
 """Compiled circuits: plan once, upload once, replay as one HIP graph.
 
 A :class:`CompiledCircuit` holds a statevector in HBM, the fused passes of a circuit, their
-matrices in one device buffer, and a HIP graph (captured through ``torch.cuda.CUDAGraph``, which
-is HIP's graph API on ROCm) of the whole sequence: state initialisation then every pass. A
-replay runs the simulation with one launch from the host, so mid-size states whose passes last a
-few microseconds are no longer bound by per-launch host overhead.
+matrices in one device buffer, and a HIP graph of the whole sequence -- state initialisation
+then every pass -- captured and owned by the library (``lsim_graph_capture_begin`` /
+``lsim_graph_capture_end`` / ``lsim_graph_launch`` in include/lsim.h). A replay runs the
+simulation with one launch from the host, so mid-size states whose passes last a few
+microseconds are no longer bound by per-launch host overhead.
 
 :meth:`CompiledCircuit.update` swaps in a circuit of the same structure (the same gates on the
 same qubits, other angles: a variational sweep) by recomputing the fused matrices on the host and
@@ -15,12 +16,14 @@ is initialised by a kernel).
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
 from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Circuit, Gate
-from lucivant_sim.engines import _NP_DTYPE, Statevector, ptr
+from lucivant_sim.engines import _NP_DTYPE, Statevector, pass_launch, ptr
 from lucivant_sim.plan import plan_statevector
 
 
@@ -37,15 +40,15 @@ class CompiledCircuit:
         self.dtype = dtype
         self.state = Statevector(circuit.num_qubits, dtype=dtype)
         self.passes = plan_statevector(circuit.operations, circuit.num_qubits, width)
-        self._layout = [(p.kind, p.bits, p.matrix.size) for p in self.passes]
+        self._layout = [(p.kind, p.bits, p.payload().size, p.moves() and p.moves()[0]) for p in self.passes]
         host = self._pack(self.passes)
         self.mats = torch.from_numpy(host).to(self.state.buffer.device)
-        self.graph: torch.cuda.CUDAGraph | None = None
+        self.graph: int | None = None  # an lsim_graph handle
         self._capture()
 
     def _pack(self, passes: list) -> np.ndarray:
         npdt = _NP_DTYPE[self.dtype]
-        flat = [np.ascontiguousarray(p.matrix, dtype=npdt).reshape(-1) for p in passes]
+        flat = [np.ascontiguousarray(p.payload(), dtype=npdt).reshape(-1) for p in passes]
         return np.concatenate(flat) if flat else np.zeros(1, dtype=npdt)
 
     def _launch_all(self, stream: int) -> None:
@@ -60,33 +63,44 @@ class CompiledCircuit:
         off = 0
         for p in self.passes:
             addr = base + off * item
-            off += p.matrix.size
-            if p.kind == "scale":
-                z = complex(p.matrix[0])
-                rc = lib.lsim_sv_scale(state, self.dtype, n, 1, z.real, z.imag, stream)
-            else:
-                fn = lib.lsim_sv_apply_dense if p.kind == "dense" else lib.lsim_sv_apply_diag
-                rc = fn(state, self.dtype, n, 1, p.k, nat.int_array(p.bits), addr, None, stream)
-            if rc:
+            off += p.payload().size
+            fn, args = pass_launch(lib, p, state, self.dtype, n, 1, addr, stream)
+            if fn(*args):
                 raise nat.NativeError(lib.lsim_last_error().decode())
 
     def _capture(self) -> None:
+        # the launches run once eagerly first (module loading stays out of the graph), then are
+        # recorded on a stream of their own
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            self._launch_all(side.cuda_stream)  # warm-up outside the graph (module loads)
-        torch.cuda.current_stream().wait_stream(side)
+            self._launch_all(side.cuda_stream)
         torch.cuda.synchronize()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self._launch_all(torch.cuda.current_stream().cuda_stream)
+        handle = ctypes.c_void_p()
+        nat.call("lsim_graph_capture_begin", side.cuda_stream)
+        try:
+            self._launch_all(side.cuda_stream)
+        finally:
+            nat.call("lsim_graph_capture_end", side.cuda_stream, ctypes.byref(handle))
+        self.graph = handle.value
+        self.nodes = int(nat.load().lsim_graph_nodes(self.graph))
         torch.cuda.synchronize()
 
     def replay(self) -> Statevector:
-        """Run the whole simulation again (one graph launch); returns the state object."""
+        """Run the whole simulation again (one graph launch on the current stream); returns the
+        state object."""
         assert self.graph is not None
-        self.graph.replay()
+        nat.call("lsim_graph_launch", self.graph, torch.cuda.current_stream().cuda_stream)
         return self.state
+
+    def __del__(self) -> None:
+        graph, self.graph = getattr(self, "graph", None), None
+        if graph is not None:
+            try:
+                torch.cuda.synchronize()
+                nat.load().lsim_graph_destroy(graph)
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
 
     def run_eager(self) -> Statevector:
         """The same launches without the graph (for comparison)."""
@@ -98,7 +112,7 @@ class CompiledCircuit:
         if _structure(circuit) != _structure(self.circuit):
             raise ValueError("update needs a circuit with the same gates on the same qubits")
         passes = plan_statevector(circuit.operations, circuit.num_qubits, self.width)
-        layout = [(p.kind, p.bits, p.matrix.size) for p in passes]
+        layout = [(p.kind, p.bits, p.payload().size, p.moves() and p.moves()[0]) for p in passes]
         if layout != self._layout:
             # a diagonal block may turn dense for other angles: the graph would not match
             raise ValueError("the new angles change the pass structure; compile the circuit anew")

@@ -78,6 +78,35 @@ class Pass:
     def k(self) -> int:
         return len(self.bits)
 
+    def moves(self) -> tuple[tuple[int, ...], np.ndarray, int] | None:
+        """For a dense pass whose matrix has one non-zero entry per row and column (a
+        permutation with phases, the reference's slice moves): the source column of each row,
+        each row's entry, and the mask of rows whose entry is exactly 1; else None. Such a pass
+        runs as lsim_sv_apply_perm. Computed once per pass."""
+        if self.kind != "dense":
+            return None
+        cached = self.__dict__.get("_moves", _UNSET)
+        if cached is _UNSET:
+            m = np.asarray(self.matrix)
+            nz = m != 0
+            cached = None
+            if np.all(nz.sum(axis=1) == 1) and np.all(nz.sum(axis=0) == 1):
+                src = nz.argmax(axis=1)
+                entries = np.ascontiguousarray(m[np.arange(len(src)), src], dtype=np.complex128)
+                unit = sum(1 << r for r, e in enumerate(entries) if e == 1)
+                cached = (tuple(int(c) for c in src), entries, unit)
+            self.__dict__["_moves"] = cached
+        return cached
+
+    def payload(self) -> np.ndarray:
+        """What the pass's launch reads from device memory: the matrix, the diagonal, the scale
+        factor, or a permutation's row entries."""
+        mv = self.moves()
+        return self.matrix if mv is None else mv[1]
+
+
+_UNSET = object()
+
 
 @dataclass
 class _Block:

@@ -373,3 +373,21 @@ def test_partial_measurement_counts_use_the_device_marginal_and_equal_the_oracle
     assert device_marginal_pays(8, [3, 6, 7])
     r = ls.run(part, 2000, seed=5, noise=noise, method="density_matrix")
     assert r.counts == orc.run_counts(ls.parse(part), 2000, 5, noise=noise, method="density_matrix")
+
+
+def test_permutation_layers_run_as_moves_and_equal_the_oracle() -> None:
+    """A circuit whose fused blocks include pure permutations (layers of x, cx, swap and ccx,
+    with s and t phases between them): statevector and density-matrix results equal the oracle's
+    within 1e-10, and the plan does contain permutation passes."""
+    from lucivant_sim.plan import plan_statevector
+
+    src = 'OPENQASM 3.0;\ninclude "stdgates.inc";\nqubit[9] q;\n'
+    for layer in range(4):
+        src += f"h q[{layer}];\nx q[{(layer + 2) % 9}];\ncx q[{layer}], q[{(layer + 3) % 9}];\nswap q[{(layer + 4) % 9}], q[{(layer + 5) % 9}];\n"
+        src += f"ccx q[{(layer + 6) % 9}], q[{(layer + 7) % 9}], q[{(layer + 8) % 9}];\ns q[{(layer + 1) % 9}];\nt q[{(layer + 4) % 9}];\n"
+    c = ls.parse(src)
+    passes = plan_statevector(c.operations, 9, 3)
+    assert any(p.moves() is not None for p in passes)
+    assert np.abs(ls.statevector(src) - orc.evolve_sv(c)).max() < 1e-10
+    noise = ls.NoiseModel.uniform(single=0.01, two=0.02)
+    assert np.abs(ls.density_matrix(src, noise) - orc.evolve_dm(c, noise)).max() < 1e-10

@@ -489,3 +489,34 @@ def test_device_marginal_equals_numpy_bit_for_bit(n: int, keep: list[int]) -> No
     got = out.cpu().numpy().reshape(2, -1)
     for b in range(2):
         assert np.array_equal(got[b], marginal(p[b], n, keep)), b
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_permutations_with_phases_equal_the_oracle(k: int, dtype: int) -> None:
+    """lsim_sv_apply_perm on a batch of three states: a random permutation of 2^k rows, random
+    phases except for the rows flagged as plain moves, targets in a scrambled order (one of them
+    the lowest bit); the third state's selector skips it. Against the oracle applying the same
+    matrix as a dense operator."""
+    n = 11
+    rng = np.random.default_rng(40 + k)
+    d = 1 << k
+    perm = rng.permutation(d)
+    unit = rng.random(d) < 0.5
+    phases = np.where(unit, 1.0 + 0j, np.exp(1j * rng.uniform(0, 2 * np.pi, d)))
+    mask = sum(1 << r for r in range(d) if unit[r])
+    m = np.zeros((d, d), dtype=np.complex128)
+    m[np.arange(d), perm] = phases
+    targets = tuple(int(x) for x in rng.permutation([0, 3, 5, 8, 10][:k]))
+    psi = states(n, 3, 7 + k)
+    s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+    ph = dev(np.stack([phases, np.conj(phases)]).astype(NPDT[dtype]))
+    sel = dev(np.array([0, 1, -1], dtype=np.int32))
+    nat.call("lsim_sv_apply_perm", ptr(s), dtype, n, 3, k, nat.int_array(targets), nat.int_array([int(x) for x in perm]), mask, ptr(ph), ptr(sel), stream_handle())
+    got = s.cpu().numpy().reshape(3, -1)
+    m_conj = np.zeros_like(m)
+    m_conj[np.arange(d), perm] = np.where(unit, 1.0 + 0j, np.conj(phases))
+    for b, mat in ((0, m), (1, m_conj)):
+        want = orc.apply_sv(psi[b].astype(NPDT[dtype]).astype(np.complex128), mat, targets, n)
+        assert np.abs(got[b] - want).max() < TOL[dtype], b
+    assert np.array_equal(got[2], psi[2].astype(NPDT[dtype]))

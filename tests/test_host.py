@@ -654,3 +654,25 @@ def test_tile_passes_equal_sequential_gates(budget: float) -> None:
             gates += p.gates
     assert np.abs(psi - want).max() < 1e-12
     assert gates == sum(1 for op in c.operations if isinstance(op, ls.circuit.Gate))
+
+
+def test_permutation_passes_are_recognised() -> None:
+    """Dense passes whose matrix has one non-zero per row and column run as permutations with
+    phases (lsim_sv_apply_perm); anything else stays dense."""
+    import numpy as np
+
+    from lucivant_sim.plan import Pass
+
+    cx = np.eye(4, dtype=np.complex128)[[0, 1, 3, 2]]
+    mv = Pass("dense", (1, 0), cx).moves()
+    assert mv is not None and mv[0] == (0, 1, 3, 2) and mv[2] == 0b1111
+    y = np.array([[0, -1j], [1j, 0]])
+    src, entries, unit = Pass("dense", (3,), y).moves()
+    assert src == (1, 0) and unit == 0 and np.allclose(entries, [-1j, 1j])
+    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    assert Pass("dense", (0,), h).moves() is None
+    assert Pass("dense", (0,), np.array([[1, 0], [0, 0]], dtype=np.complex128)).moves() is None
+    assert Pass("diag", (0,), np.array([1, 1j])).moves() is None
+    hp = Pass("dense", (0,), h)
+    assert hp.payload() is hp.matrix
+    assert np.array_equal(Pass("dense", (1, 0), cx).payload(), np.ones(4))
