@@ -34,6 +34,12 @@ sys.path.insert(0, str(ROOT / "lucivant-sim_amd"))
 sys.path.insert(0, str(ROOT))
 
 METRIC = "26q gate-apply HBM GB/s; batched circuits/s + p99 latency vs plain, 1 GPU"
+_T0 = time.perf_counter()
+
+
+def progress(what: str) -> None:
+    """One line on stderr per finished piece of work (the JSON line stays alone on stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {what}", file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 N = 26
 BYTES_PER_PASS = 2 * 16 * (1 << N)
@@ -274,52 +280,99 @@ def bench_batched_shots(shots: int = 20000, plain_shots: int = 40) -> dict:
     return out
 
 
-def bench_serving(qps: float, count: int = 48, seed: int = 1005) -> dict:
-    """The serving stream: Poisson arrivals at ``qps``; widths uniform in 10..26, depths in
-    10..100; half hardware-efficient ansatz expectation requests (16-term Pauli sums, weights in
-    [-1, 1]), half random u3 + cx count requests (1000 shots); count requests of at most 12
-    qubits carry the uniform noise model (1e-3 / 1e-2) and run as trajectories (wider noisy
-    requests would be 1000 full simulations each, so noise is capped there). The plain stream
-    serves the same requests one at a time with one kernel launch per gate at every width
-    (``serving.plain_stream``)."""
+#: arrival rates of the serving stream (requests per second; 1000 = all at once, saturation)
+SERVING_QPS = (10.0, 20.0, 30.0, 40.0, 1000.0)
+#: requests per rate
+SERVING_REQUESTS = 500
+#: requests of the saturation subset that the single-request baselines also serve
+BASELINE_REQUESTS = 64
+#: widest count request that carries the noise model (run as trajectories)
+NOISY_MAX_QUBITS = 14
+
+
+def serving_requests(count: int, seed: int):  # type: ignore[no-untyped-def]
+    """The serving mix: widths uniform in 10..26, depths in 10..100; even requests are
+    hardware-efficient ansatz expectations (16-term Pauli sums, weights uniform in [-1, 1]), odd
+    ones random u3 + cx circuits asking 1000 seeded shots, with the uniform noise model
+    (depolarising 1e-3 / 1e-2, as trajectories) up to NOISY_MAX_QUBITS qubits."""
     import numpy as np
 
     from lucivant_sim import workloads
     from lucivant_sim.noise import NoiseModel
-    from lucivant_sim.serving import Request, Server, plain_stream, serve_stream
+    from lucivant_sim.serving import Request
 
     rng = np.random.default_rng(seed)
     noise = NoiseModel.uniform(single=1e-3, two=1e-2)
+    reqs = []
+    for i in range(count):
+        n = int(rng.integers(10, 27))
+        depth = int(rng.integers(10, 101))
+        if i % 2 == 0:
+            src = workloads.hea_ansatz(n, depth, 7000 + i)
+            obs = {lbl: float(w) for lbl, w in zip(workloads.pauli_strings(n, 16, 8000 + i), rng.uniform(-1, 1, 16))}
+            reqs.append(Request(src, observable=obs))
+        else:
+            src = workloads.random_layers(n, depth, 9000 + i)
+            noisy = n <= NOISY_MAX_QUBITS
+            reqs.append(Request(src, shots=1000, seed=i, noise=noise if noisy else None, method="trajectories" if noisy else "automatic"))
+    return reqs
 
-    def make():  # type: ignore[no-untyped-def]
-        reqs = []
-        for i in range(count):
-            n = int(rng.integers(10, 27))
-            depth = int(rng.integers(10, 101))
-            if i % 2 == 0:
-                src = workloads.hea_ansatz(n, depth, 7000 + i)
-                obs = {lbl: float(w) for lbl, w in zip(workloads.pauli_strings(n, 16, 8000 + i), rng.uniform(-1, 1, 16))}
-                reqs.append(Request(src, observable=obs))
-            else:
-                src = workloads.random_layers(n, depth, 9000 + i)
-                noisy = n <= 12
-                reqs.append(Request(src, shots=1000, seed=i, noise=noise if noisy else None, method="trajectories" if noisy else "automatic"))
-        return reqs
 
-    reqs = make()
-    arrivals = list(np.cumsum(rng.exponential(1.0 / qps, size=count)))
-    served = serve_stream(Server(), reqs, arrivals)
-    plain_reqs = [Request(r.program, observable=r.observable, shots=r.shots, seed=r.seed, noise=r.noise, method=r.method) for r in reqs]
-    plain = plain_stream(plain_reqs, arrivals)
-    agree = all(
-        (abs(a.result - b.result) < 1e-9) if a.observable is not None else a.result == b.result for a, b in zip(reqs, plain_reqs)
-    )
+def bench_serving(seed: int = 1005) -> dict:
+    """The serving stream at each rate of SERVING_QPS: SERVING_REQUESTS requests of the mix
+    (serving_requests) with Poisson arrivals, through the continuous-batching server. Then the
+    first BASELINE_REQUESTS requests, all arriving at once, through the server and through the
+    single-request API one at a time, both unfused (one launch per gate) and fused (the
+    library's plans): circuits/s and latency of each, and the speedups of serving over both. The
+    three must return the same results (counts equal, expectations within 1e-9)."""
+    import numpy as np
+
+    from lucivant_sim.serving import Request, Server, plain_stream, serve_stream
+
+    points = []
+    for qps in SERVING_QPS:
+        reqs = serving_requests(SERVING_REQUESTS, seed)
+        arrivals = list(np.cumsum(np.random.default_rng(seed + int(qps)).exponential(1.0 / qps, size=len(reqs))))
+        stats = serve_stream(Server(), reqs, arrivals)
+        errors = sum(1 for r in reqs if r.error is not None)
+        progress(f"serving at {qps:g} requests/s: {stats.get('circuits_per_s', 0):.1f} circuits/s, p99 {stats.get('p99_ms', 0):.0f} ms")
+        points.append({"qps": qps, "requests": len(reqs), "errors": errors, **{k: round(v, 3) if isinstance(v, float) else v for k, v in stats.items()}})
+
+    def fresh(reqs):  # type: ignore[no-untyped-def]
+        return [Request(r.program, observable=r.observable, shots=r.shots, seed=r.seed, noise=r.noise, method=r.method) for r in reqs]
+
+    base = serving_requests(BASELINE_REQUESTS, seed)
+    at_once = [0.0] * len(base)
+    served = fresh(base)
+    served_stats = serve_stream(Server(), served, at_once)
+    progress("serving subset: served")
+    unfused = fresh(base)
+    unfused_stats = plain_stream(unfused, at_once, fused=False)
+    progress("serving subset: single requests, unfused")
+    fused = fresh(base)
+    fused_stats = plain_stream(fused, at_once, fused=True)
+    progress("serving subset: single requests, fused")
+
+    def same(a, b):  # type: ignore[no-untyped-def]
+        return all((abs(x.result - y.result) < 1e-9) if x.observable is not None else x.result == y.result for x, y in zip(a, b))
+
+    def rnd(d):  # type: ignore[no-untyped-def]
+        return {k: round(v, 3) if isinstance(v, float) else v for k, v in d.items()}
+
     return {
-        "qps": qps,
-        "requests": count,
-        "served": {k: round(v, 3) if isinstance(v, float) else v for k, v in served.items()},
-        "plain": {k: round(v, 3) if isinstance(v, float) else v for k, v in plain.items()},
-        "results_equal_plain": agree,
+        "workload": f"{SERVING_REQUESTS} requests per rate: widths 10-26, depths 10-100; half 16-term Pauli-sum "
+        f"expectations of HEA circuits, half 1000-shot counts of random u3 + cx circuits (noisy trajectories up to "
+        f"{NOISY_MAX_QUBITS} qubits); latency from arrival",
+        "points": points,
+        "saturation_subset": {
+            "requests": len(base),
+            "served": rnd(served_stats),
+            "single_request_unfused": rnd(unfused_stats),
+            "single_request_fused": rnd(fused_stats),
+            "speedup_vs_unfused": round(served_stats["circuits_per_s"] / unfused_stats["circuits_per_s"], 2),
+            "speedup_vs_fused": round(served_stats["circuits_per_s"] / fused_stats["circuits_per_s"], 2),
+            "results_equal": same(served, unfused) and same(served, fused),
+        },
     }
 
 
@@ -811,6 +864,7 @@ def main() -> int:
         from lucivant_sim import autotune
 
         tuned = autotune.tune(circuit, store=False)
+        progress("autotuner done")
         width = tuned["width"]
         nat.load().lsim_set_dense_tiling(tuned["tiled"])
         nat.load().lsim_set_dense_mfma(tuned["mfma"])
@@ -883,10 +937,23 @@ def main() -> int:
         lib.lsim_sv_scale(state, nat.C128, N, 1, 1.0, 0.0, hs)
         b.record(stream)
         copies.append((a, b))
+    # an out-of-place copy of the state into a second 1 GiB buffer (torch's vectorised copy
+    # kernel): the same read + write bytes with distinct source and destination
+    other = torch.empty_like(sv.buffer)
+    outs = []
+    for _ in range(8):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        other.copy_(sv.buffer)
+        b.record(stream)
+        outs.append((a, b))
     torch.cuda.synchronize()
+    del other
     for k, a, b in all_dense:
         durations[k].append(a.elapsed_time(b) * 1e-3)
     copy_avg = float(np.mean([a.elapsed_time(b) * 1e-3 for a, b in copies]))
+    out_of_place_avg = float(np.mean([a.elapsed_time(b) * 1e-3 for a, b in outs[1:]]))
     dense_times = [t for ts in durations.values() for t in ts]
     avg = float(np.mean(dense_times))
     achieved = BYTES_PER_PASS / avg / 1e9
@@ -938,26 +1005,39 @@ def main() -> int:
             "per_k_avg_ms": {str(k): round(float(np.mean(v)) * 1e3, 4) for k, v in durations.items() if v},
             "copy_kernel_GBps": round(BYTES_PER_PASS / copy_avg / 1e9, 1),
             "frac_of_copy": round(copy_avg / avg, 4),
+            "out_of_place_copy_GBps": round(BYTES_PER_PASS / out_of_place_avg / 1e9, 1),
+            "frac_of_out_of_place_copy": round(out_of_place_avg / avg, 4),
         },
     }
     del sv
     torch.cuda.empty_cache()
+    if pauli is not None:
+        # one Pauli-Z string is a read-only reduction of the state: the read ceiling
+        out["roofline"]["read_only_GBps"] = pauli["read_only_GBps"]
+    progress(f"headline: {ms_per_step:.2f} ms per step, {value:.0f} GB/s")
     if not args.no_extras:
-        out["pauli_26q"] = pauli
-        out["tile_plan"] = bench_tile_plan(circuit, min(args.steps, 10), ms_per_step)
-        out["batched"] = bench_batched()
-        out["batched_shots"] = bench_batched_shots()
-        out["serving"] = [bench_serving(q) for q in (5.0, 20.0, 1000.0)]
-        out["outputs_26q"] = bench_c2_outputs()
-        out["graph"] = bench_graph()
-        out["trajectories_20q"] = bench_trajectories()
-        out["density_matrix_13q"] = bench_density_matrix()
+        legs = [
+            ("pauli_26q", lambda: pauli),
+            ("tile_plan", lambda: bench_tile_plan(circuit, min(args.steps, 10), ms_per_step)),
+            ("batched", bench_batched),
+            ("batched_shots", bench_batched_shots),
+            ("serving", bench_serving),
+            ("outputs_26q", bench_c2_outputs),
+            ("graph", bench_graph),
+            ("trajectories_20q", bench_trajectories),
+            ("density_matrix_13q", bench_density_matrix),
+        ]
+        for name, leg in legs:
+            out[name] = leg()
+            progress(f"{name} done")
     if not args.no_extras:
         gpu_s = bench_reference_configs()
         gpu_s["c3"] = out["density_matrix_13q"]["run_4000_shots_seconds"]
         out["reference_runs"] = reference_runs(gpu_s)
+        progress("reference_runs done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(circuit)
+        progress("cpu_baseline done")
     print(json.dumps(out), flush=True)
     if not args.no_extras:
         from lucivant_sim import frontend

@@ -507,15 +507,19 @@ def serve_stream(server: Server, requests: list[Request], arrivals: list[float])
     return server.stats()
 
 
-def plain_stream(requests: list[Request], arrivals: list[float]) -> dict[str, float]:
-    """The same stream through the plain path: one request at a time, one launch per gate
-    (``lucivant_sim.run`` / ``expectation`` with fusion width 1 at every size, whatever the
-    autotuner stored), no batching or overlap."""
+def plain_stream(requests: list[Request], arrivals: list[float], *, fused: bool = False) -> dict[str, float]:
+    """The same stream through the single-request API, one request at a time with no batching
+    or overlap: ``lucivant_sim.run`` / ``expectation``. ``fused=False`` launches one kernel per
+    gate at every size (fusion width 1, whatever the autotuner stored; runs of one-qubit gates on
+    a qubit are still multiplied into one operator, as the reference's own evolution does);
+    ``fused=True`` keeps the library's fused plans (the autotuner's choice per size)."""
     from lucivant_sim import simulator
 
-    old, old_tuned = simulator.FUSION_WIDTH, simulator._tuned
-    simulator.FUSION_WIDTH = 1
-    simulator._tuned = lambda num_qubits: (1, 0, 0)
+    old, old_tuned, old_tile = simulator.FUSION_WIDTH, simulator._tuned, simulator._tile_choice
+    if not fused:
+        simulator.FUSION_WIDTH = 1
+        simulator._tuned = lambda num_qubits: (1, 0, 0)
+        simulator._tile_choice = lambda num_qubits: None
     t0 = time.perf_counter()
     lat = []
     try:
@@ -532,7 +536,7 @@ def plain_stream(requests: list[Request], arrivals: list[float]) -> dict[str, fl
             req.done = time.perf_counter()
             lat.append(req.done - arrival)
     finally:
-        simulator.FUSION_WIDTH, simulator._tuned = old, old_tuned
+        simulator.FUSION_WIDTH, simulator._tuned, simulator._tile_choice = old, old_tuned, old_tile
     arr = np.array(lat)
     span = time.perf_counter() - t0
     return {
