@@ -721,7 +721,7 @@ def bench_tile_plan(circuit, steps: int, dense_ms: float, budget: float = 48.0, 
     }
 
 
-PROFILE_LEGS = ("dense", "tile", "pauli", "density", "trajectories", "program")
+PROFILE_LEGS = ("dense", "tile", "pauli", "density", "trajectories", "program", "marginal")
 
 
 def profile_leg(leg: str, circuit, labels, steps: int) -> None:  # type: ignore[no-untyped-def]
@@ -732,7 +732,9 @@ def profile_leg(leg: str, circuit, labels, steps: int) -> None:  # type: ignore[
     pauli = the 8 Pauli strings and one Z string on the resident 26-qubit state;
     density = the 13-qubit density-matrix probabilities (superoperator passes over 4^13 entries);
     trajectories = 1024 shots of the 20-qubit teleport chain with noise in one batch;
-    program = 1024 distinct 12-qubit circuits in one LDS-program launch."""
+    program = 1024 distinct 12-qubit circuits in one LDS-program launch;
+    marginal = device marginals of the 26-qubit configuration's probabilities (2^26 float64)
+    over its top 10 qubits (pieces of 8192 entries) and its lowest 10 (chains of 2^16)."""
     import numpy as np
     import torch
 
@@ -771,6 +773,20 @@ def profile_leg(leg: str, circuit, labels, steps: int) -> None:  # type: ignore[
         dm = Statevector(26)
         assert lib.lsim_sv_scale(dm.buffer.data_ptr(), nat.C128, 26, 1, 1.0, 0.0, hs) == 0
         what = f"{steps} density-matrix probability runs + 1 k_scale"
+    elif leg == "marginal":
+        sv = Statevector(N)
+        state = sv.buffer.data_ptr()
+        run_plan(sv.buffer, nat.C128, N, 1, plan_statevector(circuit.operations, N, 5))
+        probs = sv.device_probabilities()
+        out = torch.empty(1 << 10, dtype=torch.float64, device=probs.device)
+        ws = torch.empty(1 << 13, dtype=torch.float64, device=probs.device)
+        for _ in range(steps):
+            for keep in (list(range(16, 26)), list(range(10))):
+                karr = nat.int_array(keep)
+                assert lib.lsim_marginal_workspace_bytes(N, 1, len(keep), karr) <= ws.numel() * 8
+                assert lib.lsim_marginal(probs.data_ptr(), N, 1, len(keep), karr, out.data_ptr(), ws.data_ptr(), hs) == 0
+            assert lib.lsim_sv_scale(state, nat.C128, N, 1, 1.0, 0.0, hs) == 0
+        what = f"{steps} x (marginal of the top 10 qubits, marginal of the lowest 10, 1 k_scale)"
     elif leg == "trajectories":
         from lucivant_sim.trajectories import run_batched
 

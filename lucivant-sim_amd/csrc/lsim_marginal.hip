@@ -27,6 +27,10 @@ namespace lsim {
 
 constexpr int kChunkBits = 13;  // log2 of NumPy's iterator buffer (8192 entries)
 
+// Operands of an ordered chain loaded ahead of their additions (per is a power of two, so a
+// chain of at least kBatch entries is a whole number of batches).
+constexpr int kBatch = 32;
+
 // Chunk-index bit of each kept qubit, in the order of the output index bits.
 struct MarginalBits {
   uint32_t pos[kMaxBits];
@@ -89,9 +93,24 @@ __global__ void __launch_bounds__(kBlock)
   const double* __restrict__ row = sums + (b << cbits);
   double acc = 0.0;
   uint64_t d = 0;
-  for (uint64_t t = 0; t < per; ++t) {
-    acc += row[base | d];
-    d = ((d | ~drop) + 1) & drop;
+  if (per >= kBatch) {
+    // the additions are a chain, but their operands are not: load a batch ahead, then add in
+    // order (one wait per batch instead of one memory latency per addition)
+    for (uint64_t t = 0; t < per; t += kBatch) {
+      double v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        v[u] = row[base | d];
+        d = ((d | ~drop) + 1) & drop;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) acc += v[u];
+    }
+  } else {
+    for (uint64_t t = 0; t < per; ++t) {
+      acc += row[base | d];
+      d = ((d | ~drop) + 1) & drop;
+    }
   }
   out[id] = acc;
 }

@@ -478,7 +478,7 @@ def launch_expectations(fn: str, buf: torch.Tensor, dtype: int, n: int, nbits: i
 NUMPY_CHUNK_BITS = 13
 #: log2 of the longest chain of ordered additions per marginal entry left to the device (each
 #: entry adds its pieces one after another, as NumPy does; longer chains stay on the host)
-DEVICE_MARGINAL_CHAIN_BITS = 20
+DEVICE_MARGINAL_CHAIN_BITS = 16
 
 
 def device_marginal_pays(n: int, keep: Sequence[int]) -> bool:

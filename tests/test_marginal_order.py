@@ -107,6 +107,7 @@ def test_a_sequential_or_fully_pairwise_order_would_differ() -> None:
 def test_device_marginal_is_chosen_for_short_chains_only() -> None:
     assert device_marginal_pays(26, [25])  # chunks of 8192, 2^12 per entry
     assert device_marginal_pays(26, list(range(10, 26)))
-    assert device_marginal_pays(26, list(range(6)))  # 2^20 entries added in order per output
+    assert device_marginal_pays(26, list(range(10)))  # 2^16 entries added in order per output
+    assert not device_marginal_pays(26, list(range(6)))  # 2^20: left to the host
     assert not device_marginal_pays(26, [0])  # 2^25 entries added in order
     assert not device_marginal_pays(26, list(range(26)))  # nothing to sum
