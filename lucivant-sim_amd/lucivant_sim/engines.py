@@ -77,6 +77,14 @@ class Workspace:
 WORKSPACE = Workspace()
 
 
+def to_device(host: np.ndarray) -> torch.Tensor:
+    """A device copy of a host array, queued on the current stream: staged through pinned
+    memory (torch's caching host allocator keeps the staging block until the copy has run), so
+    the copy does not wait for the work already queued on the stream."""
+    staged = torch.from_numpy(np.ascontiguousarray(host)).pin_memory()
+    return staged.to(require_device(), non_blocking=True)
+
+
 def upload(arrays: Sequence[np.ndarray], dtype: int = nat.C128) -> tuple[torch.Tensor, list[int]]:
     """Copy several small complex arrays to the device in one transfer; returns the buffer and
     each array's device address."""
@@ -84,7 +92,7 @@ def upload(arrays: Sequence[np.ndarray], dtype: int = nat.C128) -> tuple[torch.T
     flat = [np.ascontiguousarray(a, dtype=npdt).reshape(-1) for a in arrays]
     sizes = [f.size for f in flat]
     host = np.concatenate(flat) if flat else np.zeros(1, dtype=npdt)
-    dev = torch.from_numpy(host).to(require_device(), non_blocking=False)
+    dev = to_device(host)
     base = ptr(dev)
     item = np.dtype(npdt).itemsize
     addrs, off = [], 0
@@ -129,7 +137,7 @@ def plan_launches(buf: torch.Tensor, dtype: int, nbits: int, batch: int, plan: S
     tile_tables = []
     if tiles:
         host = np.concatenate([op.matrix.reshape(-1) for tp in tiles for op in tp.ops] or [np.zeros(1, dtype=np.complex128)])
-        tdev = torch.from_numpy(np.ascontiguousarray(host, dtype=np.complex128)).to(require_device())
+        tdev = to_device(np.ascontiguousarray(host, dtype=np.complex128))
         keep.append(tdev)
         at = 0
         for tp in tiles:

@@ -28,7 +28,7 @@ import torch
 
 from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Barrier, Circuit, Condition, Gate, GlobalPhase, IfElse, Measure, Operation, Reset
-from lucivant_sim.engines import WORKSPACE, Statevector, _NP_DTYPE, _TORCH_DTYPE, ptr, require_device, run_passes, stream_handle, upload
+from lucivant_sim.engines import WORKSPACE, Statevector, _NP_DTYPE, _TORCH_DTYPE, ptr, require_device, run_passes, stream_handle, to_device, upload
 from lucivant_sim.noise import Channel, NoiseModel, ReadoutError
 from lucivant_sim.plan import Pass, _as_pass, embed
 
@@ -195,14 +195,32 @@ def _fuse(qubits: list[int], items: list[_Queued], combos: np.ndarray) -> np.nda
     out = np.broadcast_to(np.eye(d, dtype=np.complex128), (len(combos), d, d)).copy()
     column = 0
     for it in items:
-        pos = [qubits.index(q) for q in it.qubits]
+        pos = tuple(qubits.index(q) for q in it.qubits)
         if it.idx is None:
-            out = embed(it.table[0], pos, k) @ out
+            out = embed(it.table[0], list(pos), k) @ out
             continue
-        choices = np.stack([np.eye(d, dtype=np.complex128)] + [embed(m, pos, k) for m in it.table])
-        out = choices[combos[:, column] + 1] @ out
+        out = _choices(it.table, pos, k)[combos[:, column] + 1] @ out
         column += 1
     return out
+
+
+_CHOICES: dict[tuple, tuple[list[np.ndarray], np.ndarray]] = {}
+
+
+def _choices(table: list[np.ndarray], pos: tuple[int, ...], k: int) -> np.ndarray:
+    """The identity followed by each table entry embedded at ``pos`` of a k-position operator.
+    Tables of channel branches recur from block to block, so the stack is kept per (table, pos,
+    k) while the table object lives."""
+    key = (id(table), pos, k)
+    hit = _CHOICES.get(key)
+    if hit is not None and hit[0] is table:
+        return hit[1]
+    d = 1 << k
+    stack = np.stack([np.eye(d, dtype=np.complex128)] + [embed(m, list(pos), k) for m in table])
+    if len(_CHOICES) > 512:
+        _CHOICES.clear()
+    _CHOICES[key] = (table, stack)
+    return stack
 
 
 def _combinations(columns: list[np.ndarray]) -> tuple[np.ndarray, np.ndarray]:
@@ -233,19 +251,33 @@ def _split_for_host(items: list[_Queued], k: int, pass_seconds: float) -> list[l
     budget = max(pass_seconds, 2e-4)
     parts: list[list[_Queued]] = []
     part: list[_Queued] = []
-    keys: list[np.ndarray] = []
+    # each shot's combination of the part's choices so far, as an index into the distinct ones
+    # (extended one operator at a time: the pairs (combination, new choice) are the new ones)
+    inverse: np.ndarray | None = None
     for it in items:
-        grown = keys + ([it.idx] if it.idx is not None else [])
-        combos = len(_combinations(grown)[0]) if grown else 1
-        if part and combos * (len(part) + 1) * d3 * _FUSE_SECONDS_PER_D3 > budget:
+        grown, count = _extend(inverse, it.idx)
+        if part and count * (len(part) + 1) * d3 * _FUSE_SECONDS_PER_D3 > budget:
             parts.append(part)
             part = []
-            grown = [it.idx] if it.idx is not None else []
+            grown, count = _extend(None, it.idx)
         part.append(it)
-        keys = grown
+        inverse = grown
     if part:
         parts.append(part)
     return parts
+
+
+def _extend(inverse: np.ndarray | None, idx: np.ndarray | None) -> tuple[np.ndarray | None, int]:
+    """Distinct combinations after one more per-shot choice column (None: a shared operator):
+    each shot's new combination index and the number of combinations."""
+    if idx is None:
+        return inverse, 1 if inverse is None else int(inverse.max()) + 1
+    if inverse is None:
+        uniq, inv = np.unique(idx, return_inverse=True)
+        return inv.reshape(-1), len(uniq)
+    code = inverse.astype(np.int64) * (int(idx.max()) + 2) + (idx.astype(np.int64) + 1)
+    uniq, inv = np.unique(code, return_inverse=True)
+    return inv.reshape(-1), len(uniq)
 
 
 def _descending_many(mats: np.ndarray, bits: tuple[int, ...]) -> tuple[np.ndarray, tuple[int, ...]]:
@@ -339,12 +371,15 @@ class _Batch:
         alone when sel[j] < 0."""
         if not np.any(sel >= 0):
             return
-        dev, (addr,) = upload([table], self.dtype)
-        sel_dev = torch.from_numpy(np.ascontiguousarray(sel, dtype=np.int32)).to(self.buf.device)
+        # the operators and the selectors go over in one copy: the table's bytes, then the int32
+        # selectors (the table's byte size is a multiple of 8, so they stay aligned)
+        mats = np.ascontiguousarray(table, dtype=_NP_DTYPE[self.dtype]).reshape(-1).view(np.uint8)
+        sels = np.ascontiguousarray(sel, dtype=np.int32).view(np.uint8)
+        dev = to_device(np.concatenate([mats, sels]))
+        base = ptr(dev)
         fn = "lsim_sv_apply_dense" if kind == "dense" else "lsim_sv_apply_diag"
-        nat.call(fn, ptr(self.buf), self.dtype, self.n, self.s, len(qubits), nat.int_array(qubits), addr, ptr(sel_dev), stream_handle())
+        nat.call(fn, ptr(self.buf), self.dtype, self.n, self.s, len(qubits), nat.int_array(qubits), base, base + mats.size, stream_handle())
         dev.record_stream(torch.cuda.current_stream())
-        sel_dev.record_stream(torch.cuda.current_stream())
 
     def reduced(self, qubits: tuple[int, ...]) -> np.ndarray:
         """Per-slot reduced density matrices [s][2^k][2^k] on ``qubits`` (float64 sums)."""
@@ -358,7 +393,7 @@ class _Batch:
 
     def copy_slots(self, src: np.ndarray, dst: np.ndarray) -> None:
         """State dst[j] <- state src[j] (slots in use; no slot is both a source and a target)."""
-        both = torch.from_numpy(np.ascontiguousarray(np.stack([src, dst]), dtype=np.int32)).to(self.buf.device)
+        both = to_device(np.ascontiguousarray(np.stack([src, dst]), dtype=np.int32))
         nat.call("lsim_sv_copy_states", ptr(self.buf), self.dtype, self.n, self.s, len(src), ptr(both[0]), ptr(both[1]), stream_handle())
         both.record_stream(torch.cuda.current_stream())
 
@@ -494,9 +529,16 @@ class _Batch:
         return actual
 
 
-def _mixture_choice(channel: Channel, r: np.ndarray) -> tuple[np.ndarray, list[np.ndarray]]:
-    """Branch index per shot into the non-identity unitaries (-1: identity or no branch), with
-    the cumulative sums formed in the reference's order (engines.py:162-170)."""
+_MIXTURES: dict[int, tuple[Channel, np.ndarray, np.ndarray, list[np.ndarray]]] = {}
+
+
+def _mixture_tables(channel: Channel) -> tuple[np.ndarray, np.ndarray, list[np.ndarray]]:
+    """The channel's cumulative branch probabilities in the reference's order, each branch's
+    index into the non-identity unitaries (-1: the identity, and one more -1 past the last
+    branch), and those unitaries; built once per channel object."""
+    hit = _MIXTURES.get(id(channel))
+    if hit is not None and hit[0] is channel:
+        return hit[1], hit[2], hit[3]
     assert channel.mixture is not None
     cumulative, table, index = [], [], []
     acc = 0.0
@@ -508,8 +550,18 @@ def _mixture_choice(channel: Channel, r: np.ndarray) -> tuple[np.ndarray, list[n
         else:
             index.append(len(table))
             table.append(unitary)
-    branch = np.searchsorted(np.array(cumulative), r, side="right")
-    lookup = np.array(index + [-1], dtype=np.int64)
+    out = (np.array(cumulative), np.array(index + [-1], dtype=np.int64), table)
+    if len(_MIXTURES) > 256:
+        _MIXTURES.clear()
+    _MIXTURES[id(channel)] = (channel, *out)
+    return out
+
+
+def _mixture_choice(channel: Channel, r: np.ndarray) -> tuple[np.ndarray, list[np.ndarray]]:
+    """Branch index per shot into the non-identity unitaries (-1: identity or no branch), with
+    the cumulative sums formed in the reference's order (engines.py:162-170)."""
+    cumulative, lookup, table = _mixture_tables(channel)
+    branch = np.searchsorted(cumulative, r, side="right")
     return lookup[branch], table
 
 
