@@ -52,6 +52,14 @@ int lsim_device_info(int* cu_count, long long* hbm_bytes, char* arch, int arch_l
  * (simulator.py:267); needs no GPU. */
 int lsim_host_cumsum(const double* in, double* out, size_t n);
 
+/* HOST arrays, planning help for tile passes: for each candidate set of state bits sets[i],
+ * counts[i] = how many of the pending gates (qubit masks, in program order) can run on a tile of
+ * those bits: a gate runs when its qubits lie in the set and no earlier gate that cannot run
+ * shares a qubit with it (the scan stops once every qubit is blocked, `full`). No GPU. */
+int lsim_host_tile_runnable(const unsigned long long* masks, long long count,
+                            const unsigned long long* sets, int nsets, unsigned long long full,
+                            int* counts);
+
 /* |index> in every state: replaces Statevector.__init__ / DensityMatrix.__init__
  * (engines.py:139-148, 216-225; for a density matrix pass nbits=2n, index=0). */
 int lsim_sv_set_basis(void* states, int dtype, int nbits, long long batch, long long index,

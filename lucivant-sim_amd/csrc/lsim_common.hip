@@ -128,6 +128,30 @@ int lsim_host_cumsum(const double* in, double* out, size_t n) {
   return LSIM_OK;
 }
 
+int lsim_host_tile_runnable(const unsigned long long* masks, long long count,
+                            const unsigned long long* sets, int nsets, unsigned long long full,
+                            int* counts) {
+  LSIM_REQUIRE(count >= 0 && nsets >= 0, "negative sizes");
+  LSIM_REQUIRE((count == 0 || masks != nullptr) && (nsets == 0 || (sets != nullptr && counts != nullptr)),
+               "a NULL array");
+  for (int i = 0; i < nsets; ++i) {
+    const unsigned long long s = sets[i];
+    unsigned long long blocked = 0;
+    int runs = 0;
+    for (long long g = 0; g < count; ++g) {
+      const unsigned long long m = masks[g];
+      if ((m & blocked) || (m & ~s)) {
+        blocked |= m;
+        if (blocked == full) break;
+      } else {
+        ++runs;
+      }
+    }
+    counts[i] = runs;
+  }
+  return LSIM_OK;
+}
+
 int lsim_device_info(int* cu_count, long long* hbm_bytes, char* arch, int arch_len) {
   hipDeviceProp_t p;
   int dev = 0;

@@ -59,6 +59,7 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     ),
     "lsim_dm_probabilities": ([_c_void, _c_int, _c_int, _c_ll, _c_void, _c_void], _c_int),
     "lsim_sv_apply_perm": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_int_p, ctypes.c_uint, _c_void, _c_void, _c_void], _c_int),
+    "lsim_host_tile_runnable": ([_c_void, _c_ll, _c_void, _c_int, ctypes.c_ulonglong, _c_void], _c_int),
     "lsim_graph_capture_begin": ([_c_void], _c_int),
     "lsim_graph_capture_end": ([_c_void, ctypes.POINTER(ctypes.c_void_p)], _c_int),
     "lsim_graph_nodes": ([_c_void], _c_ll),

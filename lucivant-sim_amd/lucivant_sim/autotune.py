@@ -171,9 +171,26 @@ def tune(
     return rec
 
 
+#: The tuner's choices as measured on MI355X (gfx950) for random 40-layer u3 + cx/cz circuits
+#: of 12 to 26 qubits (timings dropped), shipped so that a process with no cache of its own
+#: plans with measured choices: tile passes from 21 qubits up, fused dense passes below.
+DEFAULTS = Path(__file__).with_name("autotune_defaults.json")
+
+
+def _defaults() -> dict[str, Any]:
+    try:
+        return json.loads(DEFAULTS.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
 def best_for(num_qubits: int) -> dict[str, Any] | None:
-    """The stored choice for this device and width, or the nearest stored width."""
-    table = load_cache().get(_arch()) if torch.cuda.is_available() else None
+    """The stored choice for this device and width, or the nearest stored width; the user's
+    cache first, else the shipped measurements for this device."""
+    if not torch.cuda.is_available():
+        return None
+    arch = _arch()
+    table = load_cache().get(arch) or _defaults().get(arch)
     if not table:
         return None
     if str(num_qubits) in table:

@@ -22,6 +22,7 @@ from functools import lru_cache
 
 import numpy as np
 
+from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Barrier, Gate, GlobalPhase, Measure, Operation, Reset
 from lucivant_sim.gates import Matrix
 from lucivant_sim.noise import Channel, NoiseModel, kron_conj
@@ -404,6 +405,66 @@ def _tile_cost(p: Pass) -> float:
     return 1.0 if p.k == 1 else 2.5
 
 
+def _tile_structure(qubits: list[tuple[int, ...]], diagonal: list[bool]) -> tuple[float, int]:
+    """The cost (in :func:`_tile_cost` units) and the number of the operators
+    :func:`_fuse_in_tile` forms from gates on ``qubits``, from the qubits alone: the one-qubit runs,
+    greedy and merge at width 2 decide by qubit sets only. An operator counts as diagonal when
+    all its gates are (a product of diagonals is diagonal; an accidentally diagonal product of
+    other gates counts as dense, so the cost is never underestimated)."""
+    runs: list[list] = []  # [qubits, diagonal]
+    open_run: dict[int, int] = {}
+    for qs, dg in zip(qubits, diagonal):
+        if len(qs) == 1:
+            at = open_run.get(qs[0])
+            if at is None:
+                open_run[qs[0]] = len(runs)
+                runs.append([qs, dg])
+            else:
+                runs[at][1] = runs[at][1] and dg
+            continue
+        for q in qs:
+            open_run.pop(q, None)
+        runs.append([qs, dg])
+    blocks: list[list] = []  # [qubit list, diagonal]
+    last: dict[int, int] = {}
+    for qs, dg in runs:
+        j = max((last.get(q, -1) for q in qs), default=-1)
+        if j >= 0 and len(set(blocks[j][0]) | set(qs)) <= 2:
+            blocks[j][0].extend(q for q in qs if q not in blocks[j][0])
+            blocks[j][1] = blocks[j][1] and dg
+        else:
+            blocks.append([list(qs), dg])
+            j = len(blocks) - 1
+        for q in qs:
+            last[q] = j
+    while True:  # the merge of _merge at width 2, on qubit sets
+        before = len(blocks)
+        for backward in (False, True):
+            alive = [True] * len(blocks)
+            order = range(len(blocks) - 1, -1, -1) if backward else range(len(blocks))
+            for j in order:
+                if not alive[j]:
+                    continue
+                mine = set(blocks[j][0])
+                others = range(j - 1, -1, -1) if backward else range(j + 1, len(blocks))
+                for j2 in others:
+                    if not alive[j2]:
+                        continue
+                    other = set(blocks[j2][0])
+                    if len(mine | other) <= 2:
+                        first, second = (blocks[j2], blocks[j]) if backward else (blocks[j], blocks[j2])
+                        blocks[j2] = [first[0] + [q for q in second[0] if q not in first[0]], first[1] and second[1]]
+                        alive[j] = False
+                        break
+                    if mine & other:
+                        break
+            blocks = [b for b, a in zip(blocks, alive) if a]
+        if len(blocks) == before:
+            break
+    cost = sum((0.5 * len(b[0])) if b[1] else (1.0 if len(b[0]) == 1 else 2.5) for b in blocks)
+    return cost, len(blocks)
+
+
 def _fuse_in_tile(items: list[tuple[Matrix, tuple[int, ...]]], bits: tuple[int, ...]) -> list[Pass]:
     """The gates of one tile pass as operators on at most 2 tile bits, in an order that keeps
     every dependency (the statevector greedy and merge at width 2, then tile-local bits)."""
@@ -449,40 +510,48 @@ def plan_tiles(
         else:
             raise ValueError(f"a static statevector can't run {type(op).__name__}")
     masks = [sum(1 << q for q in qs) for _, qs in items]
+    diagonal = [is_diagonal(m) for m, _ in items]
     full = (1 << n) - 1
     low_mask = (1 << low) - 1
     window = 64 * n  # pending gates looked at when choosing bits (those beyond are blocked)
 
-    def runnable(s: int, pending: list[int]) -> int:
-        blocked = count = 0
-        for gi in pending[:window]:
-            m = masks[gi]
-            if m & blocked or m & ~s:
-                blocked |= m
-                if blocked == full:
-                    break
-            else:
-                count += 1
-        return count
+    lib = nat.load()
+
+    def runnable(sets: list[int], window_masks: np.ndarray) -> list[int]:
+        """For each candidate set: how many pending gates can run on it (a gate runs when its
+        qubits lie in the set and no earlier gate that cannot run shares a qubit with it),
+        counted by the library's host helper."""
+        cand = np.array(sets, dtype=np.uint64)
+        out = np.zeros(len(sets), dtype=np.int32)
+        rc = lib.lsim_host_tile_runnable(window_masks.ctypes.data, len(window_masks), cand.ctypes.data, len(sets), full, out.ctypes.data)
+        if rc:
+            raise nat.NativeError(lib.lsim_last_error().decode(errors="replace"))
+        return out.tolist()
+
+    def first_best(sets: list[int], window_masks: np.ndarray) -> tuple[int, int]:
+        vals = runnable(sets, window_masks)
+        best = max(range(len(sets)), key=vals.__getitem__)  # the first of the largest
+        return sets[best], vals[best]
 
     def choose(pending: list[int]) -> int:
+        wm = np.array([masks[g] for g in pending[:window]], dtype=np.uint64)
         s = low_mask | masks[pending[0]]
         while bin(s).count("1") < tile_bits:
             outside = [q for q in range(n) if not (s >> q) & 1]
-            base = runnable(s, pending)
-            best = max(outside, key=lambda q: runnable(s | (1 << q), pending))
-            if runnable(s | (1 << best), pending) == base and bin(s).count("1") <= tile_bits - 2:
-                pair = max(((a, b) for i, a in enumerate(outside) for b in outside[i + 1 :]), key=lambda ab: runnable(s | (1 << ab[0]) | (1 << ab[1]), pending))
-                s |= (1 << pair[0]) | (1 << pair[1])
+            base = runnable([s], wm)[0]
+            best, value = first_best([s | (1 << q) for q in outside], wm)
+            if value == base and bin(s).count("1") <= tile_bits - 2:
+                pairs = [s | (1 << a) | (1 << b) for i, a in enumerate(outside) for b in outside[i + 1 :]]
+                s = first_best(pairs, wm)[0]
             else:
-                s |= 1 << best
-        cur = runnable(s, pending)
+                s = best
+        cur = runnable([s], wm)[0]
         for _ in range(3):
             better = False
             for a in [q for q in range(low, n) if (s >> q) & 1 and not (masks[pending[0]] >> q) & 1]:
-                for b in [q for q in range(n) if not (s >> q) & 1]:
-                    t = (s & ~(1 << a)) | (1 << b)
-                    v = runnable(t, pending)
+                swaps = [(s & ~(1 << a)) | (1 << b) for b in range(n) if not (s >> b) & 1]
+                vals = runnable(swaps, wm)
+                for t, v in zip(swaps, vals):
                     if v > cur:
                         s, cur, better = t, v, True
                         break
@@ -509,16 +578,18 @@ def plan_tiles(
                 blocked |= m
             else:
                 taken.append(gi)
-        # the longest prefix of the runnable gates whose operators fit the budget and the table
+        # the longest prefix of the runnable gates whose operators fit the budget and the table,
+        # searched on the operators' structure (which gates fuse into which operator depends on
+        # qubits only); the matrices are multiplied once, for the prefix chosen
         lo_n, hi_n = 1, len(taken)
-        best_ops = _fuse_in_tile([items[g] for g in taken[:1]], bits)
         while lo_n < hi_n:
             mid = (lo_n + hi_n + 1) // 2
-            cand = _fuse_in_tile([items[g] for g in taken[:mid]], bits)
-            if sum(map(_tile_cost, cand)) <= budget and len(cand) <= TILE_MAX_OPS:
-                lo_n, best_ops = mid, cand
+            cost, count = _tile_structure([items[g][1] for g in taken[:mid]], [diagonal[g] for g in taken[:mid]])
+            if cost <= budget and count <= TILE_MAX_OPS:
+                lo_n = mid
             else:
                 hi_n = mid - 1
+        best_ops = _fuse_in_tile([items[g] for g in taken[:lo_n]], bits)
         used = set(taken[:lo_n])
         out.append(TilePass(bits, best_ops, lo_n))
         pending = [g for g in pending if g not in used]

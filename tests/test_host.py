@@ -676,3 +676,48 @@ def test_permutation_passes_are_recognised() -> None:
     hp = Pass("dense", (0,), h)
     assert hp.payload() is hp.matrix
     assert np.array_equal(Pass("dense", (1, 0), cx).payload(), np.ones(4))
+
+
+def test_tile_planning_helpers_agree_with_their_definitions() -> None:
+    """The tile planner's two shortcuts: the library's host count of runnable gates equals the
+    rule stated in Python (a gate runs when its qubits lie in the set and no earlier gate that
+    cannot run shares a qubit with it), and the structural estimate of a tile pass's operators
+    has the real fusion's operator count and never a lower cost."""
+    import numpy as np
+
+    from lucivant_sim import _native as nat
+    from lucivant_sim.circuit import Gate
+    from lucivant_sim.plan import _fuse_in_tile, _tile_cost, _tile_structure, is_diagonal
+
+    rng = np.random.default_rng(3)
+    n = 20
+    masks = []
+    for _ in range(300):
+        qs = rng.choice(n, size=int(rng.integers(1, 3)), replace=False)
+        masks.append(sum(1 << int(q) for q in qs))
+    full = (1 << n) - 1
+
+    def rule(s: int) -> int:
+        blocked = count = 0
+        for m in masks:
+            if m & blocked or m & ~s:
+                blocked |= m
+                if blocked == full:
+                    break
+            else:
+                count += 1
+        return count
+
+    sets = [int(sum(1 << int(q) for q in rng.choice(n, size=12, replace=False))) for _ in range(50)]
+    out = np.zeros(len(sets), dtype=np.int32)
+    arr = np.array(masks, dtype=np.uint64)
+    cand = np.array(sets, dtype=np.uint64)
+    nat.call("lsim_host_tile_runnable", arr.ctypes.data, len(arr), cand.ctypes.data, len(sets), full, out.ctypes.data)
+    assert out.tolist() == [rule(s) for s in sets]
+    for seed in range(12):
+        c = ls.parse(wl.random_layers(14, 6, seed, two_qubit=("cx", "cz")))
+        items = [(np.asarray(op.matrix), tuple(op.qubits)) for op in c.operations if isinstance(op, Gate)]
+        sub = [it for it in items if all(q < 12 for q in it[1])][:40]
+        real = _fuse_in_tile(sub, tuple(range(12)))
+        cost, count = _tile_structure([q for _, q in sub], [is_diagonal(m) for m, _ in sub])
+        assert count == len(real) and cost >= sum(map(_tile_cost, real))
