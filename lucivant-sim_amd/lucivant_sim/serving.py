@@ -97,6 +97,8 @@ class _Job:
     # outputs queued behind the last pass: (event, device tensor), read once the event completes
     finishing: tuple[torch.cuda.Event, torch.Tensor, Any] | None = None
     seq: int = 0  # admission order
+    # wide trajectory requests: their run on the trajectory worker thread
+    future: Any = None
 
 
 class Server:
@@ -137,6 +139,11 @@ class Server:
         # host sampling of wide count requests (a cumulative sum over 2^n probabilities) runs
         # beside the scheduler, so launches continue meanwhile
         self._sampler = ThreadPoolExecutor(max_workers=4, thread_name_prefix="lsim-sample")
+        # trajectory batches wider than an LDS program (host-driven: they read reduced density
+        # matrices back between launches) run one at a time on a thread and stream of their own,
+        # so the scheduler keeps launching the other requests meanwhile
+        self._trajectories = ThreadPoolExecutor(max_workers=1, thread_name_prefix="lsim-traj")
+        self._traj_stream = torch.cuda.Stream()
         self._lock = threading.Lock()
         self._sampling: list[Any] = []
         self.ticks = 0
@@ -168,6 +175,9 @@ class Server:
         self._stop.set()
         if self._thread is not None:
             self._thread.join()
+        for job in list(self.resident):
+            if job.future is not None:
+                job.future.exception()  # wait; the error, if any, was reported to the requester
         for f in self._sampling:
             f.result()
         self._sampling = []
@@ -178,8 +188,12 @@ class Server:
                 time.sleep(0.0005)
 
     def run_until_idle(self) -> None:
-        while self.tick() or self.reading or self.waiting or self.resident or not self.inbox.empty():
-            pass
+        while True:
+            worked = self.tick()
+            if not (worked or self.reading or self.waiting or self.resident or not self.inbox.empty()):
+                break
+            if not worked:
+                time.sleep(0.0005)
         for f in self._sampling:
             f.result()
         self._sampling = []
@@ -205,7 +219,7 @@ class Server:
         footprint = size * (req.shots if c.method == "trajectories" and c.num_qubits > self.max_lds else 1)
         return _Job(req, c, c.method, footprint, c.terms, seq=next(self._seq))
 
-    def _admit(self) -> None:
+    def _admit(self) -> bool:
         while True:
             try:
                 self.reading.append(self.inbox.get_nowait())
@@ -227,6 +241,7 @@ class Server:
                 self.waiting.append(self._prepare(req, c))
             except BaseException as exc:  # noqa: BLE001 - reported to the requester
                 self._finish_error(req, exc)
+        moved = len(unread) < len(self.reading)
         self.reading = unread
         still = []
         for job in self.waiting:
@@ -235,23 +250,29 @@ class Server:
                 self.peak_used = max(self.peak_used, self.used)
                 self.resident.append(job)
                 job.req.stamps["admitted"] = time.perf_counter()
+                moved = True
             else:
                 still.append(job)
         self.waiting = still
+        return moved
 
     def tick(self) -> bool:
-        """One scheduling round; True if it did any work."""
+        """One scheduling round; True if it did any work (waiting on the device or on the
+        trajectory thread is not work: the caller then sleeps briefly, which lets the host
+        threads that do the waited-for work take the interpreter)."""
         t0 = time.perf_counter()
-        self._admit()
+        admitted = self._admit()
         t1 = time.perf_counter()
         self.host_seconds["admit"] += t1 - t0
         if not self.resident:
-            return False
+            return admitted
         self.ticks += 1
+        progress = admitted
         small = [j for j in self.resident if j.state is None and (j.c.table is not None or j.c.program is not None)]
         if small:
             with torch.cuda.stream(self.small_stream):
                 self._run_small(small)
+            progress = True
         t2 = time.perf_counter()
         self.host_seconds["small"] += t2 - t1
         wide = [j for j in self.resident if j not in small]
@@ -263,11 +284,12 @@ class Server:
                 if launching >= self.max_launching:
                     continue
                 launching += 1
-            self._advance_wide(job)
+            progress |= self._advance_wide(job)
         t3 = time.perf_counter()
         self.host_seconds["wide_launch"] += t3 - t2
         for job in wide:
             if job.finishing is None and job.passes is not None and job.next_pass >= len(job.passes) and job.state is not None:
+                progress = True
                 try:
                     self._queue_outputs(job)
                 except BaseException as exc:  # noqa: BLE001 - e.g. no memory for the outputs
@@ -277,10 +299,11 @@ class Server:
         self.host_seconds["wide_outputs"] += t4 - t3
         for job in wide:
             if job.finishing is not None and job.finishing[0].query():
+                progress = True
                 self._complete_wide(job)
         self._sampling = [f for f in self._sampling if not f.done()]
         self.host_seconds["wide_finish"] += time.perf_counter() - t4
-        return True
+        return progress
 
     @staticmethod
     def _priority(job: _Job) -> tuple:
@@ -324,7 +347,8 @@ class Server:
                     self._finish_error(j.req, exc)
                     self._release(j)
 
-    def _advance_wide(self, job: _Job) -> None:
+    def _advance_wide(self, job: _Job) -> bool:
+        """Advance one wide request; True if it launched or finished anything."""
         if job.method == "density_matrix":
             from lucivant_sim.simulator import expectation, run
 
@@ -333,27 +357,28 @@ class Server:
                 if job.terms is not None:
                     # a noisy expectation: Tr(P rho) of the density matrix, as expectation() gives
                     self._finish(job, expectation(job.c.circuit, job.terms, job.req.noise))
-                    return
+                    return True
                 res = run(job.c.circuit, job.req.shots, seed=job.req.seed, noise=job.req.noise, method="density_matrix")
                 self._finish(job, res.counts)
             except BaseException as exc:  # noqa: BLE001
                 self._finish_error(job.req, exc)
                 self._release(job)
-            return
+            return True
         if job.method == "trajectories":
-            from lucivant_sim.trajectories import run_trajectories
-
+            if job.future is None:
+                job.req.stamps.setdefault("launched", time.perf_counter())
+                job.future = self._trajectories.submit(self._run_trajectories, job)
+                return True
+            if not job.future.done():
+                return False
             try:
-                c = job.c.circuit
-                assert c is not None
-                tally = run_trajectories(c, job.req.noise, job.req.shots, np.random.default_rng(job.req.seed), dtype=self.dtype)
-                counts = {c.format_bits(b): k for b, k in sorted(tally.items())} if c.num_clbits else {}
+                counts = job.future.result()
                 job.passes, job.next_pass = [], 0
                 self._finish(job, counts)
             except BaseException as exc:  # noqa: BLE001
                 self._finish_error(job.req, exc)
                 self._release(job)
-            return
+            return True
         lib = nat.load()
         if job.state is None:
             job.stream = self.streams[self._rr % len(self.streams)]
@@ -370,7 +395,7 @@ class Server:
             except BaseException as exc:  # noqa: BLE001 - reported to the requester
                 self._finish_error(job.req, exc)
                 self._release(job)
-                return
+                return True
         assert job.calls is not None
         job.req.stamps.setdefault("launched", time.perf_counter())
         _use_switches(job.c.num_qubits)
@@ -380,8 +405,20 @@ class Server:
                 exc = nat.NativeError(lib.lsim_last_error().decode(errors="replace"))
                 self._finish_error(job.req, exc)
                 self._release(job)
-                return
+                return True
+        launched = stop > job.next_pass
         job.next_pass = stop
+        return launched
+
+    def _run_trajectories(self, job: _Job) -> dict[str, int]:
+        """A wide trajectory request's counts, on the trajectory thread and stream."""
+        from lucivant_sim.trajectories import run_trajectories
+
+        c = job.c.circuit
+        assert c is not None
+        with torch.cuda.stream(self._traj_stream):
+            tally = run_trajectories(c, job.req.noise, job.req.shots, np.random.default_rng(job.req.seed), dtype=self.dtype)
+        return {c.format_bits(b): k for b, k in sorted(tally.items())} if c.num_clbits else {}
 
     def _queue_outputs(self, job: _Job) -> None:
         """Queue a wide request's outputs behind its last pass, on its stream: Pauli

@@ -21,6 +21,7 @@ Circuits whose arms draw differently are run shot after shot with the same devic
 from __future__ import annotations
 
 import os
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -31,6 +32,10 @@ from lucivant_sim.circuit import Barrier, Circuit, Condition, Gate, GlobalPhase,
 from lucivant_sim.engines import WORKSPACE, Statevector, _NP_DTYPE, _TORCH_DTYPE, ptr, require_device, run_passes, stream_handle, to_device, upload
 from lucivant_sim.noise import Channel, NoiseModel, ReadoutError
 from lucivant_sim.plan import Pass, _as_pass, embed
+
+#: Counters of the last :func:`run_batched`: host round trips (reduced density matrices read
+#: back), seconds the host waited in them for the device, and launches over the batch.
+STATS: dict[str, float] = {"syncs": 0, "wait_seconds": 0.0, "passes": 0}
 
 #: HBM the batch of trajectory states may take (bytes); shots beyond it run in further chunks.
 BATCH_BYTES = int(os.environ.get("LSIM_TRAJECTORY_BYTES", str(64 << 30)))
@@ -361,6 +366,7 @@ class _Batch:
         nat.call("lsim_sv_set_basis", ptr(self.buf), self.dtype, self.n, self.s, 0, stream_handle())
 
     def run_pass(self, p: Pass) -> None:
+        STATS["passes"] += 1
         run_passes(self.buf, self.dtype, self.n, self.s, [p])
 
     def finish(self) -> None:
@@ -371,6 +377,7 @@ class _Batch:
         alone when sel[j] < 0."""
         if not np.any(sel >= 0):
             return
+        STATS["passes"] += 1
         # the operators and the selectors go over in one copy: the table's bytes, then the int32
         # selectors (the table's byte size is a multiple of 8, so they stay aligned)
         mats = np.ascontiguousarray(table, dtype=_NP_DTYPE[self.dtype]).reshape(-1).view(np.uint8)
@@ -389,7 +396,11 @@ class _Batch:
         nat.call(
             "lsim_sv_reduced_dm", ptr(self.buf), self.dtype, self.n, self.s, k, nat.int_array(qubits), ptr(out), ptr(ws), stream_handle()
         )
-        return out.cpu().numpy().reshape(self.s, 1 << k, 1 << k)
+        t0 = time.perf_counter()
+        host = out.cpu().numpy().reshape(self.s, 1 << k, 1 << k)
+        STATS["syncs"] += 1
+        STATS["wait_seconds"] += time.perf_counter() - t0
+        return host
 
     def copy_slots(self, src: np.ndarray, dst: np.ndarray) -> None:
         """State dst[j] <- state src[j] (slots in use; no slot is both a source and a target)."""
@@ -713,6 +724,7 @@ def run_batched(
     Requires draw-balanced if/else arms (see :func:`draw_count`). ``share=False`` gives every shot
     its own state from the start (the same results, with more device work). ``record`` (a dict)
     receives ``draws`` [shots, D] and ``decisions`` [shots, D, 2] (see :func:`_run_chunk`)."""
+    STATS.update(syncs=0, wait_seconds=0.0, passes=0)
     steps, per_shot, nflags = compile_program(circuit, noise)
     item = np.dtype(_NP_DTYPE[dtype]).itemsize
     limit = batch_bytes if batch_bytes is not None else BATCH_BYTES
