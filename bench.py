@@ -287,7 +287,7 @@ SERVING_REQUESTS = 500
 #: requests of the saturation subset that the single-request baselines also serve
 BASELINE_REQUESTS = 64
 #: widest count request that carries the noise model (run as trajectories)
-NOISY_MAX_QUBITS = 14
+NOISY_MAX_QUBITS = 16
 
 
 def serving_requests(count: int, seed: int):  # type: ignore[no-untyped-def]
@@ -497,11 +497,21 @@ def bench_trajectories(shots: int = 4096) -> dict:
     noise = workloads.noise_c4()
     out = {}
     bits = {}
+    from lucivant_sim import trajectories
+
     for name, dt in (("complex128", nat.C128), ("complex64", nat.C64)):
         t0 = time.perf_counter()
         bits[name] = run_batched(c, noise, shots, np.random.default_rng(7), dtype=dt)
         el = time.perf_counter() - t0
-        out[name] = {"seconds": round(el, 3), "shots_per_s": round(shots / el, 1)}
+        st = trajectories.STATS
+        out[name] = {
+            "seconds": round(el, 3),
+            "shots_per_s": round(shots / el, 1),
+            "host_round_trips": int(st["syncs"]),
+            "launches_over_the_batch": int(st["passes"]),
+            "seconds_waiting_for_the_device": round(st["wait_seconds"], 4),
+            "seconds_of_host_work": round(el - st["wait_seconds"], 4),
+        }
     a, b = bits["complex128"], bits["complex64"]
     ua, ca = np.unique(a, return_counts=True)
     ub, cb = np.unique(b, return_counts=True)
