@@ -109,6 +109,29 @@ __device__ __forceinline__ uint32_t entry(uint32_t t, uint32_t k, uint32_t R) {
   return e;
 }
 
+// entry(t, 0, R) for the thread's own t, cheaper: the positions outside R are found on the
+// scalar unit, the wavefront's bits of t (6 and 7) are uniform, and only the 6 lane bits are
+// deposited per lane.
+__device__ __forceinline__ uint32_t thread_part(uint32_t t, uint32_t R) {
+  uint32_t free = ~R & ((1u << kTileBits) - 1u);
+  uint32_t e = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const uint32_t p = (uint32_t)__builtin_ctz(free);
+    free &= free - 1u;
+    e |= ((t >> i) & 1u) << p;
+  }
+  const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
+  uint32_t we = 0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const uint32_t p = (uint32_t)__builtin_ctz(free);
+    free &= free - 1u;
+    we |= ((w >> i) & 1u) << p;
+  }
+  return e | we;
+}
+
 // State offset of tile index e: bit i of e at state bit bits[i].
 __device__ __forceinline__ uint32_t deposit(uint32_t e, const uint32_t* bits) {
   uint32_t o = 0;
@@ -219,22 +242,29 @@ __device__ __forceinline__ void run_program(V (&a)[kRegs], V* tile, const TilePr
                                             const double2* __restrict__ mats, uint32_t t,
                                             uint32_t& tpart, uint32_t (&P)[4]) {
   uint32_t layout = uniform(prog.layout[0]);
-  tpart = entry(t, 0, layout);
+  tpart = thread_part(t, layout);
   layout_bits(layout, P);
   const int nsegs = (int)uniform((uint32_t)prog.nsegs);
   for (int s = 0; s < nsegs; ++s) {
     if (s > 0) {
       // into segment s's layout (the planner starts a segment only where the layout changes)
-      // through LDS
+      // through LDS; the thread and register parts of an index have disjoint bits, so an
+      // entry's element offset is their sum
       __syncthreads();  // the previous exchange (or the previous tile's) has been read
+      {
+        V* mine = tile + tpart;
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) tile[tpart | kpart(k, P)] = a[k];
+        for (int k = 0; k < kRegs; ++k) mine[kpart(k, P)] = a[k];
+      }
       layout = uniform(prog.layout[s]);
-      tpart = entry(t, 0, layout);
+      tpart = thread_part(t, layout);
       layout_bits(layout, P);
       __syncthreads();
+      {
+        const V* mine = tile + tpart;
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) a[k] = tile[tpart | kpart(k, P)];
+        for (int k = 0; k < kRegs; ++k) a[k] = mine[kpart(k, P)];
+      }
     }
     const int j1 = (int)uniform(prog.first[s + 1]);
     for (int j = (int)uniform(prog.first[s]); j < j1; ++j) {
