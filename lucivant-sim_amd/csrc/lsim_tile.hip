@@ -235,6 +235,37 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t q, const uint32_t* bits) 
   return q;
 }
 
+// Operator j of the program on the tile in registers (layout: thread part tpart, register
+// positions P).
+template <typename R, typename V>
+__device__ __forceinline__ void apply_op(V (&a)[kRegs], const TileProgram& prog,
+                                         const double2* __restrict__ mats, int j, uint32_t tpart,
+                                         const uint32_t (&P)[4]) {
+  const double2* M = mats + uniform(prog.mat[j]);
+  const uint32_t kind = uniform(prog.kind[j]), kk = uniform(prog.k[j]);
+  const uint32_t pa = uniform(prog.a[j]), pb = uniform(prog.b[j]);
+  if (kind == LSIM_TILE_DENSE) {
+    dense_op<R>(a, (int)kk, (int)pa, (int)pb, M);
+    return;
+  }
+  // diagonal: the entry's own bits pick the factor
+  const bool two = kk == 2;
+  const double2 d0 = M[0], d1 = M[1];
+  const double2 d2 = two ? M[2] : d0, d3 = two ? M[3] : d1;
+#pragma unroll
+  for (int k = 0; k < kRegs; ++k) {
+    const uint32_t e = tpart | kpart(k, P);
+    // factor index (hb, lb): selects written out, so no per-lane indexed array
+    const bool hb = two && ((e >> pa) & 1u);
+    const bool lb = two ? ((e >> pb) & 1u) : ((e >> pa) & 1u);
+    double2 d;
+    d.x = hb ? (lb ? d3.x : d2.x) : (lb ? d1.x : d0.x);
+    d.y = hb ? (lb ? d3.y : d2.y) : (lb ? d1.y : d0.y);
+    V y = {0, 0};
+    a[k] = mac<R>(y, d, a[k]);
+  }
+}
+
 // Every segment of the program on one tile held in registers (a, in the load layout on entry),
 // regrouping through LDS between segments; on return (tpart, P) describe the final layout.
 template <typename R, typename V>
@@ -267,31 +298,14 @@ __device__ __forceinline__ void run_program(V (&a)[kRegs], V* tile, const TilePr
       }
     }
     const int j1 = (int)uniform(prog.first[s + 1]);
-    for (int j = (int)uniform(prog.first[s]); j < j1; ++j) {
-      const double2* M = mats + uniform(prog.mat[j]);
-      const uint32_t kind = uniform(prog.kind[j]), kk = uniform(prog.k[j]);
-      const uint32_t pa = uniform(prog.a[j]), pb = uniform(prog.b[j]);
-      if (kind == LSIM_TILE_DENSE) {
-        dense_op<R>(a, (int)kk, (int)pa, (int)pb, M);
-        continue;
-      }
-      // diagonal: the entry's own bits pick the factor
-      const bool two = kk == 2;
-      const double2 d0 = M[0], d1 = M[1];
-      const double2 d2 = two ? M[2] : d0, d3 = two ? M[3] : d1;
-#pragma unroll
-      for (int k = 0; k < kRegs; ++k) {
-        const uint32_t e = tpart | kpart(k, P);
-        // factor index (hb, lb): selects written out, so no per-lane indexed array
-        const bool hb = two && ((e >> pa) & 1u);
-        const bool lb = two ? ((e >> pb) & 1u) : ((e >> pa) & 1u);
-        double2 d;
-        d.x = hb ? (lb ? d3.x : d2.x) : (lb ? d1.x : d0.x);
-        d.y = hb ? (lb ? d3.y : d2.y) : (lb ? d1.y : d0.y);
-        V y = {0, 0};
-        a[k] = mac<R>(y, d, a[k]);
-      }
+    // two operators per iteration: the register allocator can then let the first write where the
+    // second reads, instead of moving every amplitude back at the end of each operator
+    int j = (int)uniform(prog.first[s]);
+    for (; j + 1 < j1; j += 2) {
+      apply_op<R>(a, prog, mats, j, tpart, P);
+      apply_op<R>(a, prog, mats, j + 1, tpart, P);
     }
+    if (j < j1) apply_op<R>(a, prog, mats, j, tpart, P);
   }
 }
 
