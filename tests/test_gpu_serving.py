@@ -233,3 +233,31 @@ def test_tile_plans_chosen_by_the_autotuner_match_the_oracle(monkeypatch: pytest
         assert r.error is None, r.error
     assert counts20.result == counts20_want
     assert abs(hea18.result - hea18_want) < 1e-10
+
+
+def test_noisy_wide_count_requests_run_beside_the_rest() -> None:
+    """Noisy count requests too wide for an LDS program (14 and 16 qubits, trajectories through
+    the HBM batch) run on the server's trajectory thread while the other requests proceed: each
+    result equals the single-circuit API's alone and in the mix, and the 14-qubit one equals the
+    oracle's per-shot loop for the same seed."""
+    from oracle import engine as orc
+
+    noise = ls.NoiseModel.uniform(single=1e-3, two=1e-2)
+    w14 = Request(wl.random_layers(14, 3, 501), shots=300, seed=21, noise=noise, method="trajectories")
+    w16 = Request(wl.teleport_chain(16, 2), shots=400, seed=22, noise=wl.noise_c4(), method="trajectories")
+    mix = [w14, *_requests()[:5], w16]
+    server = Server(streams=3, chunk_passes=8)
+    for r in mix:
+        server.submit(r)
+    server.run_until_idle()
+    for r in mix:
+        assert r.error is None, r.error
+    want14 = orc.trajectory_counts(ls.parse(w14.program), noise, 300, np.random.default_rng(21))
+    assert w14.result == want14
+    for r in (w14, w16):
+        assert r.result == ls.run(r.program, r.shots, seed=r.seed, noise=r.noise, method="trajectories").counts
+        alone = Request(r.program, shots=r.shots, seed=r.seed, noise=r.noise, method="trajectories")
+        server = Server(streams=1)
+        server.submit(alone)
+        server.run_until_idle()
+        assert alone.error is None and alone.result == r.result
