@@ -193,11 +193,13 @@ struct PauliGroup {
   uint32_t wodd, wneg;         // bit t: w_t odd (Re[i^w z] is +-Im z), Re[i^w z] negated (w = 1, 2)
   uint8_t zeros[kTileBits];    // ascending bit positions a coset representative leaves zero
   int nterms, low, dims;
+  int diagonal;                // every X mask is 0: the strings need no partner amplitudes
 };
 
 // NT = the group's string count rounded up to a power of two (padding strings read a partner and
 // are never written out), so every loop below unrolls and every accumulator is a register.
-template <typename R, int NT>
+// SL: streaming (non-temporal) loads, for states the caches cannot keep.
+template <typename R, int NT, bool SL>
 __global__ void __launch_bounds__(kBlock)
     k_pauli_group(const typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t nb,
                   int call_terms, PauliGroup g, double* __restrict__ partial, uint32_t batch_base) {
@@ -210,8 +212,8 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t cosets = 1ull << (nbits - kTileBits);
   const uint32_t lmask = (1u << g.low) - 1u;
   // offset B(c) ^ l of each of this thread's tile entries e = tid + k * kBlock (the same in
-  // every coset)
-  uint64_t off[PER];
+  // every coset; addresses stay below 2^kMaxBits)
+  uint32_t off[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const uint32_t e = threadIdx.x + k * kBlock;
@@ -219,7 +221,7 @@ __global__ void __launch_bounds__(kBlock)
     uint64_t o = e & lmask;
     for (int i = 0; i < g.dims; ++i)
       if ((c >> i) & 1u) o ^= g.basis[i];
-    off[k] = o;
+    off[k] = (uint32_t)o;
   }
   // the sign (-1)^parity(e & tz) splits into this thread's part, parity(tid & tz), applied at the
   // end, and the entry's part kpar (e = tid + k * kBlock: the two use disjoint bits)
@@ -229,23 +231,46 @@ __global__ void __launch_bounds__(kBlock)
   double acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0;
-  for (uint64_t q = blockIdx.x; q < cosets; q += nb) {
-    uint64_t r = q;
+  // coset q's representative: q with zeros spread in at g.zeros
+  auto representative = [&](uint64_t q) {
     for (int i = 0; i < kTileBits; ++i) {
       const uint32_t p = g.zeros[i];
-      r = ((r >> p) << (p + 1)) | (r & ((1ull << p) - 1));
+      q = ((q >> p) << (p + 1)) | (q & ((1ull << p) - 1));
     }
-    {
-      V a[PER];
+    return (uint32_t)q;
+  };
+  // Software pipeline: the amplitudes of the workgroup's next coset are loaded into `a` while
+  // the strings of the current one are evaluated from LDS. (The barriers wait only for LDS, so
+  // the loads stay in flight until the next coset's LDS stores need them.) After its last coset
+  // a workgroup loads coset 0 once more, which every workgroup shares and the cache holds: the
+  // loads stay unconditional, so the registers they fill are the ones the next iteration reads.
+  R are[PER], aim[PER];  // (as two arrays of scalars: an array of V is kept in scratch memory)
+  uint32_t r = blockIdx.x < cosets ? representative(blockIdx.x) : 0u;
 #pragma unroll
-      for (int k = 0; k < PER; ++k) a[k] = st[r ^ off[k]];
+  for (int k = 0; k < PER; ++k) {
+    const V v = load_amp<SL>(st + (r ^ off[k]));
+    are[k] = v.x;
+    aim[k] = v.y;
+  }
+  for (uint64_t q = blockIdx.x; q < cosets; q += nb) {
 #pragma unroll
-      for (int k = 0; k < PER; ++k) tile[threadIdx.x + k * kBlock] = a[k];
+    for (int k = 0; k < PER; ++k) {
+      V v;
+      v.x = are[k];
+      v.y = aim[k];
+      tile[threadIdx.x + k * kBlock] = v;
     }
     __syncthreads();
     uint32_t cpar = 0;  // coset sign of each string, parity(r & Z) (wavefront-uniform)
 #pragma unroll
     for (int t = 0; t < NT; ++t) cpar |= (uint32_t)(__popcll(r & g.z[t]) & 1) << t;
+    r = q + nb < cosets ? representative(q + nb) : 0u;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const V v = load_amp<SL>(st + (r ^ off[k]));
+      are[k] = v.x;
+      aim[k] = v.y;
+    }
 #pragma unroll 2
     for (int k = 0; k < PER; ++k) {
       const uint32_t e = threadIdx.x + k * kBlock;
@@ -278,6 +303,69 @@ __global__ void __launch_bounds__(kBlock)
     double sum = 0;
 #pragma unroll
     for (int w = 0; w < kBlock / 64; ++w) sum += part[w][threadIdx.x];
+    partial[((size_t)b * call_terms + g.slot[threadIdx.x]) * nb + blockIdx.x] = sum;
+  }
+}
+
+// Groups of diagonal strings (X = 0, only I and Z): <psi|P|psi> = sum_j (-1)^popcount(j & Z)
+// |psi[j]|^2 needs no partner amplitude, so no LDS tile. Workgroups of kDiagBlock threads (no
+// LDS, so the compute units hold more of them and more loads stay in flight) step through
+// chunks of kDiagBlock x kDiagLoads amplitudes: amplitude j = chunk base + u * kDiagBlock + tid.
+// The sign splits over those disjoint bit fields: parity(tid & Z) per thread, applied at the
+// end; parity(u * kDiagBlock & Z) and parity(base & Z), wavefront-uniform.
+constexpr int kDiagBlock = 1024;
+constexpr int kDiagLoads = 4;
+static_assert(kDiagBlock * kDiagLoads == 1 << kTileBits, "a chunk is one coset tile");
+
+template <typename R, int NT, bool SL>
+__global__ void __launch_bounds__(kDiagBlock)
+    k_pauli_diag(const typename Cx<R>::V* __restrict__ states, uint32_t nbits, uint32_t nb,
+                 int call_terms, PauliGroup g, double* __restrict__ partial, uint32_t batch_base) {
+  using V = typename Cx<R>::V;
+  const uint32_t b = batch_base + blockIdx.y;
+  const V* __restrict__ st = states + ((size_t)b << nbits);
+  const uint64_t chunks = 1ull << (nbits - kTileBits);
+  uint32_t tpar = 0, upar[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    tpar |= (uint32_t)(__popcll(threadIdx.x & g.z[t]) & 1) << t;
+    upar[t] = 0;
+#pragma unroll
+    for (int u = 0; u < kDiagLoads; ++u)
+      upar[t] |= (uint32_t)(__popcll((uint64_t)(u * kDiagBlock) & g.z[t]) & 1) << u;
+  }
+  double acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = 0;
+  for (uint64_t c = blockIdx.x; c < chunks; c += nb) {
+    const uint64_t base = c << kTileBits;
+    V a[kDiagLoads];
+#pragma unroll
+    for (int u = 0; u < kDiagLoads; ++u) a[u] = load_amp<SL>(st + base + u * kDiagBlock + threadIdx.x);
+    uint32_t cpar = 0;  // parity(base & Z) of each string
+#pragma unroll
+    for (int t = 0; t < NT; ++t) cpar |= (uint32_t)(__popcll(base & g.z[t]) & 1) << t;
+#pragma unroll
+    for (int u = 0; u < kDiagLoads; ++u) {
+      const double p = (double)a[u].x * a[u].x + (double)a[u].y * a[u].y;
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[t] = fma((((upar[t] >> u) ^ (cpar >> t)) & 1u) ? -1.0 : 1.0, p, acc[t]);
+    }
+  }
+  // workgroup sums in a fixed order; string t's partial goes to its own row
+  __shared__ double part[kDiagBlock / 64][NT];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const double sum = wave_sum((tpar >> t) & 1u ? -acc[t] : acc[t]);
+    if (lane == 0) part[wave][t] = sum;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < g.nterms) {
+    double sum = 0;
+#pragma unroll
+    for (int w = 0; w < kDiagBlock / 64; ++w) sum += part[w][threadIdx.x];
     partial[((size_t)b * call_terms + g.slot[threadIdx.x]) * nb + blockIdx.x] = sum;
   }
 }
@@ -349,27 +437,41 @@ static int load_terms(int nterms, const unsigned long long* xmask, const unsigne
   return LSIM_OK;
 }
 
-template <typename R>
+template <typename R, bool SL>
 static void launch_group(int m, dim3 grid, hipStream_t s, const typename Cx<R>::V* states,
                          uint32_t nbits, uint32_t nb, int call_terms, const PauliGroup& g,
                          double* partial, uint32_t b0) {
-  if (m <= 1)
-    k_pauli_group<R, 1><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
-  else if (m <= 2)
-    k_pauli_group<R, 2><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
-  else if (m <= 4)
-    k_pauli_group<R, 4><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
-  else if (m <= 8)
-    k_pauli_group<R, 8><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
-  else
-    k_pauli_group<R, 16><<<grid, kBlock, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0);
+  const int nt = m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : 16;
+#define LSIM_PAULI_LAUNCH(KERNEL, NT, BLOCK) \
+  KERNEL<R, NT, SL><<<grid, BLOCK, 0, s>>>(states, nbits, nb, call_terms, g, partial, b0)
+  if (g.diagonal) {
+    switch (nt) {
+      case 1: LSIM_PAULI_LAUNCH(k_pauli_diag, 1, kDiagBlock); break;
+      case 2: LSIM_PAULI_LAUNCH(k_pauli_diag, 2, kDiagBlock); break;
+      case 4: LSIM_PAULI_LAUNCH(k_pauli_diag, 4, kDiagBlock); break;
+      case 8: LSIM_PAULI_LAUNCH(k_pauli_diag, 8, kDiagBlock); break;
+      default: LSIM_PAULI_LAUNCH(k_pauli_diag, 16, kDiagBlock); break;
+    }
+  } else {
+    switch (nt) {
+      case 1: LSIM_PAULI_LAUNCH(k_pauli_group, 1, kBlock); break;
+      case 2: LSIM_PAULI_LAUNCH(k_pauli_group, 2, kBlock); break;
+      case 4: LSIM_PAULI_LAUNCH(k_pauli_group, 4, kBlock); break;
+      case 8: LSIM_PAULI_LAUNCH(k_pauli_group, 8, kBlock); break;
+      default: LSIM_PAULI_LAUNCH(k_pauli_group, 16, kBlock); break;
+    }
+  }
+#undef LSIM_PAULI_LAUNCH
 }
 
 // Workgroups per state, i.e. partials per (state, string), of the expectation path for nbits.
 static uint32_t pauli_sv_blocks(int nbits) {
   if (nbits < kTileBits) return reduce_blocks(1ull << (nbits - 1));
+  // two 64 KiB tiles fit a compute unit's LDS: 512 workgroups are one round on 256 compute units,
+  // and each steps through its cosets with the next one's loads in flight (26 qubits, 8 strings
+  // in one group: 334 us with 4096 workgroups of 4 cosets, 269-288 us with 512)
   const uint64_t cosets = 1ull << (nbits - kTileBits);
-  return (uint32_t)(cosets < 4096 ? cosets : 4096);
+  return (uint32_t)(cosets < 512 ? cosets : 512);
 }
 
 static size_t pauli_sv_workspace_bytes(int nbits, long long batch, int nterms) {
@@ -396,20 +498,32 @@ static int rank_above(const uint64_t* xs, int count, int low) {
   return rank;
 }
 
-// Splits the call's strings into groups, in order: a string joins the current group while the
-// group has fewer than kGroupTerms strings and its X masks keep a span of at most kGroupDims
-// dimensions above bit 4 (runs of at least 16 contiguous amplitudes). Returns the number of
-// groups, or -1 if a group breaks an invariant the kernel's indexing relies on (low + dims ==
-// kTileBits, every tile mask below 2^kTileBits, every address bit below nbits).
+// Splits the call's strings into groups: the diagonal strings (X = 0) first, in call order, in
+// groups of up to kGroupTerms; then the others, in call order: a string joins the current group
+// while the group has fewer than kGroupTerms strings and its X masks keep a span of at most
+// kGroupDims dimensions above bit 4 (runs of at least 16 contiguous amplitudes). Each string's
+// value lands in its call position (slot). Returns the number of groups, or -1 if a group breaks
+// an invariant the kernel's indexing relies on (low + dims == kTileBits, every tile mask below
+// 2^kTileBits, every address bit below nbits).
 static int make_groups(const PauliTerms& terms, int nterms, int nbits, PauliGroup* groups) {
+  uint64_t xo[LSIM_MAX_TERMS], zo[LSIM_MAX_TERMS];
+  int io[LSIM_MAX_TERMS], n = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < nterms; ++i)
+      if ((terms.x[i] == 0) == (pass == 0)) {
+        xo[n] = terms.x[i];
+        zo[n] = terms.z[i];
+        io[n++] = i;
+      }
   int count = 0, first = 0;
   while (first < nterms) {
+    const bool diagonal = xo[first] == 0;
     int last = first + 1;  // [first, last) is the group
-    while (last < nterms && last - first < kGroupTerms &&
-           rank_above(terms.x + first, last + 1 - first, 4) <= kGroupDims)
+    while (last < nterms && last - first < kGroupTerms && (xo[last] == 0) == diagonal &&
+           rank_above(xo + first, last + 1 - first, 4) <= kGroupDims)
       ++last;
     const int m = last - first;
-    const uint64_t* xs = terms.x + first;
+    const uint64_t* xs = xo + first;
     // the widest low part that keeps the tile at kTileBits: low + rank_above(low) == kTileBits
     int low = 4;
     for (int l = 4; l <= kTileBits; ++l)
@@ -418,6 +532,7 @@ static int make_groups(const PauliTerms& terms, int nterms, int nbits, PauliGrou
     memset(&g, 0, sizeof g);
     g.nterms = m;
     g.low = low;
+    g.diagonal = diagonal ? 1 : 0;
     // reduced basis of the masks with their low bits cleared: distinct pivots, each pivot bit in
     // its own vector only
     uint64_t basis[64] = {};
@@ -442,7 +557,7 @@ static int make_groups(const PauliTerms& terms, int nterms, int nbits, PauliGrou
     for (int bit = 0; bit < low; ++bit) g.zeros[nz++] = (uint8_t)bit;
     for (int i = 0; i < g.dims; ++i) g.zeros[nz++] = (uint8_t)pivots[i];
     for (int i = 0; i < m; ++i) {
-      const uint64_t x = xs[i], z = terms.z[first + i];
+      const uint64_t x = xs[i], z = zo[first + i];
       uint32_t c = 0;
       uint64_t rest = x;
       for (int d = 0; d < g.dims; ++d)
@@ -461,7 +576,7 @@ static int make_groups(const PauliTerms& terms, int nterms, int nbits, PauliGrou
         g.kpar[i] |= (uint32_t)(__builtin_popcount(tz & (uint32_t)(k * kBlock)) & 1) << k;
       g.wodd |= (uint32_t)(w & 1) << i;
       g.wneg |= (uint32_t)(((w + 1) >> 1) & 1) << i;
-      g.slot[i] = first + i;
+      g.slot[i] = io[first + i];
     }
     if (g.low + g.dims != kTileBits) return -1;
     for (int d = 0; d < g.dims; ++d)
@@ -609,17 +724,26 @@ int launch_pauli_sv(const void* states, int dtype, int nbits, long long batch, i
     PauliGroup groups[LSIM_MAX_TERMS];
     const int ng = make_groups(terms, nterms, nbits, groups);
     if (ng < 0) return fail(LSIM_EINVAL, "Pauli strings: grouping failed its own checks");
+    // streaming loads where the dense passes stream (measured on a 1 GiB state: one diagonal
+    // string 176 -> 158 us with them)
+    const bool sl = dense_streams(((uint64_t)batch << nbits) * (dtype == LSIM_C128 ? 16 : 8));
     for (long long b0 = 0; b0 < batch; b0 += kMaxGridY) {
       const uint32_t gy = (uint32_t)(batch - b0 < kMaxGridY ? batch - b0 : kMaxGridY);
       for (int i = 0; i < ng; ++i) {
         const dim3 grid(nb, gy);
         const int m = groups[i].nterms;
-        if (dtype == LSIM_C128)
-          launch_group<double>(m, grid, s, (const double2*)states, nbits, nb, nterms, groups[i],
-                               workspace, (uint32_t)b0);
+        if (dtype == LSIM_C128 && sl)
+          launch_group<double, true>(m, grid, s, (const double2*)states, nbits, nb, nterms,
+                                     groups[i], workspace, (uint32_t)b0);
+        else if (dtype == LSIM_C128)
+          launch_group<double, false>(m, grid, s, (const double2*)states, nbits, nb, nterms,
+                                      groups[i], workspace, (uint32_t)b0);
+        else if (sl)
+          launch_group<float, true>(m, grid, s, (const float2*)states, nbits, nb, nterms,
+                                    groups[i], workspace, (uint32_t)b0);
         else
-          launch_group<float>(m, grid, s, (const float2*)states, nbits, nb, nterms, groups[i],
-                              workspace, (uint32_t)b0);
+          launch_group<float, false>(m, grid, s, (const float2*)states, nbits, nb, nterms,
+                                     groups[i], workspace, (uint32_t)b0);
       }
     }
     sum_rows(workspace, nb, (uint64_t)batch * nterms, out, s);

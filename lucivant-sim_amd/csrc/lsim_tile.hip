@@ -50,37 +50,6 @@ struct TileProgram {
   int nsegs;
 };
 
-// A 16-byte load or store, with or without the streaming (non-temporal) hint.
-typedef double d2v __attribute__((ext_vector_type(2)));
-typedef float f2v __attribute__((ext_vector_type(2)));
-
-template <bool NT>
-__device__ __forceinline__ double2 load_amp(const double2* p) {
-  d2v v = NT ? __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p)) : *reinterpret_cast<const d2v*>(p);
-  return make_double2(v.x, v.y);
-}
-template <bool NT>
-__device__ __forceinline__ float2 load_amp(const float2* p) {
-  f2v v = NT ? __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p)) : *reinterpret_cast<const f2v*>(p);
-  return make_float2(v.x, v.y);
-}
-template <bool NT>
-__device__ __forceinline__ void store_amp(double2* p, double2 a) {
-  d2v v = {a.x, a.y};
-  if (NT)
-    __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(p));
-  else
-    *reinterpret_cast<d2v*>(p) = v;
-}
-template <bool NT>
-__device__ __forceinline__ void store_amp(float2* p, float2 a) {
-  f2v v = {a.x, a.y};
-  if (NT)
-    __builtin_nontemporal_store(v, reinterpret_cast<f2v*>(p));
-  else
-    *reinterpret_cast<f2v*>(p) = v;
-}
-
 // acc + u * a in four fused multiply-adds
 template <typename R, typename V>
 __device__ __forceinline__ V mac(V acc, double2 u, V a) {

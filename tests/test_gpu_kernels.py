@@ -395,6 +395,40 @@ def _structured_labels(n: int, seed: int) -> list[str]:
 
 
 @pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
+@pytest.mark.parametrize("n", [12, 14])
+def test_diagonal_pauli_strings_interleaved_with_others(dtype: int, n: int) -> None:
+    """Strings of only I and Z are read by their own groups (no partner amplitudes), ahead of the
+    others: 20 of them (two groups) shuffled among 20 random strings, each value back in its own
+    position, for a 2-state batch, against the oracle."""
+    from lucivant_sim import workloads as wl
+
+    print(f"diagonal Pauli groups: n={n} dtype={dtype}", flush=True)
+    rng = np.random.default_rng(70 + n)
+    diag = ["I" * n, "Z" + "I" * (n - 1), "I" * (n - 1) + "Z", "Z" * n]
+    while len(diag) < 20:
+        diag.append("".join("Z" if rng.random() < 0.5 else "I" for _ in range(n)))
+    labels = diag + wl.pauli_strings(n, 20, 80 + n)
+    labels = [labels[i] for i in rng.permutation(len(labels))]
+    batch = 2
+    psi = states(n, batch, 90 + n)
+    s = dev(psi.astype(NPDT[dtype]).reshape(-1))
+    masks = [pauli_masks(lbl) for lbl in labels]
+    out = torch.empty(batch * len(labels), dtype=torch.float64, device="cuda")
+    ws = WORKSPACE.get(nat.load().lsim_reduce_workspace_bytes(n, batch, len(labels)))
+    nat.call(
+        "lsim_sv_pauli_expect",
+        ptr(s), dtype, n, batch, len(labels),
+        nat.ull_array([m[0] for m in masks]), nat.ull_array([m[1] for m in masks]),
+        ptr(out), ptr(ws), stream_handle(),
+    )  # fmt: skip
+    got = out.cpu().numpy().reshape(batch, -1)
+    for b in range(batch):
+        v = psi[b].astype(NPDT[dtype]).astype(np.complex128)
+        want = np.array([orc.expectation_sv(v, lbl, n) for lbl in labels])
+        assert np.abs(got[b] - want).max() < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [nat.C128, nat.C64])
 @pytest.mark.parametrize("n", [12, 13, 16])
 def test_grouped_pauli_strings_equal_the_oracle(dtype: int, n: int) -> None:
     """From 12 qubits on, the strings of a call are read in groups (one pass over the state per
