@@ -52,13 +52,45 @@ int lsim_device_info(int* cu_count, long long* hbm_bytes, char* arch, int arch_l
  * (simulator.py:267); needs no GPU. */
 int lsim_host_cumsum(const double* in, double* out, size_t n);
 
-/* HOST arrays, planning help for tile passes: for each candidate set of state bits sets[i],
- * counts[i] = how many of the pending gates (qubit masks, in program order) can run on a tile of
- * those bits: a gate runs when its qubits lie in the set and no earlier gate that cannot run
- * shares a qubit with it (the scan stops once every qubit is blocked, `full`). No GPU. */
-int lsim_host_tile_runnable(const unsigned long long* masks, long long count,
-                            const unsigned long long* sets, int nsets, unsigned long long full,
-                            int* counts);
+/* HOST pass planning (no GPU). A plan is a list of passes, each a list of operators; every
+ * operator is LSIM_PLAN_DENSE (a 2^k x 2^k complex128 matrix, row-major) or LSIM_PLAN_DIAG (its
+ * 2^k diagonal entries), on k targets listed from the highest bit down. The input is the static
+ * circuit's gates in program order: gate i acts on nq[i] (1..5) qubits qubits[5i .. 5i+nq[i]-1]
+ * (the first is its matrix's most significant bit), with its 4^nq[i] complex128 entries next in
+ * `mats` (interleaved re, im). The reference applies one NumPy sweep per gate and fuses only
+ * one-qubit runs (simulator.py:196-251); these plans fuse further (results change by rounding). */
+#define LSIM_PLAN_DENSE 0
+#define LSIM_PLAN_DIAG 1
+typedef struct lsim_plan lsim_plan;
+
+/* Fused operators of at most `width` qubits (at most `low_width`, when non-zero, for those that
+ * touch a qubit below `low_bits`): one-qubit runs multiplied, then greedy blocks, then (merge_blocks
+ * non-zero) blocks joined across others that commute with them. One operator per pass, on the
+ * state's own bits. */
+int lsim_host_plan_fused(int count, const int* nq, const int* qubits, const double* mats,
+                         int nbits, int width, int low_bits, int low_width, int merge_blocks,
+                         lsim_plan** plan);
+
+/* Tile passes for lsim_sv_apply_tile: each pass a set of `tile_bits` state bits (the `low` lowest
+ * always among them) and the operators (at most two targets, recorded as tile positions, the
+ * i-th lowest tile bit being position i) of the longest run of pending gates that can run on it
+ * whose cost stays within `budget` (dense one-qubit 1, dense two-qubit 2.5, diagonal 0.5 per
+ * target) and count within `max_ops`; a gate on more than two qubits is a pass of its own. */
+int lsim_host_plan_tiles(int count, const int* nq, const int* qubits, const double* mats,
+                         int nbits, double budget, int tile_bits, int low, int max_ops,
+                         lsim_plan** plan);
+
+/* Sizes of a plan: passes, operators, complex values. */
+int lsim_host_plan_sizes(const lsim_plan* plan, int* passes, int* ops, long long* values);
+
+/* Copies a plan out: per pass its tile bits (0: an operator pass on the state's own bits) and
+ * (first operator, operator count) in pass_ops; per operator 8 ints (kind, k, 5 targets, gates it
+ * stands for) and the offset of its values in `values` (complex entries). */
+int lsim_host_plan_read(const lsim_plan* plan, unsigned long long* tiles, int* pass_ops,
+                        int* op_info, long long* op_at, double* values);
+
+/* Releases a plan (NULL is ignored). */
+void lsim_host_plan_free(lsim_plan* plan);
 
 /* |index> in every state: replaces Statevector.__init__ / DensityMatrix.__init__
  * (engines.py:139-148, 216-225; for a density matrix pass nbits=2n, index=0). */

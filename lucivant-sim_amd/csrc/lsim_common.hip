@@ -110,7 +110,7 @@ extern "C" {
 
 const char* lsim_last_error(void) { return lsim::g_error; }
 
-int lsim_abi_version(void) { return 2; }
+int lsim_abi_version(void) { return 3; }
 
 // A plain sequential loop: the same float64 additions in the same order as NumPy's accumulate
 // (out[0] = in[0], then one addition per element), which is all bit-exact sampling needs, at
@@ -124,30 +124,6 @@ int lsim_host_cumsum(const double* in, double* out, size_t n) {
   for (size_t i = 1; i < n; ++i) {
     c += in[i];
     out[i] = c;
-  }
-  return LSIM_OK;
-}
-
-int lsim_host_tile_runnable(const unsigned long long* masks, long long count,
-                            const unsigned long long* sets, int nsets, unsigned long long full,
-                            int* counts) {
-  LSIM_REQUIRE(count >= 0 && nsets >= 0, "negative sizes");
-  LSIM_REQUIRE((count == 0 || masks != nullptr) && (nsets == 0 || (sets != nullptr && counts != nullptr)),
-               "a NULL array");
-  for (int i = 0; i < nsets; ++i) {
-    const unsigned long long s = sets[i];
-    unsigned long long blocked = 0;
-    int runs = 0;
-    for (long long g = 0; g < count; ++g) {
-      const unsigned long long m = masks[g];
-      if ((m & blocked) || (m & ~s)) {
-        blocked |= m;
-        if (blocked == full) break;
-      } else {
-        ++runs;
-      }
-    }
-    counts[i] = runs;
   }
   return LSIM_OK;
 }

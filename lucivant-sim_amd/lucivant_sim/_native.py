@@ -59,7 +59,11 @@ SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     ),
     "lsim_dm_probabilities": ([_c_void, _c_int, _c_int, _c_ll, _c_void, _c_void], _c_int),
     "lsim_sv_apply_perm": ([_c_void, _c_int, _c_int, _c_ll, _c_int, _c_int_p, _c_int_p, ctypes.c_uint, _c_void, _c_void, _c_void], _c_int),
-    "lsim_host_tile_runnable": ([_c_void, _c_ll, _c_void, _c_int, ctypes.c_ulonglong, _c_void], _c_int),
+    "lsim_host_plan_fused": ([_c_int, _c_void, _c_void, _c_void, _c_int, _c_int, _c_int, _c_int, _c_int, _c_void], _c_int),
+    "lsim_host_plan_tiles": ([_c_int, _c_void, _c_void, _c_void, _c_int, ctypes.c_double, _c_int, _c_int, _c_int, _c_void], _c_int),
+    "lsim_host_plan_sizes": ([_c_void, _c_void, _c_void, _c_void], _c_int),
+    "lsim_host_plan_read": ([_c_void, _c_void, _c_void, _c_void, _c_void, _c_void], _c_int),
+    "lsim_host_plan_free": ([_c_void], None),
     "lsim_graph_capture_begin": ([_c_void], _c_int),
     "lsim_graph_capture_end": ([_c_void, ctypes.POINTER(ctypes.c_void_p)], _c_int),
     "lsim_graph_nodes": ([_c_void], _c_ll),
@@ -87,6 +91,8 @@ class TileOp(ctypes.Structure):
 
 
 TILE_DENSE, TILE_DIAG = 0, 1
+#: Operator kinds of a host plan (LSIM_PLAN_DENSE, LSIM_PLAN_DIAG)
+PLAN_DENSE, PLAN_DIAG = 0, 1
 
 
 class NativeError(RuntimeError):

@@ -21,7 +21,7 @@ import torch
 
 from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Circuit
-from lucivant_sim.engines import ptr, require_device, stream_handle
+from lucivant_sim.engines import ptr, require_device, stream_handle, to_device
 from lucivant_sim.noise import NoiseModel
 from lucivant_sim.plan import is_diagonal
 
@@ -129,6 +129,10 @@ def compile_trajectory(circuit: Circuit, noise: NoiseModel | None) -> Program:
     if nflags > MAX_FLAGS:
         raise Unsupported(f"more than {MAX_FLAGS} if/else blocks")
     prog = Program(circuit.num_qubits, draws=draws)
+    # a noise model attaches the same channel object to every gate it matches: its tables (the
+    # cumulative weights and unitaries of a mixture, the Kraus stack) go into the program once and
+    # every op of that channel points at them
+    shared: dict[int, tuple[object, int]] = {}
     for s in steps:
         mask = value = 0
         for flag, polarity in s.guard:
@@ -143,22 +147,26 @@ def compile_trajectory(circuit: Circuit, noise: NoiseModel | None) -> Program:
         elif s.kind == "channel":
             ch = s.channel
             assert ch is not None
+            known = shared.get(id(ch))
             if ch.mixture is not None:
-                aux = len(prog.reals)
-                acc = 0.0
-                offsets = []
-                for p, unitary in ch.mixture:
-                    acc += p
-                    prog.reals.append(acc)
-                    identity = np.allclose(unitary, np.eye(unitary.shape[0]))
-                    offsets.append(-1.0 if identity else float(prog.add_matrix(unitary)))
-                prog.reals.extend(offsets)
-                prog.op(OP_MIX, s.qubits, nb=len(ch.mixture), aux=aux, draw=s.column, guard=guard)
+                if known is None:
+                    aux = len(prog.reals)
+                    acc = 0.0
+                    offsets = []
+                    for p, unitary in ch.mixture:
+                        acc += p
+                        prog.reals.append(acc)
+                        identity = np.allclose(unitary, np.eye(unitary.shape[0]))
+                        offsets.append(-1.0 if identity else float(prog.add_matrix(unitary)))
+                    prog.reals.extend(offsets)
+                    known = shared[id(ch)] = (ch, aux)
+                prog.op(OP_MIX, s.qubits, nb=len(ch.mixture), aux=known[1], draw=s.column, guard=guard)
             else:
                 if len(s.qubits) > 2:
                     raise Unsupported("Kraus channels on more than 2 qubits")
-                mat = prog.add_matrix(np.stack(ch.kraus))
-                prog.op(OP_KRAUS, s.qubits, mat=mat, nb=len(ch.kraus), draw=s.column, guard=guard)
+                if known is None:
+                    known = shared[id(ch)] = (ch, prog.add_matrix(np.stack(ch.kraus)))
+                prog.op(OP_KRAUS, s.qubits, mat=known[1], nb=len(ch.kraus), draw=s.column, guard=guard)
         elif s.kind == "measure":
             aux = -1
             if s.readout is not None:
@@ -326,7 +334,9 @@ def run_table(
         real_host = np.zeros(1)
 
     def up(a: np.ndarray) -> torch.Tensor:
-        return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        # staged through pinned memory, so the copy queues behind the stream's work instead of
+        # waiting for it
+        return to_device(a)
 
     d_ops = up(table.view(np.uint8)) if table.size else up(np.zeros(ctypes.sizeof(ProgOp), dtype=np.uint8))
     d_start, d_count = up(op_start), up(op_count)
@@ -451,9 +461,11 @@ def _validate(table: np.ndarray, n: int, mat_size: int, reals: np.ndarray, ndraw
 def _rebase_mix_offsets(prog: Program, reals: np.ndarray, mat_base: int) -> np.ndarray:
     """Shift the matrix offsets a MIX op stores among its reals (-1 stays -1)."""
     out = reals.copy()
+    done = set()  # ops of one channel share its reals: each segment moves once
     for o in prog.ops:
-        if o[0] == OP_MIX:
+        if o[0] == OP_MIX and o[8] not in done:
             aux, nb = o[8], o[7]
+            done.add(aux)
             seg = out[aux + nb : aux + 2 * nb]
             out[aux + nb : aux + 2 * nb] = np.where(seg >= 0, seg + mat_base, seg)
     return out

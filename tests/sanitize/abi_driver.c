@@ -46,6 +46,62 @@ int main(void) {
     free(in); free(out);
   }
   EXPECT(lsim_host_cumsum(NULL, NULL, 4), LSIM_EINVAL);
+  /* the host planners: bad arguments, then a 14-qubit chain of h, rz and cx planned both ways */
+  {
+    enum { G = 60 };
+    int nq[G], qb[5 * G];
+    double* m = (double*)calloc(16 * 2 * G, sizeof(double));
+    double* at = m;
+    memset(qb, 0, sizeof qb);
+    for (int i = 0; i < G; ++i) {
+      const double s = 0.7071067811865476;
+      if (i % 3 == 2) {
+        nq[i] = 2; qb[5 * i] = i % 14; qb[5 * i + 1] = (i * 5 + 1) % 14;
+        if (qb[5 * i + 1] == qb[5 * i]) qb[5 * i + 1] = (qb[5 * i] + 1) % 14;
+        double cx[32] = {1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0};
+        memcpy(at, cx, sizeof cx); at += 32;
+      } else if (i % 3 == 1) {
+        nq[i] = 1; qb[5 * i] = (i * 7) % 14;
+        double rz[8] = {0.8, -0.6, 0, 0, 0, 0, 0.8, 0.6};
+        memcpy(at, rz, sizeof rz); at += 8;
+      } else {
+        nq[i] = 1; qb[5 * i] = (i * 3) % 14;
+        double h[8] = {s, 0, s, 0, s, 0, -s, 0};
+        memcpy(at, h, sizeof h); at += 8;
+      }
+    }
+    lsim_plan* plan = NULL;
+    EXPECT(lsim_host_plan_fused(G, nq, qb, m, 14, 6, 0, 0, 1, &plan), LSIM_EINVAL);
+    EXPECT(lsim_host_plan_fused(G, nq, qb, m, 10, 4, 0, 0, 1, &plan), LSIM_EINVAL);
+    EXPECT(lsim_host_plan_fused(G, NULL, qb, m, 14, 4, 0, 0, 1, &plan), LSIM_EINVAL);
+    EXPECT(lsim_host_plan_tiles(G, nq, qb, m, 14, 24.0, 12, 11, 64, &plan), LSIM_EINVAL);
+    EXPECT(lsim_host_plan_tiles(G, nq, qb, m, 14, 24.0, 15, 4, 64, &plan), LSIM_EINVAL);
+    EXPECT(lsim_host_plan_sizes(NULL, NULL, NULL, NULL), LSIM_EINVAL);
+    for (int kind = 0; kind < 2; ++kind) {
+      plan = NULL;
+      if (kind == 0)
+        EXPECT(lsim_host_plan_fused(G, nq, qb, m, 14, 4, 3, 2, 1, &plan), LSIM_OK);
+      else
+        EXPECT(lsim_host_plan_tiles(G, nq, qb, m, 14, 6.0, 12, 4, 64, &plan), LSIM_OK);
+      if (!plan) { printf("FAIL no plan\n"); fails++; continue; }
+      int np = 0, no = 0;
+      long long nv = 0;
+      EXPECT(lsim_host_plan_sizes(plan, &np, &no, &nv), LSIM_OK);
+      unsigned long long* tiles = (unsigned long long*)malloc((size_t)(np + 1) * sizeof *tiles);
+      int* pass_ops = (int*)malloc((size_t)(2 * np + 1) * sizeof(int));
+      int* info = (int*)malloc((size_t)(8 * no + 1) * sizeof(int));
+      long long* offs = (long long*)malloc((size_t)(no + 1) * sizeof(long long));
+      double* vals = (double*)malloc((size_t)(2 * nv + 2) * sizeof(double));
+      EXPECT(lsim_host_plan_read(plan, tiles, pass_ops, info, offs, vals), LSIM_OK);
+      int gates = 0;
+      for (int i = 0; i < no; ++i) gates += info[8 * i + 7];
+      if (gates != G || np < 1) { printf("FAIL plan %d: %d passes, %d gates\n", kind, np, gates); fails++; }
+      free(tiles); free(pass_ops); free(info); free(offs); free(vals);
+      lsim_host_plan_free(plan);
+    }
+    lsim_host_plan_free(NULL);
+    free(m);
+  }
   printf("%s: %d failures\n", fails ? "FAILED" : "ok", fails);
   return fails != 0;
 }

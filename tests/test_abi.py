@@ -44,7 +44,7 @@ def test_ctypes_signatures_cover_the_header() -> None:
 
 
 def test_abi_version(lib: ctypes.CDLL) -> None:
-    assert lib.lsim_abi_version() == 2
+    assert lib.lsim_abi_version() == 3
 
 
 def _err(lib: ctypes.CDLL) -> str:

@@ -678,46 +678,66 @@ def test_permutation_passes_are_recognised() -> None:
     assert np.array_equal(Pass("dense", (1, 0), cx).payload(), np.ones(4))
 
 
-def test_tile_planning_helpers_agree_with_their_definitions() -> None:
-    """The tile planner's two shortcuts: the library's host count of runnable gates equals the
-    rule stated in Python (a gate runs when its qubits lie in the set and no earlier gate that
-    cannot run shares a qubit with it), and the structural estimate of a tile pass's operators
-    has the real fusion's operator count and never a lower cost."""
-    import numpy as np
+def _same_passes(got: list, want: list) -> None:
+    from lucivant_sim.plan import TilePass
 
+    assert len(got) == len(want)
+    for x, y in zip(got, want):
+        assert type(x) is type(y)
+        if isinstance(x, TilePass):
+            assert x.bits == y.bits and x.gates == y.gates
+            _same_passes(x.ops, y.ops)
+        else:
+            assert (x.kind, x.bits, x.gates) == (y.kind, y.bits, y.gates)
+            assert np.abs(np.asarray(x.matrix) - np.asarray(y.matrix)).max() < 1e-13
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_native_fusion_equals_the_python_steps(seed: int) -> None:
+    """lsim_host_plan_fused forms the passes that the plan module's Python steps (one-qubit runs,
+    greedy, merge, descending targets) form -- the same blocks, targets, kinds and gate counts,
+    matrices within 1e-13 -- at every width, low-qubit cap and with and without the merge, on
+    circuits with swap, iswap, a three-qubit gate and a global phase."""
+    from plan_reference import plan_statevector_reference
+
+    src = wl.random_layers(9, 8, 100 + seed, two_qubit=("cx", "cz", "swap", "iswap"), measure=True)
+    src = src.replace("c = measure q;", "gphase(0.3); ccx q[0], q[3], q[6]; c = measure q;")
+    c = ls.parse(src)
+    for width in (1, 2, 3, 4, 5):
+        for low_width in (0, 2, 3):
+            for merge in (True, False):
+                got = plan_statevector(c.operations, 9, width, 3, low_width, merge)
+                _same_passes(got, plan_statevector_reference(c.operations, 9, width, 3, low_width, merge))
+
+
+@pytest.mark.parametrize("budget", [6.0, 24.0, 48.0])
+def test_native_tile_plan_equals_the_python_planner(budget: float) -> None:
+    """lsim_host_plan_tiles chooses the tile bits, the gates and the operators that the Python
+    tile planner (counting runnable gates by its rule) chooses: 14 qubits with a wide gate and a
+    global phase, a 20-qubit ansatz and a 22-qubit random circuit."""
+    from lucivant_sim.plan import plan_tiles
+    from plan_reference import plan_tiles_reference
+
+    cases = [
+        (14, wl.random_layers(14, 10, 5, two_qubit=("cx", "cz"), measure=False) + "ccx q[0], q[7], q[13];\nh q[13];\ngphase(0.3);\ncx q[13], q[2];\n"),
+        (20, wl.hea_ansatz(20, 12, 3)),
+        (22, wl.random_layers(22, 12, 9)),
+    ]
+    for n, src in cases:
+        c = ls.parse(src)
+        _same_passes(plan_tiles(c.operations, n, budget=budget), plan_tiles_reference(c.operations, n, budget=budget))
+
+
+def test_native_planners_refuse_bad_input() -> None:
+    """Out-of-range widths, qubits and tile shapes come back as errors with the library's
+    message, not as plans."""
     from lucivant_sim import _native as nat
-    from lucivant_sim.circuit import Gate
-    from lucivant_sim.plan import _fuse_in_tile, _tile_cost, _tile_structure, is_diagonal
+    from lucivant_sim.plan import plan_tiles
 
-    rng = np.random.default_rng(3)
-    n = 20
-    masks = []
-    for _ in range(300):
-        qs = rng.choice(n, size=int(rng.integers(1, 3)), replace=False)
-        masks.append(sum(1 << int(q) for q in qs))
-    full = (1 << n) - 1
-
-    def rule(s: int) -> int:
-        blocked = count = 0
-        for m in masks:
-            if m & blocked or m & ~s:
-                blocked |= m
-                if blocked == full:
-                    break
-            else:
-                count += 1
-        return count
-
-    sets = [int(sum(1 << int(q) for q in rng.choice(n, size=12, replace=False))) for _ in range(50)]
-    out = np.zeros(len(sets), dtype=np.int32)
-    arr = np.array(masks, dtype=np.uint64)
-    cand = np.array(sets, dtype=np.uint64)
-    nat.call("lsim_host_tile_runnable", arr.ctypes.data, len(arr), cand.ctypes.data, len(sets), full, out.ctypes.data)
-    assert out.tolist() == [rule(s) for s in sets]
-    for seed in range(12):
-        c = ls.parse(wl.random_layers(14, 6, seed, two_qubit=("cx", "cz")))
-        items = [(np.asarray(op.matrix), tuple(op.qubits)) for op in c.operations if isinstance(op, Gate)]
-        sub = [it for it in items if all(q < 12 for q in it[1])][:40]
-        real = _fuse_in_tile(sub, tuple(range(12)))
-        cost, count = _tile_structure([q for _, q in sub], [is_diagonal(m) for m, _ in sub])
-        assert count == len(real) and cost >= sum(map(_tile_cost, real))
+    c = ls.parse(wl.random_layers(14, 2, 1))
+    with pytest.raises(ValueError):
+        plan_statevector(c.operations, 14, 6)
+    with pytest.raises(nat.NativeError, match="low bits"):
+        plan_tiles(c.operations, 14, low=11)
+    with pytest.raises(ValueError):
+        plan_tiles(c.operations, 14, tile_bits=15)
