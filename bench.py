@@ -619,6 +619,14 @@ def passes_roofline(passes, nbits: int) -> dict:  # type: ignore[no-untyped-def]
     }
 
 
+def diagonal_strings(n: int, count: int, seed: int) -> list[str]:
+    """``count`` random strings of only I and Z on ``n`` qubits (each letter Z with probability 1/2)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    return ["".join("Z" if z else "I" for z in rng.integers(0, 2, n)) for _ in range(count)]
+
+
 def bench_pauli(buf, labels, reps: int = 20) -> dict:  # type: ignore[no-untyped-def]
     """Pauli expectations on the resident 26-qubit state (2^26 complex128 amplitudes, 1 GiB):
     the 26-qubit configuration's 8 strings in one ``lsim_sv_pauli_expect`` call (strings whose X
@@ -646,15 +654,17 @@ def bench_pauli(buf, labels, reps: int = 20) -> dict:  # type: ignore[no-untyped
 
     t8 = timed(labels)
     t1 = timed(["Z" + "I" * (N - 1)])
+    t16 = timed(diagonal_strings(N, 16, 1004))
     return {
-        "workload": "26-qubit state resident in HBM: 8 Pauli strings (seed 1003) in one call, and one Z string",
+        "workload": "26-qubit state resident in HBM: 8 Pauli strings (seed 1003) in one call, one Z string, and 16 strings of I and Z (seed 1004) in one call",
         "strings_8_ms": round(t8 * 1e3, 4),
         "string_1_ms": round(t1 * 1e3, 4),
+        "diagonal_strings_16_ms": round(t16 * 1e3, 4),
         "read_only_GBps": round(state_bytes / t1 / 1e9, 1),
         "strings_8_over_1": round(t8 / t1, 2),
         "roofline": {
             "bound": "hbm",
-            "kernel": "lsim::k_pauli_group: one read of the state per group of strings",
+            "kernel": "lsim::k_pauli_diag (one Z string): one read of the state per group of strings",
             "achieved": round(state_bytes / t1 / 1e9, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -770,11 +780,13 @@ def profile_leg(leg: str, circuit, labels, steps: int) -> None:  # type: ignore[
             what = f"{steps} steps of {len(plan)} tile-plan passes"
         else:
             run_plan(sv.buffer, nat.C128, N, 1, plan_statevector(circuit.operations, N, 5))
+            diagonal = diagonal_strings(N, 16, 1004)
             for _ in range(steps):
                 launch_expectations("lsim_sv_pauli_expect", sv.buffer, nat.C128, N, N, labels)
                 launch_expectations("lsim_sv_pauli_expect", sv.buffer, nat.C128, N, N, ["Z" + "I" * (N - 1)])
+                launch_expectations("lsim_sv_pauli_expect", sv.buffer, nat.C128, N, N, diagonal)
                 assert lib.lsim_sv_scale(state, nat.C128, N, 1, 1.0, 0.0, hs) == 0
-            what = f"{steps} x (8 strings, 1 string, 1 k_scale)"
+            what = f"{steps} x (8 strings, 1 string, 16 diagonal strings, 1 k_scale)"
     elif leg == "density":
         src, noise = workloads.config_c3(), workloads.noise_c3()
         c = ls.parse(src)
