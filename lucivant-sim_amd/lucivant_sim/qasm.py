@@ -297,6 +297,8 @@ _QUICK = re.compile(
     rf"({_Q_OPER}(?:[ \t]*,[ \t]*{_Q_OPER})*)[ \t]*;"
 )
 _Q_OPERAND = re.compile(r"([A-Za-z_]\w*)[ \t]*\[[ \t]*([0-9]+)")
+#: decimal places of the parameters that key the matrix cache (one per parameter)
+_DIGITS = (14,) * 16
 
 _REFUSED = ("for", "while", "def", "defcal", "cal", "extern", "opaque", "return", "break", "continue")
 _CLASSICAL = ("input", "const", "float", "angle", "int", "uint", "bool")
@@ -431,11 +433,8 @@ class _Reader:
             ops: list[Operation] = []
             stream = self.s
             while True:
-                if stream.at_boundary():
-                    plain = self.plain_gate()
-                    if plain is not None:
-                        ops += plain
-                        continue
+                if stream.at_boundary() and self.plain_gates(ops):
+                    continue
                 if stream.peek()[0] == "eof":
                     return ops
                 ops += self.statement(plain_tried=True)
@@ -521,8 +520,47 @@ class _Reader:
         s.pos = m.end()
         return [Gate(canonical_name(name), tuple(qubits), self.builtin(name, params), params)]
 
+    def plain_gates(self, ops: list[Operation]) -> bool:
+        """:meth:`plain_gate` over a run of consecutive statements, appending to ``ops``; True if it
+        read at least one. (One call per run keeps the per-statement work to the match itself.)"""
+        s = self.s
+        assert s is not None
+        if not PLAIN_STATEMENTS:
+            return False
+        text, pos = s.text, s.pos
+        skip, quick, operand = _SKIP.match, _QUICK.match, _Q_OPERAND.findall
+        qregs, user, builtin, append = self.qregs, self.gates, self.builtin, ops.append
+        took = False
+        while True:
+            m = quick(text, skip(text, pos).end())
+            if m is None:
+                break
+            name, args, operands = m.group(1, 2, 3)
+            spec = GATES.get(name)
+            if spec is None or name in _HANDLERS or name in user:
+                break
+            params = tuple(map(float, args.split(","))) if args else ()
+            if len(params) != spec.params:
+                break
+            qubits = []
+            for reg_name, idx in operand(operands):
+                reg = qregs.get(reg_name)
+                i = int(idx)
+                if reg is None or i >= reg.size:
+                    qubits = None
+                    break
+                qubits.append(reg.offset + i)
+            if qubits is None or len(qubits) != spec.qubits or len(set(qubits)) != len(qubits):
+                break
+            pos = m.end()
+            append(Gate(canonical_name(name), tuple(qubits), builtin(name, params), params))
+            took = True
+        if took:
+            s.pos = pos
+        return took
+
     def builtin(self, name: str, params: tuple[float, ...]) -> Matrix:
-        key = (name, tuple(round(p, 14) for p in params))
+        key = (name, tuple(map(round, params, _DIGITS)))
         mat = self.matrices.get(key)
         if mat is None:
             mat = self.matrices[key] = gate_matrix(name, params)
