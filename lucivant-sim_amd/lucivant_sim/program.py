@@ -21,7 +21,7 @@ import torch
 
 from lucivant_sim import _native as nat
 from lucivant_sim.circuit import Circuit
-from lucivant_sim.engines import ptr, require_device, stream_handle
+from lucivant_sim.engines import ptr, require_device, stream_handle, to_device
 from lucivant_sim.noise import NoiseModel
 from lucivant_sim.plan import is_diagonal
 
@@ -334,7 +334,7 @@ def run_table(
         real_host = np.zeros(1)
 
     def up(a: np.ndarray) -> torch.Tensor:
-        return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        return to_device(a)
 
     d_ops = up(table.view(np.uint8)) if table.size else up(np.zeros(ctypes.sizeof(ProgOp), dtype=np.uint8))
     d_start, d_count = up(op_start), up(op_count)
