@@ -6,8 +6,10 @@ time. The scheduler runs in ticks; at every tick it
 1. admits waiting requests in arrival order while their HBM footprint fits the budget
    (statevector: 16 B x 2^n; trajectories: shots x 16 B x 2^n, chunked), so requests that arrive
    while others are on the GPU join as memory frees up;
-2. launches every admitted request of at most ``program.max_qubits`` qubits that hasn't run yet,
-   grouped by width, as one LDS-program launch (one workgroup per circuit or per shot);
+2. hands every admitted request of at most ``program.max_qubits`` qubits that hasn't run yet to
+   the small-batch thread, which runs them grouped by width as one LDS-program launch each (one
+   workgroup per circuit or per shot) on a stream of its own, waits for the outputs and samples
+   the counts, while the scheduler goes on launching;
 3. advances resident wide requests by up to ``chunk_passes`` fused passes each on their own
    HIP streams (states stay resident in HBM between ticks), so wide circuits overlap one another
    and small batches slip in between their chunks. At most ``max_launching`` wide requests are
@@ -97,7 +99,8 @@ class _Job:
     # outputs queued behind the last pass: (event, device tensor), read once the event completes
     finishing: tuple[torch.cuda.Event, torch.Tensor, Any] | None = None
     seq: int = 0  # admission order
-    # wide trajectory requests: their run on the trajectory worker thread
+    # wide trajectory requests: their run on the trajectory worker thread; small requests: the
+    # batch run on the small-batch thread that carries them
     future: Any = None
 
 
@@ -144,11 +147,16 @@ class Server:
         # so the scheduler keeps launching the other requests meanwhile
         self._trajectories = ThreadPoolExecutor(max_workers=1, thread_name_prefix="lsim-traj")
         self._traj_stream = torch.cuda.Stream()
+        # small batches (LDS programs: tables, draws, the launch, waiting for its outputs and
+        # sampling) run on a thread of their own too, one batch at a time on small_stream
+        self._small = ThreadPoolExecutor(max_workers=1, thread_name_prefix="lsim-small")
+        self._small_runs: list[Any] = []
         self._lock = threading.Lock()
         self._sampling: list[Any] = []
         self.ticks = 0
         self.peak_used = 0
-        #: scheduler-thread time per phase (admission, small batches, wide launches, completion)
+        #: scheduler-thread time per phase (admission, handing small batches to their thread,
+        #: wide launches, completion)
         self.host_seconds = {"admit": 0.0, "small": 0.0, "wide_launch": 0.0, "wide_outputs": 0.0, "wide_finish": 0.0, "copy_out": 0.0}
 
     # -- submission ---------------------------------------------------------------------
@@ -178,6 +186,9 @@ class Server:
         for job in list(self.resident):
             if job.future is not None:
                 job.future.exception()  # wait; the error, if any, was reported to the requester
+        for f in self._small_runs:
+            f.result()
+        self._small_runs = []
         for f in self._sampling:
             f.result()
         self._sampling = []
@@ -194,6 +205,9 @@ class Server:
                 break
             if not worked:
                 time.sleep(0.0005)
+        for f in self._small_runs:
+            f.result()
+        self._small_runs = []
         for f in self._sampling:
             f.result()
         self._sampling = []
@@ -269,10 +283,14 @@ class Server:
         self.ticks += 1
         progress = admitted
         small = [j for j in self.resident if j.state is None and (j.c.table is not None or j.c.program is not None)]
-        if small:
-            with torch.cuda.stream(self.small_stream):
-                self._run_small(small)
+        fresh = [j for j in small if j.future is None]
+        if fresh:
+            run = self._small.submit(self._run_small_on_stream, fresh)
+            for j in fresh:
+                j.future = run
+            self._small_runs.append(run)
             progress = True
+        self._small_runs = [f for f in self._small_runs if not f.done()]
         t2 = time.perf_counter()
         self.host_seconds["small"] += t2 - t1
         wide = [j for j in self.resident if j not in small]
@@ -314,6 +332,10 @@ class Server:
         work = len(job.c.passes or ()) << n
         count = job.terms is None and job.method == "statevector"
         return (0 if work < (16 << 24) else 1, -(1 << n) if count else 0, job.seq)
+
+    def _run_small_on_stream(self, jobs: list[_Job]) -> None:
+        with torch.cuda.stream(self.small_stream):
+            self._run_small(jobs)
 
     def _run_small(self, jobs: list[_Job]) -> None:
         by_key: dict[tuple, list[_Job]] = {}
