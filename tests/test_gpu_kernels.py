@@ -554,3 +554,29 @@ def test_permutations_with_phases_equal_the_oracle(k: int, dtype: int) -> None:
         want = orc.apply_sv(psi[b].astype(NPDT[dtype]).astype(np.complex128), mat, targets, n)
         assert np.abs(got[b] - want).max() < TOL[dtype], b
     assert np.array_equal(got[2], psi[2].astype(NPDT[dtype]))
+
+
+@pytest.mark.gpu
+def test_uploads_through_the_staging_ring_keep_their_contents() -> None:
+    """Uploads queued behind long device work, several times the ring's size in total and of
+    mixed sizes and dtypes, each read back equal to its host array: no staging span is reused
+    before the copy that reads it has run."""
+    from lucivant_sim import engines
+
+    print("staging ring: 400 uploads behind queued work", flush=True)
+    ring = engines._Staging(1 << 20)  # a small ring, so it wraps many times
+    device = engines.require_device()
+    rng = np.random.default_rng(11)
+    big = torch.randn(4096, 4096, dtype=torch.float64, device=device)
+    hosts, devs = [], []
+    for i in range(400):
+        if i % 50 == 0:
+            big = big @ big * 1e-3  # keeps the stream busy while uploads queue behind it
+        n = 100_000 if i % 97 == 0 else int(rng.integers(1, 12_000))  # the large ones bypass the ring
+        kind = i % 3
+        host = rng.random(n) if kind == 0 else (rng.random(n) + 1j * rng.random(n)) if kind == 1 else rng.integers(-5, 5, n).astype(np.int32)
+        hosts.append(host)
+        devs.append(ring.upload(host, device))
+    torch.cuda.synchronize()
+    for host, dev in zip(hosts, devs):
+        assert np.array_equal(dev.cpu().numpy(), host)
