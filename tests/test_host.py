@@ -728,6 +728,54 @@ def test_native_tile_plan_equals_the_python_planner(budget: float) -> None:
         _same_passes(plan_tiles(c.operations, n, budget=budget), plan_tiles_reference(c.operations, n, budget=budget))
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_native_plans_of_random_gates_equal_sequential_gates(seed: int) -> None:
+    """Random gates on 1-5 qubits (random unitaries, diagonals and permutations among them) on 12
+    qubits: the fused passes at every width and the tile passes at two budgets apply, in order,
+    to the state the gates give one by one (oracle application, within 1e-12); every gate is
+    counted once."""
+    from lucivant_sim.circuit import Gate
+    from lucivant_sim.plan import TilePass, plan_tiles
+
+    rng = np.random.default_rng(600 + seed)
+    n = 12
+    ops = []
+    for _ in range(160):
+        k = int(rng.choice([1, 1, 2, 2, 2, 3, 4, 5]))
+        qs = tuple(int(q) for q in rng.choice(n, size=k, replace=False))
+        d = 1 << k
+        kind = rng.integers(3)
+        if kind == 0:
+            m = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))[0]
+        elif kind == 1:
+            m = np.diag(np.exp(1j * rng.uniform(0, 6.3, d)))
+        else:
+            m = np.eye(d, dtype=np.complex128)[rng.permutation(d)] * np.exp(1j * rng.uniform(0, 6.3, d))[:, None]
+        ops.append(Gate("unitary", qs, np.ascontiguousarray(m, dtype=np.complex128)))
+    psi0 = np.zeros(1 << n, dtype=np.complex128)
+    psi0[0] = 1
+    want = psi0
+    for op in ops:
+        want = orc.apply_sv(want, op.matrix, op.qubits, n)
+    for width in (1, 2, 3, 4, 5):
+        passes = plan_statevector(tuple(ops), n, width)
+        assert sum(p.gates for p in passes) == len(ops)
+        assert np.abs(_run_passes_on_cpu(psi0, passes, n) - want).max() < 1e-12
+    for budget in (6.0, 48.0):
+        psi = psi0
+        gates = 0
+        for p in plan_tiles(tuple(ops), n, budget=budget):
+            gates += p.gates
+            if isinstance(p, TilePass):
+                for op in p.ops:
+                    m = np.diag(op.matrix) if op.kind == "diag" else op.matrix
+                    psi = orc.apply_sv(psi, m, tuple(p.bits[b] for b in op.bits), n)
+            else:
+                psi = _run_passes_on_cpu(psi, [p], n)
+        assert gates == len(ops)
+        assert np.abs(psi - want).max() < 1e-12
+
+
 def test_native_planners_refuse_bad_input() -> None:
     """Out-of-range widths, qubits and tile shapes come back as errors with the library's
     message, not as plans."""
