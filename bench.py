@@ -16,7 +16,7 @@ Printed as one JSON line:
   then its size-dependent slice dispatch, engines.py:45-119) applying the circuit's first gates
   at 26 qubits on one host thread, in the same unit;
 - ``reference_runs``: the reference's own recorded seconds on the 12-, 13- and 20-qubit
-  configurations beside this build's GPU seconds for the same runs.
+  configurations beside the GPU seconds of the same runs here.
 """
 
 from __future__ import annotations
@@ -100,7 +100,7 @@ def cpu_baseline(circuit, budget_s: float = 15.0) -> dict:  # type: ignore[no-un
 def reference_runs(gpu: dict) -> dict:
     """The reference's own seconds on the configurations it could run, recorded when the golden
     fixtures were written (tests/golden/runs.json, full_c3_reference.json: lucivant-sim 0.3.0 on an
-    8-CPU container, thread pools unset), beside this build's GPU seconds for the same runs."""
+    8-CPU container, thread pools unset), beside the GPU seconds of the same runs here."""
     gold = ROOT / "tests" / "golden"
     runs = json.loads((gold / "runs.json").read_text())
     c3 = json.loads((gold / "full_c3_reference.json").read_text())
@@ -285,7 +285,7 @@ SERVING_QPS = (10.0, 20.0, 30.0, 40.0, 1000.0)
 #: requests per rate
 SERVING_REQUESTS = 500
 #: requests of the saturation subset that the single-request baselines also serve
-BASELINE_REQUESTS = 64
+COMPARED_REQUESTS = 64
 #: widest count request that carries the noise model (run as trajectories)
 NOISY_MAX_QUBITS = 16
 
@@ -321,7 +321,7 @@ def serving_requests(count: int, seed: int):  # type: ignore[no-untyped-def]
 def bench_serving(seed: int = 1005) -> dict:
     """The serving stream at each rate of SERVING_QPS: SERVING_REQUESTS requests of the mix
     (serving_requests) with Poisson arrivals, through the continuous-batching server. Then the
-    first BASELINE_REQUESTS requests, all arriving at once, through the server and through the
+    first COMPARED_REQUESTS requests, all arriving at once, through the server and through the
     single-request API one at a time, both unfused (one launch per gate) and fused (the
     library's plans): circuits/s and latency of each, and the speedups of serving over both. The
     three must return the same results (counts equal, expectations within 1e-9)."""
@@ -341,7 +341,7 @@ def bench_serving(seed: int = 1005) -> dict:
     def fresh(reqs):  # type: ignore[no-untyped-def]
         return [Request(r.program, observable=r.observable, shots=r.shots, seed=r.seed, noise=r.noise, method=r.method) for r in reqs]
 
-    base = serving_requests(BASELINE_REQUESTS, seed)
+    base = serving_requests(COMPARED_REQUESTS, seed)
     at_once = [0.0] * len(base)
     served = fresh(base)
     served_stats = serve_stream(Server(), served, at_once)
@@ -861,7 +861,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(
-            "bench.py measures one MI355X: the statevector path runs one state per GPU and this build has no "
+            "bench.py measures one MI355X: the statevector path runs one state per GPU and the library has no "
             "multi-GPU sharding (run with --gpus 1)",
             file=sys.stderr,
         )

@@ -5,7 +5,7 @@ src/lucivant_sim/stabilizer.py:457-467 so ``choose_method`` picks the same metho
 fingerprint includes the method): every gate must map each Pauli string to a Pauli string with a
 real sign (checked, like the reference, on the matrix rounded to 12 decimals with a 1e-8
 tolerance, and only for gates on at most 3 qubits), and every attached noise channel must be a
-mixture of Pauli strings. The tableau engine itself is not part of this build.
+mixture of Pauli strings. The library has no tableau engine of its own.
 """
 
 from __future__ import annotations

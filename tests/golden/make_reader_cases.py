@@ -6,7 +6,7 @@ corpus of programs, valid and invalid.
 The reference is imported from its checkout in this container only, to record data: for each
 valid program its circuit digest (which covers every operation, qubit, parameter and matrix to 12
 decimals), sizes, registers and operation labels; for each invalid one the error's message, line
-and column. The reader of this build is checked against these records (tests/test_reader.py).
+and column. The library's reader is checked against these records (tests/test_reader.py).
 """
 
 from __future__ import annotations
