@@ -1,7 +1,7 @@
 // Host-side pass planning (no device code): the fusion of a gate sequence into operators of at
-// most `width` qubits, and the choice of tile bits for tile passes. Python's planner did the same
-// steps with NumPy and spent more time per gate than the device does applying it; here they run
-// in C++ and the Python layer only wraps the result.
+// most `width` qubits, and the choice of tile bits for tile passes. Planning runs here, in C++,
+// so that it costs less host time per gate than the device takes to apply the gates; the Python
+// layer only wraps the result (tests/plan_reference.py restates the same steps on NumPy).
 //
 // Fusion (the passes of a statevector plan):
 //   1. one-qubit runs: consecutive one-qubit gates on a qubit with no other gate on that qubit in
