@@ -189,6 +189,7 @@ struct PauliGroup {
   uint32_t tx[kGroupTerms];    // tile-index X mask (c_t << low | l_t)
   uint32_t tz[kGroupTerms];    // tile-index Z mask
   uint32_t kpar[kGroupTerms];  // bit k: parity of tz & (k * kBlock), the sign of entry k of a thread
+  int32_t pairbit[kGroupTerms];  // a bit b of tx >> 8 (the k bits), or -1 when tx has none there
   int32_t slot[kGroupTerms];   // the string's position in the call's output (-1: padding)
   uint32_t wodd, wneg;         // bit t: w_t odd (Re[i^w z] is +-Im z), Re[i^w z] negated (w = 1, 2)
   uint8_t zeros[kTileBits];    // ascending bit positions a coset representative leaves zero
@@ -271,21 +272,37 @@ __global__ void __launch_bounds__(kBlock)
       are[k] = v.x;
       aim[k] = v.y;
     }
-#pragma unroll 2
-    for (int k = 0; k < PER; ++k) {
-      const uint32_t e = threadIdx.x + k * kBlock;
-      const V own = tile[e];
-      const double ax = (double)own.x, ay = (double)own.y;
+    // String by string. Entries e and e ^ tx have equal terms (the term of j ^ X is the complex
+    // conjugate of the term of j inside Re[i^w ...], times (-1)^popcount(X & Z) = (-1)^w), so a
+    // string whose tx has a bit b among the k bits (8..11 of e = tid + k * kBlock) visits only the
+    // k with bit b clear and doubles the sum; the others visit every k. Own amplitude and partner
+    // come from LDS; the form of the term (Re or Im of conj(p) a, by the parity of w) is a
+    // wavefront-uniform branch per string.
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const V p = tile[(e ^ g.tx[t]) & kTileMask];
-        // Re or Im of conj(p) a, by the parity of w
-        const bool odd = (g.wodd >> t) & 1u;
-        const double u1 = odd ? ay : ax, u2 = odd ? -ax : ay;
-        const double v = (double)p.x * u1 + (double)p.y * u2;
-        const double sg = (((g.kpar[t] >> k) ^ (cpar >> t)) & 1u) ? -1.0 : 1.0;
-        acc[t] = fma(sg, v, acc[t]);
+    for (int t = 0; t < NT; ++t) {
+      if (t >= g.nterms) break;
+      const uint32_t tx = g.tx[t];
+      const V* __restrict__ pt = tile + (threadIdx.x ^ (tx & (kBlock - 1u)));  // partner's thread part
+      const V* __restrict__ ot = tile + threadIdx.x;
+      const uint32_t xh = tx >> 8;  // the partner of k is k ^ xh
+      const int pb = g.pairbit[t];
+      const uint32_t neg = g.kpar[t] ^ (((cpar >> t) & 1u) ? 0xFFFFu : 0u);  // bit k: term sign
+      const int count = pb >= 0 ? PER / 2 : PER;
+      double sum = 0.0;
+      if ((g.wodd >> t) & 1u) {
+        for (int kk = 0; kk < count; ++kk) {
+          const uint32_t k = pb >= 0 ? (((uint32_t)kk >> pb) << (pb + 1)) | ((uint32_t)kk & ((1u << pb) - 1u)) : (uint32_t)kk;
+          const V a = ot[k * kBlock], p = pt[(k ^ xh) * kBlock];
+          sum = fma(((neg >> k) & 1u) ? -1.0 : 1.0, (double)p.x * a.y - (double)p.y * a.x, sum);
+        }
+      } else {
+        for (int kk = 0; kk < count; ++kk) {
+          const uint32_t k = pb >= 0 ? (((uint32_t)kk >> pb) << (pb + 1)) | ((uint32_t)kk & ((1u << pb) - 1u)) : (uint32_t)kk;
+          const V a = ot[k * kBlock], p = pt[(k ^ xh) * kBlock];
+          sum = fma(((neg >> k) & 1u) ? -1.0 : 1.0, (double)p.x * a.x + (double)p.y * a.y, sum);
+        }
       }
+      acc[t] += pb >= 0 ? 2.0 * sum : sum;
     }
     __syncthreads();  // the tile is refilled by the next coset
   }
@@ -574,6 +591,7 @@ static int make_groups(const PauliTerms& terms, int nterms, int nbits, PauliGrou
       g.z[i] = z;
       for (int k = 0; k < (1 << kTileBits) / kBlock; ++k)
         g.kpar[i] |= (uint32_t)(__builtin_popcount(tz & (uint32_t)(k * kBlock)) & 1) << k;
+      g.pairbit[i] = (g.tx[i] >> 8) ? 31 - __builtin_clz(g.tx[i] >> 8) : -1;
       g.wodd |= (uint32_t)(w & 1) << i;
       g.wneg |= (uint32_t)(((w + 1) >> 1) & 1) << i;
       g.slot[i] = io[first + i];
