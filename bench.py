@@ -673,7 +673,7 @@ def bench_pauli(buf, labels, reps: int = 20) -> dict:  # type: ignore[no-untyped
     }
 
 
-def bench_tile_plan(circuit, steps: int, dense_ms: float, budget: float = 48.0, low: int = 4) -> dict:  # type: ignore[no-untyped-def]
+def bench_tile_plan(circuit, steps: int, dense_ms: float, budget: float = 64.0, low: int = 4) -> dict:  # type: ignore[no-untyped-def]
     """The same 26-qubit simulation planned into tile passes instead: each pass loads tiles of
     4096 amplitudes (12 state bits, the 4 lowest among them) into registers, applies a run of
     up to 64 one- and two-qubit operators to them (regrouping registers through LDS when an
@@ -748,7 +748,7 @@ def profile_leg(leg: str, circuit, labels, steps: int) -> None:  # type: ignore[
     """The kernels of one bench leg, ``steps`` times, for a profiler run (rocprofv3 names the
     kernels; each leg that sweeps a 1 GiB vector adds one k_scale pass per step, a streaming
     read and write of exactly 2^30 B, to calibrate the HBM counters):
-    tile = the 26-qubit simulation as tile passes (budget 48, 4 low bits);
+    tile = the 26-qubit simulation as tile passes (budget 64, 4 low bits);
     pauli = the 8 Pauli strings and one Z string on the resident 26-qubit state;
     density = the 13-qubit density-matrix probabilities (superoperator passes over 4^13 entries);
     trajectories = 1024 shots of the 20-qubit teleport chain with noise in one batch;
@@ -770,7 +770,7 @@ def profile_leg(leg: str, circuit, labels, steps: int) -> None:  # type: ignore[
         sv = Statevector(N)
         state = sv.buffer.data_ptr()
         if leg == "tile":
-            plan = plan_tiles(circuit.operations, N, budget=48.0, low=4)
+            plan = plan_tiles(circuit.operations, N, budget=64.0, low=4)
             keep, calls = plan_launches(sv.buffer, nat.C128, N, 1, plan, hs)
             for _ in range(steps):
                 assert lib.lsim_sv_set_basis(state, nat.C128, N, 1, 0, hs) == 0
