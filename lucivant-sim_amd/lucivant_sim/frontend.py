@@ -9,9 +9,15 @@ carries everything the launching process needs: the op table and matrix pool for
 kernel, the measured (qubit, clbit) pairs and register layout for sampling, and the digest.
 ``batch.run_many`` and ``serving.Server`` use the pool when it is running.
 
-The pool is forked, so it must be started before this process first uses the GPU (children must
-not inherit a live device context): :func:`start` returns 0 and starts nothing when the GPU is
-already in use, and callers then keep the in-process path.
+:func:`start` also forks trajectory workers: processes that run wide trajectory requests for the
+server on the GPU, each in its own interpreter. A trajectory batch's host side (choices, fused
+per-combination operators, shared-state bookkeeping) is Python; on a thread of the serving process
+it competes for one interpreter lock with the scheduler, and at saturation it was the slowest
+path through the whole stream.
+
+The pools are forked, so they must be started before this process first uses the GPU (children
+must not inherit a live device context): :func:`start` returns 0 and starts nothing when the GPU
+is already in use, and callers then keep the in-process path.
 """
 
 from __future__ import annotations
@@ -27,6 +33,8 @@ from lucivant_sim.circuit import Circuit, Register
 
 _POOL: Any = None
 _WORKERS = 0
+_TRAJ_POOL: Any = None
+_TRAJ_WORKERS = 0
 
 
 @dataclass
@@ -82,9 +90,15 @@ def default_workers() -> int:
     return max(1, min(15, cpus - 1))
 
 
-def start(workers: int | None = None) -> int:
-    """Start the worker pool; returns its size (0: not started because the GPU is in use)."""
-    global _POOL, _WORKERS
+def default_trajectory_workers() -> int:
+    """Trajectory worker count: ``LSIM_TRAJECTORY_WORKERS``, else 2."""
+    return max(0, int(os.environ.get("LSIM_TRAJECTORY_WORKERS", "2")))
+
+
+def start(workers: int | None = None, trajectory_workers: int | None = None) -> int:
+    """Start the worker pool and the trajectory workers; returns the pool's size (0: not started
+    because the GPU is in use)."""
+    global _POOL, _WORKERS, _TRAJ_POOL, _TRAJ_WORKERS
     if _POOL is not None:
         return _WORKERS
     import torch
@@ -94,17 +108,24 @@ def start(workers: int | None = None) -> int:
     if torch.cuda.is_initialized() or (nat.loaded() and torch.cuda.is_available()):
         return 0
     n = workers or default_workers()
-    _POOL = multiprocessing.get_context("fork").Pool(n)
+    ctx = multiprocessing.get_context("fork")
+    _POOL = ctx.Pool(n)
     _WORKERS = n
+    t = default_trajectory_workers() if trajectory_workers is None else trajectory_workers
+    # (counting devices does not open one)
+    if t > 0 and torch.cuda.device_count() > 0:
+        _TRAJ_POOL = ctx.Pool(t, initializer=_trajectory_init)
+        _TRAJ_WORKERS = t
     return n
 
 
 def stop() -> None:
-    global _POOL, _WORKERS
-    if _POOL is not None:
-        _POOL.close()
-        _POOL.join()
-    _POOL, _WORKERS = None, 0
+    global _POOL, _WORKERS, _TRAJ_POOL, _TRAJ_WORKERS
+    for pool in (_POOL, _TRAJ_POOL):
+        if pool is not None:
+            pool.close()
+            pool.join()
+    _POOL, _WORKERS, _TRAJ_POOL, _TRAJ_WORKERS = None, 0, None, 0
 
 
 def active() -> bool:
@@ -113,6 +134,10 @@ def active() -> bool:
 
 def workers() -> int:
     return _WORKERS
+
+
+def trajectory_workers() -> int:
+    return _TRAJ_WORKERS
 
 
 def compile_one(
@@ -233,3 +258,67 @@ def submit(*args: Any) -> Any:
     if _POOL is None:
         raise RuntimeError("the front-end pool isn't running")
     return _POOL.apply_async(_job, (args,))
+
+
+# ------------------------------------------------------------------------------- trajectory workers
+
+#: a trajectory worker's own stream (set up by its initializer)
+_WORKER_STREAM: Any = None
+#: cached device memory a trajectory worker keeps between requests (bytes); beyond it the
+#: worker returns its cache, so memory the server admitted by footprint is free again
+WORKER_CACHE_BYTES = 4 << 30
+
+
+def _trajectory_init() -> None:
+    """In a new trajectory worker: open the device and load the library, so the first request
+    does not pay for it."""
+    global _WORKER_STREAM
+    import torch
+
+    from lucivant_sim import _native as nat
+
+    try:
+        nat.load()
+        _WORKER_STREAM = torch.cuda.Stream()
+    except Exception:  # noqa: BLE001 - a pool respawns workers whose initializer raises; the
+        # first request then raises the same error to the requester instead
+        _WORKER_STREAM = None
+
+
+def _trajectory_job(args: tuple) -> dict[str, int]:
+    import torch
+
+    from lucivant_sim.trajectories import run_trajectories
+
+    circuit, noise, shots, seed, dtype = args
+    if _WORKER_STREAM is None:
+        _trajectory_init()
+    try:
+        with torch.cuda.stream(_WORKER_STREAM):
+            tally = run_trajectories(circuit, noise, shots, np.random.default_rng(seed), dtype=dtype)
+    finally:
+        if torch.cuda.memory_reserved() > WORKER_CACHE_BYTES:
+            torch.cuda.empty_cache()
+    return {circuit.format_bits(b): k for b, k in sorted(tally.items())} if circuit.num_clbits else {}
+
+
+class _Pending:
+    """An ``AsyncResult`` with the ``done()`` / ``result()`` of a concurrent future."""
+
+    def __init__(self, res: Any) -> None:
+        self._res = res
+
+    def done(self) -> bool:
+        return bool(self._res.ready())
+
+    def result(self) -> Any:
+        return self._res.get()
+
+
+def submit_trajectories(circuit: Circuit, noise: Any, shots: int, seed: Any, dtype: int) -> _Pending:
+    """A wide trajectory request's counts (formatted bit strings), computed by a trajectory
+    worker: the same ``run_trajectories`` call the serving process would make, so the counts are
+    the same."""
+    if _TRAJ_POOL is None:
+        raise RuntimeError("the trajectory workers aren't running")
+    return _Pending(_TRAJ_POOL.apply_async(_trajectory_job, ((circuit, noise, shots, seed, dtype),)))

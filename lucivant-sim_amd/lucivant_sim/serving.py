@@ -19,6 +19,11 @@ time. The scheduler runs in ticks; at every tick it
 4. completes finished requests: probabilities or expectations come back to the host, counts are
    sampled with the request's own seeded generator exactly as ``run`` samples them.
 
+Noisy count requests wider than an LDS program run as trajectory batches, whose host side is
+Python between launches: on the trajectory workers that ``frontend.start`` forks (processes of
+their own, so that work does not contend for this process's interpreter lock) when they run,
+else on a thread of this process, one request at a time.
+
 A request's result doesn't depend on what else is in flight: each circuit runs in its own
 workgroup or its own state buffer with the same kernels, so a result alone and in the mix is the
 same (tests/test_gpu_serving.py checks it). Latency is measured from arrival to result.
@@ -99,8 +104,8 @@ class _Job:
     # outputs queued behind the last pass: (event, device tensor), read once the event completes
     finishing: tuple[torch.cuda.Event, torch.Tensor, Any] | None = None
     seq: int = 0  # admission order
-    # wide trajectory requests: their run on the trajectory worker thread; small requests: the
-    # batch run on the small-batch thread that carries them
+    # wide trajectory requests: their run on a trajectory worker (process or thread); small
+    # requests: the batch run on the small-batch thread that carries them
     future: Any = None
 
 
@@ -143,8 +148,9 @@ class Server:
         # beside the scheduler, so launches continue meanwhile
         self._sampler = ThreadPoolExecutor(max_workers=4, thread_name_prefix="lsim-sample")
         # trajectory batches wider than an LDS program (host-driven: they read reduced density
-        # matrices back between launches) run one at a time on a thread and stream of their own,
-        # so the scheduler keeps launching the other requests meanwhile
+        # matrices back between launches) run on the front end's trajectory worker processes
+        # when they run, else one at a time on a thread and stream of their own, so the
+        # scheduler keeps launching the other requests meanwhile
         self._trajectories = ThreadPoolExecutor(max_workers=1, thread_name_prefix="lsim-traj")
         self._traj_stream = torch.cuda.Stream()
         # small batches (LDS programs: tables, draws, the launch, waiting for its outputs and
@@ -389,7 +395,12 @@ class Server:
         if job.method == "trajectories":
             if job.future is None:
                 job.req.stamps.setdefault("launched", time.perf_counter())
-                job.future = self._trajectories.submit(self._run_trajectories, job)
+                if frontend.trajectory_workers():
+                    # a worker process of its own (its own interpreter lock), same counts
+                    assert job.c.circuit is not None
+                    job.future = frontend.submit_trajectories(job.c.circuit, job.req.noise, job.req.shots, job.req.seed, self.dtype)
+                else:
+                    job.future = self._trajectories.submit(self._run_trajectories, job)
                 return True
             if not job.future.done():
                 return False

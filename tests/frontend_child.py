@@ -1,6 +1,7 @@
-"""Child process of tests/test_gpu_frontend.py: starts the front-end pool before touching the
-GPU, runs a mixed batch through ``run_many`` and a request stream through the server with the
-pool, then the same work in this process without it, and prints one JSON line comparing them."""
+"""Child process of tests/test_gpu_frontend.py: starts the front-end pool (and its trajectory
+workers) before touching the GPU, runs a mixed batch through ``run_many`` and a request stream
+(noisy 14- and 15-qubit count requests among them) through the server with the pool, then the
+same work in this process without it, and prints one JSON line comparing them."""
 
 from __future__ import annotations
 
@@ -66,6 +67,9 @@ def main() -> None:
             reqs.append(Request(wl.hea_ansatz(n, 3, 60 + i), observable=obs))
         else:
             reqs.append(Request(wl.random_layers(n, 4, 90 + i), shots=500, seed=i))
+    # noisy count requests wider than an LDS program: run by the trajectory worker processes
+    for i, n in enumerate([14, 15]):
+        reqs.append(Request(wl.random_layers(n, 6, 95 + i), shots=400, seed=20 + i, noise=noise, method="trajectories"))
     server = Server(streams=2, chunk_passes=8)
     for r in reqs:
         server.submit(r)
@@ -76,10 +80,13 @@ def main() -> None:
             serve_same = False
         elif r.observable is not None:
             serve_same &= abs(r.result - ls.expectation(r.program, r.observable)) < 1e-10
+        elif r.noise is not None:
+            serve_same &= r.result == ls.run(r.program, r.shots, seed=r.seed, noise=r.noise, method=r.method).counts
         else:
             serve_same &= r.result == ls.run(r.program, r.shots, seed=r.seed).counts
+    trajectory_workers = frontend.trajectory_workers()
     frontend.stop()
-    print(json.dumps({"workers": workers, "batch_same": batch_same, "noisy_same": noisy_same, "error_kept": error_kept, "serve_same": bool(serve_same)}), flush=True)
+    print(json.dumps({"workers": workers, "trajectory_workers": trajectory_workers, "batch_same": batch_same, "noisy_same": noisy_same, "error_kept": error_kept, "serve_same": bool(serve_same)}), flush=True)
 
 
 if __name__ == "__main__":
