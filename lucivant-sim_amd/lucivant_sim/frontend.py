@@ -24,6 +24,8 @@ from __future__ import annotations
 
 import multiprocessing
 import os
+from concurrent.futures import Future, ProcessPoolExecutor
+from concurrent.futures.process import BrokenProcessPool
 from dataclasses import dataclass, field
 from typing import Any
 
@@ -114,17 +116,21 @@ def start(workers: int | None = None, trajectory_workers: int | None = None) -> 
     t = default_trajectory_workers() if trajectory_workers is None else trajectory_workers
     # (counting devices does not open one)
     if t > 0 and torch.cuda.device_count() > 0:
-        _TRAJ_POOL = ctx.Pool(t, initializer=_trajectory_init)
+        _TRAJ_POOL = ProcessPoolExecutor(t, mp_context=ctx, initializer=_trajectory_init)
+        # the executor forks all its workers at the first submission: make that happen now,
+        # before this process opens the GPU
+        _TRAJ_POOL.submit(int).result()
         _TRAJ_WORKERS = t
     return n
 
 
 def stop() -> None:
     global _POOL, _WORKERS, _TRAJ_POOL, _TRAJ_WORKERS
-    for pool in (_POOL, _TRAJ_POOL):
-        if pool is not None:
-            pool.close()
-            pool.join()
+    if _POOL is not None:
+        _POOL.close()
+        _POOL.join()
+    if _TRAJ_POOL is not None:
+        _TRAJ_POOL.shutdown(wait=True)
     _POOL, _WORKERS, _TRAJ_POOL, _TRAJ_WORKERS = None, 0, None, 0
 
 
@@ -280,17 +286,16 @@ def _trajectory_init() -> None:
     try:
         nat.load()
         _WORKER_STREAM = torch.cuda.Stream()
-    except Exception:  # noqa: BLE001 - a pool respawns workers whose initializer raises; the
-        # first request then raises the same error to the requester instead
+    except Exception:  # noqa: BLE001 - an initializer that raises breaks the whole executor;
+        # the first request then raises the same error to its requester instead
         _WORKER_STREAM = None
 
 
-def _trajectory_job(args: tuple) -> dict[str, int]:
+def _trajectory_job(circuit: Circuit, noise: Any, shots: int, seed: Any, dtype: int) -> dict[str, int]:
     import torch
 
     from lucivant_sim.trajectories import run_trajectories
 
-    circuit, noise, shots, seed, dtype = args
     if _WORKER_STREAM is None:
         _trajectory_init()
     try:
@@ -302,23 +307,17 @@ def _trajectory_job(args: tuple) -> dict[str, int]:
     return {circuit.format_bits(b): k for b, k in sorted(tally.items())} if circuit.num_clbits else {}
 
 
-class _Pending:
-    """An ``AsyncResult`` with the ``done()`` / ``result()`` of a concurrent future."""
-
-    def __init__(self, res: Any) -> None:
-        self._res = res
-
-    def done(self) -> bool:
-        return bool(self._res.ready())
-
-    def result(self) -> Any:
-        return self._res.get()
-
-
-def submit_trajectories(circuit: Circuit, noise: Any, shots: int, seed: Any, dtype: int) -> _Pending:
+def submit_trajectories(circuit: Circuit, noise: Any, shots: int, seed: Any, dtype: int) -> Future:
     """A wide trajectory request's counts (formatted bit strings), computed by a trajectory
     worker: the same ``run_trajectories`` call the serving process would make, so the counts are
-    the same."""
-    if _TRAJ_POOL is None:
+    the same. If a worker has died, the requests it held fail with ``BrokenProcessPool``, and this
+    raises it too and reports no trajectory workers from then on (callers then run trajectories
+    themselves)."""
+    global _TRAJ_WORKERS
+    if _TRAJ_POOL is None or _TRAJ_WORKERS == 0:
         raise RuntimeError("the trajectory workers aren't running")
-    return _Pending(_TRAJ_POOL.apply_async(_trajectory_job, ((circuit, noise, shots, seed, dtype),)))
+    try:
+        return _TRAJ_POOL.submit(_trajectory_job, circuit, noise, shots, seed, dtype)
+    except BrokenProcessPool:
+        _TRAJ_WORKERS = 0
+        raise

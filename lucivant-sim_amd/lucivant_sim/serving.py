@@ -398,8 +398,11 @@ class Server:
                 if frontend.trajectory_workers():
                     # a worker process of its own (its own interpreter lock), same counts
                     assert job.c.circuit is not None
-                    job.future = frontend.submit_trajectories(job.c.circuit, job.req.noise, job.req.shots, job.req.seed, self.dtype)
-                else:
+                    try:
+                        job.future = frontend.submit_trajectories(job.c.circuit, job.req.noise, job.req.shots, job.req.seed, self.dtype)
+                    except Exception:  # noqa: BLE001 - a worker died: run it on this process's thread
+                        job.future = None
+                if job.future is None:
                     job.future = self._trajectories.submit(self._run_trajectories, job)
                 return True
             if not job.future.done():

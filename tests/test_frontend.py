@@ -70,3 +70,27 @@ def test_async_submit(pool) -> None:  # type: ignore[no-untyped-def]
     src = wl.random_layers(5, 2, 9)
     c = frontend.submit(src, None, None, "automatic", 13).get(timeout=60)
     assert c.digest == ls.parse(src).digest()
+
+
+def test_trajectory_workers_that_died_are_reported_and_dropped(monkeypatch) -> None:  # type: ignore[no-untyped-def]
+    """A trajectory worker that dies breaks its executor: the request it held fails with
+    BrokenProcessPool, the next submission raises it too, and from then on no trajectory workers
+    are reported (the server then runs trajectory requests on its own thread)."""
+    import multiprocessing
+    import os
+    from concurrent.futures import ProcessPoolExecutor
+    from concurrent.futures.process import BrokenProcessPool
+
+    ex = ProcessPoolExecutor(1, mp_context=multiprocessing.get_context("fork"))
+    try:
+        held = ex.submit(os._exit, 3)
+        with pytest.raises(BrokenProcessPool):
+            held.result(timeout=60)
+        monkeypatch.setattr(frontend, "_TRAJ_POOL", ex)
+        monkeypatch.setattr(frontend, "_TRAJ_WORKERS", 1)
+        assert frontend.trajectory_workers() == 1
+        with pytest.raises(BrokenProcessPool):
+            frontend.submit_trajectories(ls.parse(wl.random_layers(3, 1, 1)), None, 10, 1, 0)
+        assert frontend.trajectory_workers() == 0
+    finally:
+        ex.shutdown()
