@@ -53,7 +53,7 @@ from lucivant_sim.simulator import FUSION_WIDTH, _sample, _sample_many, _tile_ch
 _ids = itertools.count()
 
 
-@dataclass
+@dataclass(eq=False)  # identity: the scheduler's membership tests must not compare fields
 class Request:
     """One unit of work. ``observable`` set: an expectation; otherwise counts of ``shots``."""
 
@@ -87,7 +87,7 @@ class Request:
         return self.done - self.arrival
 
 
-@dataclass
+@dataclass(eq=False)
 class _Job:
     req: Request
     c: Compiled
@@ -288,7 +288,10 @@ class Server:
             return admitted
         self.ticks += 1
         progress = admitted
-        small = [j for j in self.resident if j.state is None and (j.c.table is not None or j.c.program is not None)]
+        small: list[_Job] = []
+        wide: list[_Job] = []
+        for j in self.resident:
+            (small if j.state is None and (j.c.table is not None or j.c.program is not None) else wide).append(j)
         fresh = [j for j in small if j.future is None]
         if fresh:
             run = self._small.submit(self._run_small_on_stream, fresh)
@@ -299,7 +302,6 @@ class Server:
         self._small_runs = [f for f in self._small_runs if not f.done()]
         t2 = time.perf_counter()
         self.host_seconds["small"] += t2 - t1
-        wide = [j for j in self.resident if j not in small]
         launching = sum(1 for j in wide if j.state is not None and j.passes is not None and j.next_pass < len(j.passes))
         for job in sorted(wide, key=self._priority):
             if job.finishing is not None:
