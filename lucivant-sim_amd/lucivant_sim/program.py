@@ -440,11 +440,14 @@ def _validate(table: np.ndarray, n: int, mat_size: int, reals: np.ndarray, ndraw
     mix = kind == OP_MIX
     if np.any(mix & ((table["aux"] < 0) | (table["aux"] + 2 * table["nb"] > real_size))):
         raise ValueError("a mixture's probabilities lie outside the reals")
-    for row in np.flatnonzero(mix):
-        aux, nb, dd = int(table["aux"][row]), int(table["nb"][row]), int(d[row]) ** 2
-        offs = reals[aux + nb : aux + 2 * nb]
-        if np.any((offs >= 0) & (offs + dd > mat_size)) or np.any((offs < 0) & (offs != -1)):
-            raise ValueError("a mixture branch lies outside the matrix pool")
+    rows = np.flatnonzero(mix)
+    if rows.size:
+        # the ops of one channel share its reals: check each distinct (aux, nb, 4^k) once
+        trip = np.stack([table["aux"][rows].astype(np.int64), table["nb"][rows].astype(np.int64), d[rows] * d[rows]], axis=1)
+        for aux, nb, dd in np.unique(trip, axis=0).tolist():
+            offs = reals[aux + nb : aux + 2 * nb]
+            if np.any((offs >= 0) & (offs + dd > mat_size)) or np.any((offs < 0) & (offs != -1)):
+                raise ValueError("a mixture branch lies outside the matrix pool")
     meas = kind == OP_MEASURE
     if np.any(meas & (table["aux"] >= 0) & (table["aux"] + 2 > real_size)):
         raise ValueError("a readout error lies outside the reals")

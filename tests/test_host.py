@@ -331,6 +331,17 @@ def test_program_tables_are_bounds_checked_before_launch() -> None:
             pg._validate(bad, 9, prog.mat_size, reals, prog.draws)
     with pytest.raises(ValueError, match="draw"):
         pg._validate(table, 9, prog.mat_size, reals, 0)
+    # a mixture's branch offsets (kept among the reals, shared by the ops of one channel): each
+    # must be -1 or a whole matrix inside the pool
+    mixes = np.flatnonzero(table["kind"] == pg.OP_MIX)
+    assert mixes.size
+    for row in (mixes[0], mixes[-1]):
+        aux, nb = int(table["aux"][row]), int(table["nb"][row])
+        for off in (prog.mat_size, -2):
+            bad_reals = reals.copy()
+            bad_reals[aux + nb + nb - 1] = off
+            with pytest.raises(ValueError, match="mixture branch"):
+                pg._validate(table, 9, prog.mat_size, bad_reals, prog.draws)
 
 
 def test_large_distributions_sample_as_the_oracle_with_a_reused_buffer() -> None:
