@@ -1,7 +1,8 @@
 """Child process of tests/test_gpu_frontend.py: starts the front-end pool (and its trajectory
 workers) before touching the GPU, runs a mixed batch through ``run_many`` and a request stream
-(noisy 14- and 15-qubit count requests among them) through the server with the pool, then the
-same work in this process without it, and prints one JSON line comparing them."""
+(noisy count requests of 14 and 15 qubits among them, one of them a dynamic circuit) through the
+server with the pool, then the same work in this process without it, and prints one JSON line
+comparing them."""
 
 from __future__ import annotations
 
@@ -68,8 +69,10 @@ def main() -> None:
         else:
             reqs.append(Request(wl.random_layers(n, 4, 90 + i), shots=500, seed=i))
     # noisy count requests wider than an LDS program: run by the trajectory worker processes
+    # (one of them with mid-circuit measurements and conditioned corrections)
     for i, n in enumerate([14, 15]):
         reqs.append(Request(wl.random_layers(n, 6, 95 + i), shots=400, seed=20 + i, noise=noise, method="trajectories"))
+    reqs.append(Request(wl.teleport_chain(14, 3), shots=300, seed=31, noise=noise, method="trajectories"))
     server = Server(streams=2, chunk_passes=8)
     for r in reqs:
         server.submit(r)
