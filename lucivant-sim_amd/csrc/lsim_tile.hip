@@ -250,6 +250,10 @@ __device__ __forceinline__ void run_program(V (&a)[kRegs], V* tile, const TilePr
       // into segment s's layout (the planner starts a segment only where the layout changes)
       // through LDS; the thread and register parts of an index have disjoint bits, so an
       // entry's element offset is their sum
+      // A wave in an exchange holds up its workgroup at the barriers, while the other
+      // workgroup on the compute unit has arithmetic to fill the gaps: the exchanging waves issue
+      // first until their reads are out (2% per pass at 26 qubits).
+      __builtin_amdgcn_s_setprio(1);
       __syncthreads();  // the previous exchange (or the previous tile's) has been read
       {
         V* mine = tile + tpart;
@@ -265,6 +269,7 @@ __device__ __forceinline__ void run_program(V (&a)[kRegs], V* tile, const TilePr
 #pragma unroll
         for (int k = 0; k < kRegs; ++k) a[k] = mine[kpart(k, P)];
       }
+      __builtin_amdgcn_s_setprio(0);
     }
     const int j1 = (int)uniform(prog.first[s + 1]);
     // two operators per iteration: the register allocator can then let the first write where the
