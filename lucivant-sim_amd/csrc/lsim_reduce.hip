@@ -254,6 +254,10 @@ __global__ void __launch_bounds__(kBlock)
     aim[k] = v.y;
   }
   for (uint64_t q = blockIdx.x; q < cosets; q += nb) {
+    // a wave staging its amplitudes holds up its workgroup at the barrier, while the other
+    // workgroup on the compute unit has strings to evaluate: it issues first until the barrier
+    // (2% per 8-string call at 26 qubits)
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       V v;
@@ -262,6 +266,7 @@ __global__ void __launch_bounds__(kBlock)
       tile[threadIdx.x + k * kBlock] = v;
     }
     __syncthreads();
+    __builtin_amdgcn_s_setprio(0);
     uint32_t cpar = 0;  // coset sign of each string, parity(r & Z) (wavefront-uniform)
 #pragma unroll
     for (int t = 0; t < NT; ++t) cpar |= (uint32_t)(__popcll(r & g.z[t]) & 1) << t;
